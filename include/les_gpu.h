@@ -1,0 +1,218 @@
+/*
+ * les_gpu.h - C ABI of the MI355X (gfx950) LES time-step engine.
+ *
+ * The reference solver (les_combustion_solver.c) has no library API: its hot
+ * path is a set of void functions over process-global Grid3D arrays, called
+ * only from main() (les_combustion_solver.c:4390-4603).  This header is the
+ * drop-in boundary a host program calls instead.  Every entry point is plain C:
+ * POD structs, pointers and sizes, int status codes, no exceptions, no exit().
+ *
+ * Field layout everywhere: row-major [i][j][k], k fastest, exactly the
+ * reference's Grid3D (les_combustion_solver.c:286), one contiguous block of
+ * nx*ny*nz values per field.
+ *
+ * Reference interfaces replaced (file:line in les_combustion_solver.c):
+ *   les_gpu_step          <- the time loop body :4390-4603 (DT :4399-4411, SEM :4414,
+ *                            RK4 :4420-4544, BCs :4555, sampling :4558-4574)
+ *   les_gpu_rhs           <- calculate_RHS :2494-2963 (one RHS evaluation)
+ *   les_gpu_upload_state  <- load_quick_checkpoint :3701-3737 (field order u,v,w,T,G,rho,Y0..Y4)
+ *   les_gpu_download_state<- save_quick_checkpoint :3671-3698 (same order)
+ *   les_gpu_init_state    <- initialize :1085-1133 + setting :1135-1285
+ *   les_gpu_probe_fetch   <- record_flame_region_data :3403-3471 (device sampling, host CSV)
+ *   les_gpu_point_fetch   <- point sampling :4558-4569 and flush_point_data :3824-3850
+ *   les_gpu_stats_fetch   <- update_time_averages :3167-3192 / calculate_resolved_reynolds_stresses :3195-3219
+ *   les_gpu_set_params    <- the #define block :11-284 (constants become runtime values)
+ *   les_mixture_load      <- load_chem_bin :324-356 / load_tran_bin :359-382
+ */
+#ifndef LES_GPU_H
+#define LES_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference exits instead; see :1299, :3858) ---- */
+#define LES_OK               0
+#define LES_ERR_HIP         -1   /* a HIP runtime call failed */
+#define LES_ERR_PARAM       -2   /* invalid argument or parameter */
+#define LES_ERR_NAN         -3   /* a non-finite value appeared in the state */
+#define LES_ERR_ADMISSION   -4   /* serving: request does not fit */
+#define LES_ERR_IO          -5   /* file could not be read or written */
+#define LES_ERR_STATE       -6   /* call not valid in the current state */
+
+/* ---- scheme selectors ---- */
+#define LES_MODE_REFERENCE   0   /* the reference's scheme, quirks included */
+#define LES_MODE_DUCT        1   /* periodic duct, SSP-RK3, table properties, level-set G */
+
+#define LES_INT_RK4          0   /* classic RK4 (:4416-4544) */
+#define LES_INT_SSPRK3       1   /* Shu-Osher SSP-RK3 */
+
+#define LES_ADV_TENO5        0   /* TENO5 for G (:2190-2298), MUSCL-superbee for T/Y (:2037-2159) */
+#define LES_ADV_WENO5        1   /* Jiang-Shu WENO5 upwind reconstruction */
+#define LES_ADV_MUSCL_MINMOD 2   /* MUSCL with the minmod limiter */
+
+#define LES_PREC_FP64        0
+#define LES_PREC_FP32        1
+
+#define LES_NSPECIES         5   /* C3H8, O2, N2, CO2, H2O (:66, :129-133) */
+#define LES_NFIELDS          11  /* u, v, w, T, G, rho, Y0..Y4 */
+#define LES_NPROBE_MAX       16  /* flame-region x-stations */
+#define LES_NPOINT_MAX       64  /* point time series (:198) */
+
+/* Per-species data as read from chem.bin / tran.bin (:292-300, :305). */
+typedef struct les_species {
+    char   name[17];
+    double mw;              /* g/mol */
+    double t_mid;           /* K */
+    double nasa_high[7];
+    double nasa_low[7];
+    double lambda_fit[4];   /* ln(lambda[erg/cm/s/K]) cubic in ln T */
+    double eta_fit[4];      /* ln(eta[poise]) cubic in ln T */
+} les_species;
+
+typedef struct les_mixture {
+    les_species sp[LES_NSPECIES];
+    double diff_fit[LES_NSPECIES][LES_NSPECIES][4];   /* ln(D_kj[cm^2/s]) */
+} les_mixture;
+
+/*
+ * Run parameters.  Names follow the reference's #defines (:46-284 and the
+ * in-function constants :2648, :2740, :2834, :4101); les_params_default()
+ * fills the values of a preset.
+ */
+typedef struct les_params {
+    /* grid and domain (:46-51) */
+    int    nx, ny, nz;
+    double lx, ly, lz;
+
+    /* scheme selection */
+    int    mode;                 /* LES_MODE_* */
+    int    integrator;           /* LES_INT_* */
+    int    advection;            /* LES_ADV_* */
+    int    precision;            /* LES_PREC_* (bulk arithmetic; statistics stay FP64) */
+
+    /* inflow / boundary (:53-63) */
+    double inlet_radius;
+    double inlet_velocity;
+    double inlet_temperature;
+    double ambient_temperature;
+    double turbulence_intensity;
+    int    num_vortices;
+    double turbulent_length_scale;
+    double inlet_noise_amp;      /* white-noise amplitude / U (:2987) */
+
+    /* combustion (:66-70, :4101) */
+    double s_l;
+    double burnt_temperature;
+    int    initial_flame_front_i;
+    double equivalence_ratio;
+
+    /* LES / turbulence (:75-78, :151-157) */
+    double pr_t, sc_t, fractal_d3;
+    double filter_weight_center;
+    double nu_unburnt_approx;
+    double rho_unburnt_approx, rho_burnt_approx;   /* SSR tau (:2701-2703) */
+
+    /* time step (:82-83) */
+    double courant, courant_diffusion;
+
+    /* flame-acceleration model and the optimiser's constants (:234-262, :2648) */
+    double flame_accel_strength, flame_accel_width, flame_accel_center;
+    double cs_unburned, cs_burnt, cs_flame_boost;
+    double tip_protection_radius, max_flame_force, flame_ramp_distance;
+    double weight_x;
+    double relaxation_coeff;     /* nudging rate (:2834) */
+    int    scheme_transition_step;  /* WENO5 |grad G| from this step (:2740) */
+
+    /* duct mode (mode LES_MODE_DUCT) */
+    double rho_unburnt;          /* kg/m^3 at T_u; rho = rho_unburnt*T_u/T */
+    double prandtl;              /* lambda = mu*cp/Pr */
+    double flame_x0;             /* initial planar flame position (m) */
+    int    reinit_interval;      /* level-set reinitialisation every n steps (0 = off) */
+    int    reinit_iters;         /* pseudo-time iterations per reinitialisation */
+    int    proj_iters;           /* projection relaxation sweeps per step (0 = off) */
+    double table_t0, table_dt;   /* cp/mu table start and spacing (K) */
+    int    table_n;
+
+    /* sampling / statistics */
+    int    num_probe_x;          /* flame-region x-stations */
+    double probe_x[LES_NPROBE_MAX];
+    int    stats_start_step;     /* FP64 accumulators from this step on (<0 = off) */
+
+    uint64_t seed;               /* SEM / noise seed */
+} les_params;
+
+/* Presets for les_params_default(). */
+#define LES_PRESET_REFERENCE   0   /* reference #define values, 60x20x40, mode R */
+#define LES_PRESET_DUCT64      1   /* 64x32x32 duct, WENO5, SSP-RK3, FP64 */
+#define LES_PRESET_DUCT128     2   /* 128^3 duct */
+#define LES_PRESET_ENSEMBLE    3   /* 64x48x48 duct member, FP32 */
+#define LES_PRESET_LARGE       4   /* 256x192x192 duct, FP32 */
+
+/* Report of one les_gpu_step() call (the last step it ran). */
+typedef struct les_step_report {
+    int    step;                 /* index of the last step run */
+    double dt;                   /* DT_sim of that step (:4411) */
+    double max_velocity;         /* (:4399) */
+    double total_mass;           /* monitor reductions (:3885-3897) */
+    double min_T, max_T, min_G, max_G, min_Y_fuel;
+    int    nonfinite;            /* 1 if a non-finite value was found */
+    double proj_residual;        /* duct mode: FP64 projection residual (L2) */
+} les_step_report;
+
+/* One flame-region probe row (the CSV line of :3453). */
+typedef struct les_probe_row {
+    int    timestep;
+    int    station;
+    double x_location, y_flame, y_physical, y_relative;
+    double u, v, w, G, nu_t, S_mag;
+} les_probe_row;
+
+typedef struct les_gpu les_gpu;   /* opaque context */
+
+/* ---- parameters and property data ---- */
+int les_params_default(les_params* p, int preset);
+int les_params_validate(const les_params* p);
+/* The engine's synthetic 5-species property set (chem.bin/tran.bin stand-in). */
+int les_mixture_synthetic(les_mixture* m);
+/* Read the reference's Fortran-record property files (:311-382). */
+int les_mixture_load(les_mixture* m, const char* chem_bin, const char* tran_bin);
+/* Write property files in the same format (the reader above reads them back). */
+int les_mixture_save(const les_mixture* m, const char* chem_bin, const char* tran_bin);
+
+/* ---- context ---- */
+int  les_gpu_create(const les_params* p, const les_mixture* mix, les_gpu** out);
+void les_gpu_destroy(les_gpu* ctx);
+const char* les_gpu_last_error(const les_gpu* ctx);
+
+/* Initial state on the device (initialize + setting + initialize_vortices). */
+int les_gpu_init_state(les_gpu* ctx);
+/* Host <-> device state in checkpoint order u,v,w,T,G,rho,Y0..Y4 (n_fields <= 11). */
+int les_gpu_upload_state(les_gpu* ctx, const double* const* fields, int n_fields);
+int les_gpu_download_state(les_gpu* ctx, double* const* fields, int n_fields);
+/* Named diagnostic fields: "nu_t", "S_mag", "nu", "alpha", "attribute" (as double). */
+int les_gpu_download_field(les_gpu* ctx, const char* name, double* out);
+
+/* Advance n_steps time steps; rep (may be NULL) describes the last one. */
+int les_gpu_step(les_gpu* ctx, int n_steps, les_step_report* rep);
+/* Testing hook: one RHS evaluation on the current state.  out holds
+ * 5 + LES_NSPECIES fields in the order drhou, drhov, drhow, dT, dG, dY0..dY4. */
+int les_gpu_rhs(les_gpu* ctx, int t_loop, double* const* out, int n_out);
+
+/* Probe rows recorded since the last fetch (up to max_rows); *n_rows = count. */
+int les_gpu_probe_fetch(les_gpu* ctx, les_probe_row* rows, int max_rows, int* n_rows);
+/* Change the optimiser-tunable constants between steps. */
+int les_gpu_set_params(les_gpu* ctx, const les_params* p);
+/* Host-visible SEM state (mode R keeps the eddies and glibc rand() stream on the host). */
+int les_gpu_get_step(const les_gpu* ctx);
+/* Average kernel time (ms) of the RHS kernel over the last les_gpu_step() call,
+ * measured with HIP events on the engine's stream; *launches = launches timed. */
+int les_gpu_kernel_timing(const les_gpu* ctx, double* rhs_ms, int* launches, double* step_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LES_GPU_H */
