@@ -1,0 +1,77 @@
+// les_engine.hpp - context of the gfx950 LES engine (host side, internal).
+#pragma once
+#define _GNU_SOURCE
+#include <hip/hip_runtime.h>
+#include <stdlib.h>
+
+#include <vector>
+
+#include "les_ref_kernels.hpp"
+
+struct DuctState;   // duct-mode device state (les_duct_kernels.hip)
+
+// Device buffers of the reference scheme.
+struct R {
+    int* attr = nullptr;
+    double *u, *v, *w, *T, *G, *rho, *rhou, *rhov, *rhow;
+    double* Y[LES_NSP];
+    double *rhou0, *rhov0, *rhow0, *T0, *G0;
+    double* Y0[LES_NSP];
+    double *ut, *vt, *wt, *Tt, *Gt, *rhot;
+    double* Yt[LES_NSP];
+    double* k[4][LES_NRHS];
+    double *uf, *vf, *wf, *Tf, *Gf, *rhof;
+    double* Yf[LES_NSP];
+    double *nu, *mu_mix, *cp, *lam, *alpha, *nu_t, *S_mag, *ST, *gradG;
+    double* Dk[LES_NSP];
+    double *inlet, *noise, *vort, *partial, *rho_avg;
+    unsigned long long* maxbits;
+    double* probe_rows;
+    int* probe_flags;
+    double* stat[10];
+};
+
+struct les_gpu {
+    les_params p;
+    les_mixture mix;
+    GridDims g;
+    hipStream_t stream = nullptr;
+    les_mixture* d_mix = nullptr;
+    std::vector<void*> allocs;
+    char err[512] = {0};
+    int step = 0;
+    // composition (:4101-4120)
+    double inletY[LES_NSP], ambY[LES_NSP], st_o2, st_co2, st_h2o;
+    // host-side glibc rand() stream (reference scheme)
+    struct random_data rd;
+    char rstate[128];
+    std::vector<double> vort;      // num_vortices x 7
+    std::vector<double> noise_h;   // 3 per inlet-plane cell
+    std::vector<int> attr_h;
+    std::vector<les_probe_row> probe;
+    int stat_count = 0;
+    // timing (HIP events on the engine's stream)
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int timing = 0;
+    double rhs_ms_sum = 0.0;
+    int rhs_launches = 0;
+    double step_ms = 0.0;
+    hipGraphExec_t graph_exec = nullptr;
+    double* mon_partial = nullptr;
+    R r;
+    DuctState* d = nullptr;
+};
+
+int monitor(les_gpu* c, const double* rho, const double* T, const double* G, const double* Y0, const double* u,
+            les_step_report* rep);
+
+// duct mode (les_duct_kernels.hip)
+int  duct_create(les_gpu* c);
+void duct_destroy(les_gpu* c);
+int  duct_init_state(les_gpu* c);
+int  duct_upload_state(les_gpu* c, const double* const* f, int nf);
+int  duct_download_state(les_gpu* c, double* const* f, int nf);
+int  duct_download_field(les_gpu* c, const char* name, double* out);
+int  duct_step(les_gpu* c, int n_steps, les_step_report* rep);
+int  duct_rhs(les_gpu* c, double* const* out, int n_out);
+void duct_params_changed(les_gpu* c);

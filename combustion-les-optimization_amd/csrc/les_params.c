@@ -353,3 +353,13 @@ int les_mixture_load(les_mixture* m, const char* chem_bin, const char* tran_bin)
     fclose(fp);
     return LES_OK;
 }
+
+int les_abi_struct_sizes(size_t out[4])
+{
+    if (!out) return LES_ERR_PARAM;
+    out[0] = sizeof(les_params);
+    out[1] = sizeof(les_mixture);
+    out[2] = sizeof(les_step_report);
+    out[3] = sizeof(les_probe_row);
+    return LES_OK;
+}

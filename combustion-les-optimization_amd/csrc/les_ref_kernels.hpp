@@ -1,0 +1,43 @@
+// les_ref_kernels.hpp - pointer bundles shared by the reference-scheme kernels
+// and the engine that launches them (all passed to kernels by value).
+#pragma once
+#include "les_device.hpp"
+
+// Inputs / outputs of one RHS evaluation (calculate_RHS, :2494-2963).
+struct RefPtrs {
+    const int* attr;
+    const les_mixture* mix;
+    // RHS input state (the step state or the RK intermediate state)
+    const double *u, *v, *w, *T, *G, *rho;
+    const double* Y[LES_NSP];
+    // filtered fields
+    double *uf, *vf, *wf, *Tf, *Gf, *rhof;
+    double* Yf[LES_NSP];
+    // properties and SGS quantities
+    double *nu, *mu_mix, *cp, *lam, *alpha, *nu_t, *S_mag, *ST, *gradG;
+    double* Dk[LES_NSP];
+    // RHS outputs: drhou, drhov, drhow, dT, dG, dY0..dY4
+    double* out[LES_NRHS];
+    const double* rho_avg;
+};
+
+// State, Phi_n, intermediate state and stage derivatives of the RK update.
+struct RefStagePtrs {
+    const int* attr;
+    double *u, *v, *w, *T, *G, *rho, *rhou, *rhov, *rhow;
+    double* Y[LES_NSP];
+    double *rhou0, *rhov0, *rhow0, *T0, *G0;
+    double* Y0[LES_NSP];
+    double *ut, *vt, *wt, *Tt, *Gt, *rhot;
+    double* Yt[LES_NSP];
+    double* k[4][LES_NRHS];
+};
+
+struct ProbeArgs {
+    int n_station;
+    int i_probe[LES_NPROBE_MAX];
+};
+
+struct StatPtrs {
+    double* acc[10];   // sums of u, v, w, G, uu, vv, ww, uv, uw, vw
+};

@@ -19,8 +19,8 @@
  *   les_gpu_download_state<- save_quick_checkpoint :3671-3698 (same order)
  *   les_gpu_init_state    <- initialize :1085-1133 + setting :1135-1285
  *   les_gpu_probe_fetch   <- record_flame_region_data :3403-3471 (device sampling, host CSV)
- *   les_gpu_point_fetch   <- point sampling :4558-4569 and flush_point_data :3824-3850
- *   les_gpu_stats_fetch   <- update_time_averages :3167-3192 / calculate_resolved_reynolds_stresses :3195-3219
+ *   les_gpu_download_field("stat0".."stat9")
+ *                         <- update_time_averages :3167-3192 accumulators (FP64)
  *   les_gpu_set_params    <- the #define block :11-284 (constants become runtime values)
  *   les_mixture_load      <- load_chem_bin :324-356 / load_tran_bin :359-382
  */
@@ -211,6 +211,13 @@ int les_gpu_get_step(const les_gpu* ctx);
 /* Average kernel time (ms) of the RHS kernel over the last les_gpu_step() call,
  * measured with HIP events on the engine's stream; *launches = launches timed. */
 int les_gpu_kernel_timing(const les_gpu* ctx, double* rhs_ms, int* launches, double* step_ms);
+/* Turn per-launch HIP-event timing of the RHS kernel on (1) or off (0). */
+int les_gpu_set_timing(les_gpu* ctx, int on);
+/* Number of steps accumulated in the FP64 statistics (:3191 time_avg_steps). */
+int les_gpu_stats_count(const les_gpu* ctx);
+/* sizeof(les_params), sizeof(les_mixture), sizeof(les_step_report),
+ * sizeof(les_probe_row): lets bindings check their struct mirrors. */
+int les_abi_struct_sizes(size_t out[4]);
 
 #ifdef __cplusplus
 }
