@@ -150,7 +150,7 @@ class LesGpu:
         return rep
 
     def rhs(self, t_loop=0):
-        out = [np.empty(self.n) for _ in range(10)]
+        out = [np.zeros(self.n) for _ in range(10)]
         self._check(lib().les_gpu_rhs(self.h, t_loop, _ptrs(out), 10), "les_gpu_rhs")
         return [o.reshape(self.shape) for o in out]
 

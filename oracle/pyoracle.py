@@ -136,7 +136,9 @@ class Oracle:
         return rep
 
     def rhs(self, t_loop=0):
-        out = [np.empty(self.n) for _ in range(10)]
+        # cells the RHS does not write (boundary layer) read as zero, as the
+        # reference's freshly allocated k-arrays do
+        out = [np.zeros(self.n) for _ in range(10)]
         assert lib().ora_rhs(self.h, t_loop, _ptrs(out), 10) == 0
         return [o.reshape(self.shape) for o in out]
 
