@@ -5,7 +5,9 @@
 //
 // One context = one run.  All device buffers live for the context's lifetime;
 // a step allocates nothing.
+#ifndef _GNU_SOURCE
 #define _GNU_SOURCE
+#endif
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdarg.h>
@@ -704,3 +706,11 @@ extern "C" int les_gpu_set_timing(les_gpu* c, int on)
 }
 
 extern "C" int les_gpu_stats_count(const les_gpu* c) { return c ? c->stat_count : -1; }
+
+extern "C" int les_gpu_point_fetch(les_gpu* c, double* out, int max_steps, int* n_steps, int* n_points)
+{
+    if (!c || !out || !n_steps || max_steps < 0) return LES_ERR_PARAM;
+    if (c->p.mode != LES_MODE_DUCT) return set_err(c, LES_ERR_STATE, "point series are recorded in duct mode");
+    if (n_points) *n_points = c->d->npts;
+    return duct_point_fetch(c, out, max_steps, n_steps);
+}

@@ -1,6 +1,8 @@
 // les_engine.hpp - context of the gfx950 LES engine (host side, internal).
 #pragma once
+#ifndef _GNU_SOURCE
 #define _GNU_SOURCE
+#endif
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 
@@ -75,3 +77,4 @@ int  duct_download_field(les_gpu* c, const char* name, double* out);
 int  duct_step(les_gpu* c, int n_steps, les_step_report* rep);
 int  duct_rhs(les_gpu* c, double* const* out, int n_out);
 void duct_params_changed(les_gpu* c);
+int  duct_point_fetch(les_gpu* c, double* out, int max_steps, int* n_steps);

@@ -204,6 +204,10 @@ int les_gpu_rhs(les_gpu* ctx, int t_loop, double* const* out, int n_out);
 
 /* Probe rows recorded since the last fetch (up to max_rows); *n_rows = count. */
 int les_gpu_probe_fetch(les_gpu* ctx, les_probe_row* rows, int max_rows, int* n_rows);
+/* Fixed-point time series (duct mode: 64 points, 4 x-stations x 4 y x 4 z)
+ * recorded since the last fetch: out[step][point][u, v, w, G], oldest first
+ * (point sampling :4558-4569; flush_point_data :3824-3850 writes them out). */
+int les_gpu_point_fetch(les_gpu* ctx, double* out, int max_steps, int* n_steps, int* n_points);
 /* Change the optimiser-tunable constants between steps. */
 int les_gpu_set_params(les_gpu* ctx, const les_params* p);
 /* Host-visible SEM state (mode R keeps the eddies and glibc rand() stream on the host). */

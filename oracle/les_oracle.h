@@ -46,6 +46,18 @@ int  ora_sem_plane(ora_ctx* c, int t_loop, double dt);
 int  ora_boundary(ora_ctx* c);
 /* glibc rand() stream helper: n draws from srand(seed) (for the RNG pin). */
 int  ora_rand_stream(unsigned seed, int n, int* out);
+/* Duct-mode operators and hooks (les_oracle_duct.c). */
+double ora_weno5_face(double v0, double v1, double v2, double v3, double v4);
+double ora_minmod(double a, double b);
+double ora_muscl_face(double v1, double v2, double v3);
+double ora_duct_table(const double* tab, double t0, double dt, int n, double T);
+double ora_duct_progress(double phi, double dx);
+double ora_duct_uniform(uint64_t seed, uint64_t a, uint64_t b, uint64_t slot);
+int  ora_duct_sem(ora_ctx* c, double dt);
+int  ora_duct_reinit(ora_ctx* c, int iters);
+int  ora_duct_set_eddies(ora_ctx* c, const double* e, int n);
+int  ora_duct_set_dt_state(ora_ctx* c, double max_nu, double max_alpha);
+int  ora_duct_profile_count(const ora_ctx* c);
 /* Statistics: 10 FP64 accumulators (u,v,w,G,uu,vv,ww,uv,uw,vw) and count. */
 int  ora_get_stats(const ora_ctx* c, double* const* acc, int* count);
 

@@ -1,0 +1,154 @@
+"""Duct-mode HIP engine against the CPU oracle.
+
+FP64 path: 1e-10 relative to each field's own scale (differences only from
+reduction order: domain-mean density, SEM plane moments).  FP32 form: error
+against the FP64 oracle reported and bounded (5e-4 relative on every state
+field after 5 steps of the 64x32x32 duct).  Graph replay: a step replayed from
+the captured hipGraph is bit-identical to the same step launched directly.
+"""
+import numpy as np
+import pytest
+
+from common import rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+DUCT = ["rhou", "rhov", "rhow", "phi", "T"]
+
+
+def make_pair(lesgpu, oracle, advection=1, precision=0, **over):
+    p = lesgpu.params(1, advection=advection, precision=precision, **over)
+    po = oracle.params(1, advection=advection, **over)
+    return lesgpu.LesGpu(p), oracle.Oracle(po)
+
+
+@pytest.mark.parametrize("advection", [1, 2], ids=["weno5", "muscl_minmod"])
+def test_duct_init_and_rhs(lesgpu, oracle, advection):
+    print("case: duct64 init + RHS, advection=%d" % advection, flush=True)
+    g, o = make_pair(lesgpu, oracle, advection)
+    try:
+        g.init_state()
+        o.init_state()
+        for nm, a, b in zip(DUCT, g.download_state(5), o.get_state(5)):
+            assert np.array_equal(a, b), nm
+        # perturb the state so every branch (both upwind signs) is exercised
+        st = o.get_state(5)
+        rng = np.random.default_rng(11)
+        rho = 1.2 * 300.0 / st[4]
+        for f in range(3):
+            st[f] = st[f] + rho * rng.uniform(-1.5, 1.5, st[f].shape)
+        st[3] = st[3] + 2e-4 * rng.standard_normal(st[3].shape)
+        g.upload_state(st)
+        o.set_state(st)
+        rg, ro = g.rhs(0)[:5], o.rhs(0)[:5]
+        for nm, a, b in zip(DUCT, rg, ro):
+            assert rel_err(a, b) <= TOL, (nm, rel_err(a, b))
+        for nm in ["nu_t", "ST", "u_f", "T_f", "phi_f"]:
+            assert rel_err(g.field(nm), o.field(nm)) <= TOL, nm
+    finally:
+        g.close()
+        o.close()
+
+
+@pytest.mark.parametrize("advection", [1, 2], ids=["weno5", "muscl_minmod"])
+def test_duct_steps_fp64(lesgpu, oracle, advection):
+    print("case: duct64 FP64 10 steps (reinit every 5), advection=%d" % advection, flush=True)
+    g, o = make_pair(lesgpu, oracle, advection, reinit_interval=5, reinit_iters=3)
+    try:
+        g.init_state()
+        o.init_state()
+        for s in range(10):
+            rg = g.step(1)
+            ro = o.step(1)
+            assert abs(rg.dt - ro.dt) <= 1e-12 * ro.dt, (s, rg.dt, ro.dt)
+        for nm, a, b in zip(DUCT, g.download_state(5), o.get_state(5)):
+            assert rel_err(a, b) <= TOL, (nm, rel_err(a, b))
+        # SEM inflow planes and eddies
+        P = g.p.ny * g.p.nz
+        ig = g.field("inlet", 3 * P)
+        io = oracle_field(o, "inlet", 3 * P)
+        assert rel_err(ig, io) <= TOL
+        # statistics: plane profiles (FP64 accumulators)
+        pg = g.field("profiles", 10 * g.p.nx)
+        po_ = oracle_field(o, "profiles", 10 * g.p.nx)
+        assert rel_err(pg, po_) <= TOL
+    finally:
+        g.close()
+        o.close()
+
+
+def oracle_field(o, name, count):
+    import ctypes as C
+    out = np.empty(count)
+    rc = o_lib().ora_get_field(o.h, name.encode(), out.ctypes.data_as(C.POINTER(C.c_double)))
+    assert rc == 0
+    return out
+
+
+def o_lib():
+    import pyoracle
+    return pyoracle.lib()
+
+
+def test_duct_fp32_error_vs_fp64(lesgpu, oracle):
+    print("case: duct64 FP32 form vs FP64 oracle, 5 steps", flush=True)
+    g, o = make_pair(lesgpu, oracle, 1, precision=1)
+    try:
+        g.init_state()
+        o.init_state()
+        g.step(5)
+        o.step(5)
+        errs = {nm: rel_err(a, b) for nm, a, b in zip(DUCT, g.download_state(5), o.get_state(5))}
+        print("FP32 rel errors after 5 steps:", errs)
+        for nm, e in errs.items():
+            assert e <= 5e-4, (nm, e)
+    finally:
+        g.close()
+        o.close()
+
+
+def test_graph_replay_bitwise(lesgpu):
+    print("case: hipGraph replay vs direct launches", flush=True)
+    p = lesgpu.params(1)
+    a = lesgpu.LesGpu(p)
+    b = lesgpu.LesGpu(p)
+    try:
+        a.init_state()
+        b.init_state()
+        b.set_timing(True)          # timing forces direct launches
+        for _ in range(3):
+            a.step(2)
+            b.step(2)
+            for x, y in zip(a.download_state(5), b.download_state(5)):
+                assert np.array_equal(x, y)
+    finally:
+        a.close()
+        b.close()
+
+
+def test_point_series(lesgpu):
+    print("case: fixed-point series", flush=True)
+    import ctypes as C
+    g = lesgpu.LesGpu(lesgpu.params(1))
+    try:
+        g.init_state()
+        g.step(7)
+        out = np.zeros(16 * 64 * 4)
+        ns, npnt = C.c_int(), C.c_int()
+        L = lesgpu.lib()
+        L.les_gpu_point_fetch.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int)]
+        assert L.les_gpu_point_fetch(g.h, out.ctypes.data_as(C.POINTER(C.c_double)), 16, C.byref(ns),
+                                     C.byref(npnt)) == 0
+        assert ns.value == 7 and npnt.value == 64
+        ser = out[: 7 * 64 * 4].reshape(7, 64, 4)
+        # last sample equals the final state at the sampled cells
+        st = g.download_state(5)
+        rho = 1.2 * 300.0 / st[4]
+        u = (st[0] / rho).reshape(-1)
+        nx, ny, nz = g.shape
+        idx = [((a + 1) * nx // 5 * ny + (2 * b + 1) * ny // 8) * nz + (2 * e + 1) * nz // 8
+               for a in range(4) for b in range(4) for e in range(4)]
+        assert np.allclose(ser[-1, :, 0], u[idx], rtol=0, atol=0)
+    finally:
+        g.close()
