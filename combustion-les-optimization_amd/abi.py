@@ -97,6 +97,7 @@ class Params(C.Structure):
         ("num_probe_x", C.c_int),
         ("probe_x", C.c_double * LES_NPROBE_MAX),
         ("stats_start_step", C.c_int),
+        ("stats_per_cell", C.c_int),
         ("seed", C.c_uint64),
     ]
 

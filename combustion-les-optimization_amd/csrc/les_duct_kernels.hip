@@ -16,6 +16,10 @@
 // device, so a whole step is captured once into a hipGraph and replayed.
 #include "les_engine.hpp"
 
+// The duct scheme is this engine's own specification: its kernels may fuse
+// multiply-adds (the reference-scheme kernels above keep strict IEEE order).
+#pragma clang fp contract(fast)
+
 #define DNF 5
 
 // ---------------------------------------------------------------------------
@@ -53,7 +57,10 @@ struct DuctBufs {
     double* inlet;     // 3 planes
     double* rho_part;  // block partial sums of rho(T_f)
     double* scal;      // [0] dt, [1] rho_avg, [2] maxvel, [3] last dt, [4] proj residual
-    unsigned long long* maxbits;   // [0] |u| max, [1] nu max, [2] alpha max
+    unsigned long long* maxbits;   // unused by the duct step (kept for upload paths)
+    double* vpart;     // per-block max |u| of the new state (RHS stage 3, then inflow plane)
+    double* dpart;     // per-block max nu, alpha: [3 stages][blocks][2]
+    double* spart;     // statistics plane partial sums: [nx][chunks][10]
     double* stat[10];  // per-cell FP64 accumulators
     double* prof;      // nx x 10 FP64 plane accumulators
     double* points;    // point time series ring: cap x npts x 4
@@ -69,6 +76,8 @@ struct DuctState {
     int point_cap = 0;
     int point_count = 0;
     int filter_blocks = 0;
+    int bc_blocks = 0;
+    int stat_chunks = 0;
     int prof_count = 0;
     size_t esize = 8;
     hipGraph_t graph = nullptr;
@@ -185,11 +194,11 @@ __global__ void __launch_bounds__(256) k_duct_filter(GridDims g, DuctConsts k, c
     double rsum = 0.0;
     if (kk < g.nz && j < g.ny) {
         const long long q = lidx(g, i, j, kk);
-        const real rut = (real)k.rho_u_Tu;
+        const real rut = (real)k.rho_u_Tu, irut = (real)(1.0 / k.rho_u_Tu);
         real fu, fv, fw, fT, fp;
         if (i == 0 || i == g.nx - 1) {
-            real T = q4[q], rho = rut / T;
-            fu = q0[q] / rho; fv = q1[q] / rho; fw = q2[q] / rho; fT = T; fp = q3[q];
+            real T = q4[q], ri = T * irut;          // 1/rho
+            fu = q0[q] * ri; fv = q1[q] * ri; fw = q2[q] * ri; fT = T; fp = q3[q];
         } else {
             // neighbour order (-x, -y, -z, centre, +z, +y, +x) as :1329-1331
             long long r[7];
@@ -203,19 +212,20 @@ __global__ void __launch_bounds__(256) k_duct_filter(GridDims g, DuctConsts k, c
             real pu[7], pv[7], pw[7], pT[7], pp[7];
 #pragma unroll
             for (int t = 0; t < 7; ++t) {
-                real T = q4[r[t]], rho = rut / T;
-                pu[t] = q0[r[t]] / rho; pv[t] = q1[r[t]] / rho; pw[t] = q2[r[t]] / rho;
+                real T = q4[r[t]], ri = T * irut;
+                pu[t] = q0[r[t]] * ri; pv[t] = q1[r[t]] * ri; pw[t] = q2[r[t]] * ri;
                 pT[t] = T; pp[t] = q3[r[t]];
             }
-            const real wc = (real)k.wc, den = (real)6.0 + wc;
-            fu = (pu[0] + pu[1] + pu[2] + wc * pu[3] + pu[4] + pu[5] + pu[6]) / den;
-            fv = (pv[0] + pv[1] + pv[2] + wc * pv[3] + pv[4] + pv[5] + pv[6]) / den;
-            fw = (pw[0] + pw[1] + pw[2] + wc * pw[3] + pw[4] + pw[5] + pw[6]) / den;
-            fT = (pT[0] + pT[1] + pT[2] + wc * pT[3] + pT[4] + pT[5] + pT[6]) / den;
-            fp = (pp[0] + pp[1] + pp[2] + wc * pp[3] + pp[4] + pp[5] + pp[6]) / den;
+            const real wc = (real)k.wc, iden = (real)(1.0 / (6.0 + k.wc));
+            fu = (pu[0] + pu[1] + pu[2] + wc * pu[3] + pu[4] + pu[5] + pu[6]) * iden;
+            fv = (pv[0] + pv[1] + pv[2] + wc * pv[3] + pv[4] + pv[5] + pv[6]) * iden;
+            fw = (pw[0] + pw[1] + pw[2] + wc * pw[3] + pw[4] + pw[5] + pw[6]) * iden;
+            fT = (pT[0] + pT[1] + pT[2] + wc * pT[3] + pT[4] + pT[5] + pT[6]) * iden;
+            fp = (pp[0] + pp[1] + pp[2] + wc * pp[3] + pp[4] + pp[5] + pp[6]) * iden;
         }
         uf[q] = fu; vf[q] = fv; wf[q] = fw; Tf[q] = fT; pf[q] = fp;
         rsum = (double)(rut / fT);
+        (void)irut;
     }
     red[tid] = rsum;
     __syncthreads();
@@ -254,8 +264,9 @@ __global__ void __launch_bounds__(256) k_duct_sgs(GridDims g, DuctConsts k, cons
                                                   const real* __restrict__ vf, const real* __restrict__ wf,
                                                   const real* __restrict__ Tf, const real* __restrict__ pf,
                                                   real* __restrict__ nut, real* __restrict__ st,
-                                                  unsigned long long* __restrict__ maxbits)
+                                                  double* __restrict__ dpart)
 {
+    __shared__ double red[2][4];
     const int kk = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int i = blockIdx.z;
@@ -277,15 +288,16 @@ __global__ void __launch_bounds__(256) k_duct_sgs(GridDims g, DuctConsts k, cons
             long long xp = q + g.sx, xm = q - g.sx;
             long long yp = lidx(g, i, wrapi(j + 1, g.ny), kk), ym = lidx(g, i, wrapi(j - 1, g.ny), kk);
             long long zp = lidx(g, i, j, wrapi(kk + 1, g.nz)), zm = lidx(g, i, j, wrapi(kk - 1, g.nz));
-            real du_dx = (uf[xp] - uf[xm]) / ((real)2.0 * DX);
-            real dv_dy = (vf[yp] - vf[ym]) / ((real)2.0 * DY);
-            real dw_dz = (wf[zp] - wf[zm]) / ((real)2.0 * DZ);
-            real du_dy = (uf[yp] - uf[ym]) / ((real)2.0 * DY);
-            real du_dz = (uf[zp] - uf[zm]) / ((real)2.0 * DZ);
-            real dv_dx = (vf[xp] - vf[xm]) / ((real)2.0 * DX);
-            real dv_dz = (vf[zp] - vf[zm]) / ((real)2.0 * DZ);
-            real dw_dx = (wf[xp] - wf[xm]) / ((real)2.0 * DX);
-            real dw_dy = (wf[yp] - wf[ym]) / ((real)2.0 * DY);
+            const real hx = (real)(0.5 / g.dx), hy = (real)(0.5 / g.dy), hz = (real)(0.5 / g.dz);
+            real du_dx = (uf[xp] - uf[xm]) * hx;
+            real dv_dy = (vf[yp] - vf[ym]) * hy;
+            real dw_dz = (wf[zp] - wf[zm]) * hz;
+            real du_dy = (uf[yp] - uf[ym]) * hy;
+            real du_dz = (uf[zp] - uf[zm]) * hz;
+            real dv_dx = (vf[xp] - vf[xm]) * hx;
+            real dv_dz = (vf[zp] - vf[zm]) * hz;
+            real dw_dx = (wf[xp] - wf[xm]) * hx;
+            real dw_dy = (wf[yp] - wf[ym]) * hy;
             real S12 = (real)0.5 * (du_dy + dv_dx), S13 = (real)0.5 * (du_dz + dw_dx), S23 = (real)0.5 * (dv_dz + dw_dy);
             real ss = (real)2.0 * (S12 * S12 + S13 * S13 + S23 * S23) + du_dx * du_dx + dv_dy * dv_dy + dw_dz * dw_dz;
             real smag = sqrt((real)2.0 * ss);
@@ -312,13 +324,19 @@ __global__ void __launch_bounds__(256) k_duct_sgs(GridDims g, DuctConsts k, cons
             st[q] = fmax(SL, SL * fmax((real)0.0, wr + te));
         }
     }
+    // block max (wavefront shuffles, then LDS across the 4 wavefronts); one
+    // slot per block, reduced by k_duct_dt: no atomics, order-independent
     for (int o = 32; o > 0; o >>= 1) {
         mn = fmax(mn, __shfl_xor(mn, o));
         ma = fmax(ma, __shfl_xor(ma, o));
     }
-    if (((threadIdx.y * blockDim.x + threadIdx.x) & 63) == 0) {
-        atomic_max_pos(&maxbits[1], mn);
-        atomic_max_pos(&maxbits[2], ma);
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    if ((tid & 63) == 0) { red[0][tid >> 6] = mn; red[1][tid >> 6] = ma; }
+    __syncthreads();
+    if (tid == 0) {
+        long long b = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        dpart[2 * b] = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+        dpart[2 * b + 1] = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
     }
 }
 
@@ -334,84 +352,184 @@ struct RhsArgs {
     const double* mu_tab;
     const double* cp_tab;
     const double* scal;
-    unsigned long long* maxbits;
+    double* vpart;
     int stage;
     double ca, cb;            // q_out = ca*q0 + cb*(q_in + dt*L) for stages 1, 2
     int track_vel;            // last stage: max |u| of the new state
     real* Lout[DNF];          // optional: the RHS itself (testing hook)
 };
 
-template <typename real>
-__device__ __forceinline__ void line7(const real* __restrict__ f, const long long* idx, real* v)
+// Flat index of the point at offset o (-3..3) from cell (i, j, k) along axis d;
+// y and z wrap (periodic duct), x never leaves the domain for the offsets used.
+__device__ __forceinline__ int line_idx(const GridDims& g, int base_i, int j, int kk, int d, int o)
 {
-#pragma unroll
-    for (int o = 0; o < 7; ++o) v[o] = f[idx[o]];
+    if (d == 0) return base_i + o * (int)g.sx + j * g.nz + kk;
+    if (d == 1) return base_i + wrapi(j + o, g.ny) * g.nz + kk;
+    return base_i + j * g.nz + wrapi(kk + o, g.nz);
 }
 
+// K3a: scalar RHS (phi: G equation :2706-2818; T :2912-2955) + SSP-RK3 combine
 template <typename real>
-__global__ void __launch_bounds__(256) k_duct_rhs(GridDims g, DuctConsts k, RhsArgs<real> a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_duct_rhs_scalar(GridDims g, DuctConsts k, RhsArgs<real> a)
 {
+    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (kk >= g.nz || j >= g.ny) return;
+    const int bi = i * (int)g.sx;
+    const int q = bi + j * g.nz + kk;
+    if (i == 0 || i == g.nx - 1) {
+        a.qout[3][q] = a.qin[3][q];
+        a.qout[4][q] = a.qin[4][q];
+        return;
+    }
+    const real DX = (real)g.dx, DY = (real)g.dy, DZ = (real)g.dz;
+    const real rut = (real)k.rho_u_Tu;
+    const bool xband = (i < 3 || i > g.nx - 4);
+    const int scheme = k.scheme;
+    const real* vel[3] = {a.uf, a.vf, a.wf};
+    const real T0 = a.Tf[q];
+    const real nu0 = a.nut[q];
+    const int xp = q + (int)g.sx, xm = q - (int)g.sx;
+    const int yp = line_idx(g, bi, j, kk, 1, 1), ym = line_idx(g, bi, j, kk, 1, -1);
+    const int zp = line_idx(g, bi, j, kk, 2, 1), zm = line_idx(g, bi, j, kk, 2, -1);
+    const real* S2[2] = {a.pf, a.Tf};
+    real cv[2], sg[2], lap[2], grad[3];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const real* F = S2[s];
+        const real isc2 = (real)(0.5 / ((s == 0) ? k.sc_t : k.pr_t));
+        real acc = (real)0, sacc = (real)0, lacc = (real)0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double dd64 = d == 0 ? g.dx : (d == 1 ? g.dy : g.dz);
+            const real id = (real)(1.0 / dd64), id2 = (real)(1.0 / (dd64 * dd64)), hid = (real)(0.5 / dd64);
+            const int qp = d == 0 ? xp : (d == 1 ? yp : zp);
+            const int qm = d == 0 ? xm : (d == 1 ? ym : zm);
+            real ua = vel[d][q];
+            real vm1 = F[qm], v0 = F[q], vp1 = F[qp];
+            real t;
+            if (d == 0 && xband) {
+                t = (ua >= (real)0.0) ? ua * (v0 - vm1) * id : ua * (vp1 - v0) * id;
+            } else {
+                real v[7];
+#pragma unroll
+                for (int o = 0; o < 7; ++o) v[o] = F[line_idx(g, bi, j, kk, d, o - 3)];
+                bool pos = ua >= (real)0.0;
+                real fp = face_plus<real>(scheme, v, pos);
+                real fm = face_minus<real>(scheme, v, pos);
+                t = ua * (fp - fm) * id;
+            }
+            acc = (d == 0) ? t : acc + t;
+            real ntp = a.nut[qp], ntm = a.nut[qm];
+            real gp = (ntp + nu0) * isc2, gm = (nu0 + ntm) * isc2;
+            real t2 = (gp * (vp1 - v0) - gm * (v0 - vm1)) * id2;
+            sacc = (d == 0) ? t2 : sacc + t2;
+            real t3 = (vp1 - (real)2.0 * v0 + vm1) * id2;
+            lacc = (d == 0) ? t3 : lacc + t3;
+            if (s == 0) grad[d] = (vp1 - vm1) * hid;
+        }
+        cv[s] = acc; sg[s] = sacc; lap[s] = lacc;
+    }
+    const real g0 = progress<real>(a.pf[q], DX);
+    const real wdt = (real)k.accel_width, ctr = (real)k.accel_center;
+    real gmag = sqrt(grad[0] * grad[0] + grad[1] * grad[1] + grad[2] * grad[2]);
+    real art = (real)0;
+    if (g0 > (real)0.1 && g0 < (real)0.99) {
+        real df = g0 - ctr;
+        real win = exp(-(df * df) / ((real)2.0 * wdt * wdt));
+        art = (real)5.0 * (nu0 + (real)1.0e-5) * win * lap[0];
+    }
+    real L3 = -cv[0] + a.st[q] * gmag + sg[0] + art;
+    const real rf0 = rut / T0;
+    real mu = table_lookup<real>(a.mu_tab, k, T0);
+    real cp = table_lookup<real>(a.cp_tab, k, T0);
+    real lam = mu * cp / (real)k.prandtl;
+    real al = lam / (rf0 * cp);
+    const real Tu = (real)k.Tu, Tb = (real)k.Tb;
+    real Ttg = Tu + g0 * (Tb - Tu);
+    real L4 = -cv[1] + al * lap[1] + sg[1] + (real)k.relaxation * (Ttg - T0);
+    if (a.Lout[0]) { a.Lout[3][q] = L3; a.Lout[4][q] = L4; }
+    const real DT = (real)a.scal[0];
+    real y3 = a.qin[3][q] + DT * L3, y4 = a.qin[4][q] + DT * L4;
+    real v3, v4;
+    if (a.stage == 0) { v3 = y3; v4 = y4; }
+    else { v3 = (real)a.ca * a.q0[3][q] + (real)a.cb * y3; v4 = (real)a.ca * a.q0[4][q] + (real)a.cb * y4; }
+    v4 = fmax((real)200.0, fmin((real)3000.0, v4));
+    a.qout[3][q] = v3;
+    a.qout[4][q] = v4;
+}
+
+// K3b: momentum RHS (:2552-2691 structure) + SSP-RK3 combine; in the last stage
+// also the per-block max |u| of the new state (its T is written by K3a first)
+template <typename real>
+__global__ void __launch_bounds__(256) k_duct_rhs_momentum(GridDims g, DuctConsts k, RhsArgs<real> a)
+{
+    __shared__ double vred[4];
     const int kk = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int i = blockIdx.z;
     double vmax = 0.0;
     if (kk < g.nz && j < g.ny) {
-        const long long q = lidx(g, i, j, kk);
+        const int bi = i * (int)g.sx;
+        const int q = bi + j * g.nz + kk;
         if (i == 0 || i == g.nx - 1) {
 #pragma unroll
-            for (int f = 0; f < DNF; ++f) a.qout[f][q] = a.qin[f][q];
+            for (int f = 0; f < 3; ++f) a.qout[f][q] = a.qin[f][q];
         } else {
             const real DX = (real)g.dx, DY = (real)g.dy, DZ = (real)g.dz;
-            const real dd[3] = {DX, DY, DZ};
             const real rut = (real)k.rho_u_Tu;
             const bool xband = (i < 3 || i > g.nx - 4);
             const int scheme = k.scheme;
-            // flat indices of the 7-point lines along x, y, z
-            long long L[3][7];
-#pragma unroll
-            for (int o = 0; o < 7; ++o) {
-                int ii = i + o - 3;
-                ii = ii < 0 ? 0 : (ii > g.nx - 1 ? g.nx - 1 : ii);
-                L[0][o] = lidx(g, ii, j, kk);
-                L[1][o] = lidx(g, i, wrapi(j + o - 3, g.ny), kk);
-                L[2][o] = lidx(g, i, j, wrapi(kk + o - 3, g.nz));
-            }
             const real* vel[3] = {a.uf, a.vf, a.wf};
-            const real rf0 = rut / a.Tf[q];
-            // momentum: conservative flux, mass flux from central faces, upwind face values
+            const real T0 = a.Tf[q];
+            const real rf0 = rut / T0;
             real conv[3] = {(real)0, (real)0, (real)0};
+#pragma unroll 1
             for (int d = 0; d < 3; ++d) {
+                const real id = (real)(1.0 / (d == 0 ? g.dx : (d == 1 ? g.dy : g.dz)));
+                const int qp = line_idx(g, bi, j, kk, d, 1), qm = line_idx(g, bi, j, kk, d, -1);
                 const real* U = vel[d];
-                real up = (real)0.5 * (U[L[d][4]] + U[q]);
-                real um = (real)0.5 * (U[q] + U[L[d][2]]);
-                real rp = (real)0.5 * (rut / a.Tf[L[d][4]] + rf0);
-                real rm = (real)0.5 * (rf0 + rut / a.Tf[L[d][2]]);
+                real up = (real)0.5 * (U[qp] + U[q]);
+                real um = (real)0.5 * (U[q] + U[qm]);
+                real rp = (real)0.5 * (rut / a.Tf[qp] + rf0);
+                real rm = (real)0.5 * (rf0 + rut / a.Tf[qm]);
                 real mp = rp * up, mm = rm * um;
+                const bool band = (d == 0 && xband);
+#pragma unroll
                 for (int m = 0; m < 3; ++m) {
-                    real v[7];
-                    line7<real>(vel[m], L[d], v);
+                    const real* V = vel[m];
                     real fp, fm;
-                    if (d == 0 && xband) {
-                        fp = (up >= (real)0.0) ? v[3] : v[4];
-                        fm = (um >= (real)0.0) ? v[2] : v[3];
+                    if (band) {
+                        real v2 = V[qm], v3 = V[q], v4 = V[qp];
+                        fp = (up >= (real)0.0) ? v3 : v4;
+                        fm = (um >= (real)0.0) ? v2 : v3;
                     } else {
+                        real v[7];
+#pragma unroll
+                        for (int o = 0; o < 7; ++o) v[o] = V[line_idx(g, bi, j, kk, d, o - 3)];
                         fp = face_plus<real>(scheme, v, up >= (real)0.0);
                         fm = face_minus<real>(scheme, v, um >= (real)0.0);
                     }
-                    real term = (mp * fp - mm * fm) / dd[d];
+                    real term = (mp * fp - mm * fm) * id;
                     conv[m] = (d == 0) ? term : conv[m] + term;
                 }
             }
-            const real T0 = a.Tf[q];
             real mu = table_lookup<real>(a.mu_tab, k, T0);
             real nu0 = a.nut[q];
             real mu_eff = mu + rf0 * nu0;
+            const int xp = q + (int)g.sx, xm = q - (int)g.sx;
+            const int yp = line_idx(g, bi, j, kk, 1, 1), ym = line_idx(g, bi, j, kk, 1, -1);
+            const int zp = line_idx(g, bi, j, kk, 2, 1), zm = line_idx(g, bi, j, kk, 2, -1);
+            const real ix2 = (real)(1.0 / (g.dx * g.dx)), iy2 = (real)(1.0 / (g.dy * g.dy)), iz2 = (real)(1.0 / (g.dz * g.dz));
             real diff[3];
+#pragma unroll
             for (int m = 0; m < 3; ++m) {
                 const real* U = vel[m];
-                real lx = (U[L[0][4]] - (real)2.0 * U[q] + U[L[0][2]]) / (DX * DX);
-                real ly = (U[L[1][4]] - (real)2.0 * U[q] + U[L[1][2]]) / (DY * DY);
-                real lz = (U[L[2][4]] - (real)2.0 * U[q] + U[L[2][2]]) / (DZ * DZ);
+                real uc = U[q];
+                real lx = (U[xp] - (real)2.0 * uc + U[xm]) * ix2;
+                real ly = (U[yp] - (real)2.0 * uc + U[ym]) * iy2;
+                real lz = (U[zp] - (real)2.0 * uc + U[zm]) * iz2;
                 diff[m] = mu_eff * (lx + ly + lz);
             }
             const real rho_avg = (real)a.scal[1];
@@ -421,9 +539,9 @@ __global__ void __launch_bounds__(256) k_duct_rhs(GridDims g, DuctConsts k, RhsA
             real fx = (real)0, fy = (real)0, fz = (real)0;
             const real wdt = (real)k.accel_width, ctr = (real)k.accel_center, Fm = (real)k.max_force;
             if (g0 > (real)0.01 && g0 < (real)0.99) {
-                real gx = (progress<real>(P[L[0][4]], DX) - progress<real>(P[L[0][2]], DX)) / ((real)2.0 * DX);
-                real gy = (progress<real>(P[L[1][4]], DX) - progress<real>(P[L[1][2]], DX)) / ((real)2.0 * DY);
-                real gz = (progress<real>(P[L[2][4]], DX) - progress<real>(P[L[2][2]], DX)) / ((real)2.0 * DZ);
+                real gx = (progress<real>(P[xp], DX) - progress<real>(P[xm], DX)) * (real)(0.5 / g.dx);
+                real gy = (progress<real>(P[yp], DX) - progress<real>(P[ym], DX)) * (real)(0.5 / g.dy);
+                real gz = (progress<real>(P[zp], DX) - progress<real>(P[zm], DX)) * (real)(0.5 / g.dz);
                 real df = g0 - ctr;
                 real win = exp(-(df * df) / ((real)2.0 * wdt * wdt));
                 real common = (real)k.accel_strength * rf0 * win;
@@ -434,79 +552,24 @@ __global__ void __launch_bounds__(256) k_duct_rhs(GridDims g, DuctConsts k, RhsA
             fx = fmax(-Fm, fmin(Fm, fx));
             fy = fmax(-Fm, fmin(Fm, fy));
             fz = fmax(-Fm, fmin(Fm, fz));
-            real Lq[DNF];
+            real Lq[3];
             Lq[0] = -conv[0] + diff[0] + grav + fx;
             Lq[1] = -conv[1] + diff[1] + fy;
             Lq[2] = -conv[2] + diff[2] + fz;
-            // scalars: phi (G equation) and T
-            real ntp[3], ntm[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) { ntp[d] = a.nut[L[d][4]]; ntm[d] = a.nut[L[d][2]]; }
-            const real* S2[2] = {a.pf, a.Tf};
-            real cv[2], sg[2], lap[2], grad[3];
-            for (int s = 0; s < 2; ++s) {
-                const real* F = S2[s];
-                const real sc = (s == 0) ? (real)k.sc_t : (real)k.pr_t;
-                real acc = (real)0, sacc = (real)0, lacc = (real)0;
-                for (int d = 0; d < 3; ++d) {
-                    real v[7];
-                    line7<real>(F, L[d], v);
-                    real ua = vel[d][q];
-                    real t;
-                    if (d == 0 && xband) {
-                        t = (ua >= (real)0.0) ? ua * (v[3] - v[2]) / dd[d] : ua * (v[4] - v[3]) / dd[d];
-                    } else {
-                        bool pos = ua >= (real)0.0;
-                        real fp = face_plus<real>(scheme, v, pos);
-                        real fm = face_minus<real>(scheme, v, pos);
-                        t = ua * (fp - fm) / dd[d];
-                    }
-                    acc = (d == 0) ? t : acc + t;
-                    real gp = (ntp[d] + nu0) / ((real)2.0 * sc), gm = (nu0 + ntm[d]) / ((real)2.0 * sc);
-                    real t2 = (gp * (v[4] - v[3]) / dd[d] - gm * (v[3] - v[2]) / dd[d]) / dd[d];
-                    sacc = (d == 0) ? t2 : sacc + t2;
-                    real t3 = (v[4] - (real)2.0 * v[3] + v[2]) / (dd[d] * dd[d]);
-                    lacc = (d == 0) ? t3 : lacc + t3;
-                    if (s == 0) grad[d] = (v[4] - v[2]) / ((real)2.0 * dd[d]);
-                }
-                cv[s] = acc; sg[s] = sacc; lap[s] = lacc;
-            }
-            real gmag = sqrt(grad[0] * grad[0] + grad[1] * grad[1] + grad[2] * grad[2]);
-            real art = (real)0;
-            if (g0 > (real)0.1 && g0 < (real)0.99) {
-                real df = g0 - ctr;
-                real win = exp(-(df * df) / ((real)2.0 * wdt * wdt));
-                art = (real)5.0 * (nu0 + (real)1.0e-5) * win * lap[0];
-            }
-            Lq[3] = -cv[0] + a.st[q] * gmag + sg[0] + art;
-            // molecular diffusivity alpha at the cell (same expression as K2)
-            real rf = rut / T0;
-            real cp = table_lookup<real>(a.cp_tab, k, T0);
-            real lam = mu * cp / (real)k.prandtl;
-            real al = lam / (rf * cp);
-            const real Tu = (real)k.Tu, Tb = (real)k.Tb;
-            real Ttg = Tu + g0 * (Tb - Tu);
-            Lq[4] = -cv[1] + al * lap[1] + sg[1] + (real)k.relaxation * (Ttg - T0);
             if (a.Lout[0]) {
 #pragma unroll
-                for (int f = 0; f < DNF; ++f) a.Lout[f][q] = Lq[f];
+                for (int f = 0; f < 3; ++f) a.Lout[f][q] = Lq[f];
             }
-            // SSP-RK3 combine (Shu-Osher form)
             const real DT = (real)a.scal[0];
-            real out[DNF];
+            real out[3];
 #pragma unroll
-            for (int f = 0; f < DNF; ++f) {
+            for (int f = 0; f < 3; ++f) {
                 real y = a.qin[f][q] + DT * Lq[f];
-                real v;
-                if (a.stage == 0) v = y;
-                else v = (real)a.ca * a.q0[f][q] + (real)a.cb * y;
-                if (f == 4) v = fmax((real)200.0, fmin((real)3000.0, v));
-                out[f] = v;
+                out[f] = (a.stage == 0) ? y : (real)a.ca * a.q0[f][q] + (real)a.cb * y;
+                a.qout[f][q] = out[f];
             }
-#pragma unroll
-            for (int f = 0; f < DNF; ++f) a.qout[f][q] = out[f];
-            if (a.track_vel && i <= g.nx - 2) {
-                real rho = rut / out[4];
+            if (a.track_vel) {
+                real rho = rut / a.qout[4][q];
                 real uu = out[0] / rho, vv = out[1] / rho, ww = out[2] / rho;
                 vmax = (double)sqrt(uu * uu + vv * vv + ww * ww);
             }
@@ -514,33 +577,50 @@ __global__ void __launch_bounds__(256) k_duct_rhs(GridDims g, DuctConsts k, RhsA
     }
     if (a.track_vel) {
         for (int o = 32; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o));
-        if (((threadIdx.y * blockDim.x + threadIdx.x) & 63) == 0) atomic_max_pos(&a.maxbits[0], vmax);
+        const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+        if ((tid & 63) == 0) vred[tid >> 6] = vmax;
+        __syncthreads();
+        if (tid == 0) {
+            long long b = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+            a.vpart[b] = fmax(fmax(vred[0], vred[1]), fmax(vred[2], vred[3]));
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// DT (:4399-4411 structure): one thread, reads the exact maxima, resets them
+// DT (:4399-4411 structure): one block reduces the per-block maxima of the
+// previous step (max |u| of the new state, max nu / alpha over its 3 stages)
 // ---------------------------------------------------------------------------
-__global__ void k_duct_dt(double dx, DuctConsts k, double* scal, unsigned long long* maxbits)
+__global__ void __launch_bounds__(1024) k_duct_dt(double dx, DuctConsts k, double* scal, const double* vpart,
+                                                  int nv, const double* dpart, int nd)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double mv = __longlong_as_double((long long)maxbits[0]);
-    double mn = __longlong_as_double((long long)maxbits[1]);
-    double ma = __longlong_as_double((long long)maxbits[2]);
+    __shared__ double r[3][1024];
+    double mv = 0.0, mn = 0.0, ma = 0.0;
+    for (int b = threadIdx.x; b < nv; b += 1024) mv = fmax(mv, vpart[b]);
+    for (int b = threadIdx.x; b < nd; b += 1024) { mn = fmax(mn, dpart[2 * b]); ma = fmax(ma, dpart[2 * b + 1]); }
+    r[0][threadIdx.x] = mv; r[1][threadIdx.x] = mn; r[2][threadIdx.x] = ma;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int a = 0; a < 3; ++a) r[a][threadIdx.x] = fmax(r[a][threadIdx.x], r[a][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    mv = r[0][0]; mn = r[1][0]; ma = r[2][0];
     double maxvel = mv > LES_SMALL ? mv : k.U;
     double dtc = k.courant * dx / (maxvel + LES_SMALL);
     double dtn = k.courant_diff * (dx * dx) / (mn + LES_SMALL);
     double dta = k.courant_diff * (dx * dx) / (ma + LES_SMALL);
     scal[0] = fmin(dtc, fmin(dtn, dta));
     scal[2] = maxvel;
-    maxbits[0] = 0ull; maxbits[1] = 0ull; maxbits[2] = 0ull;
 }
 
 // max |u| over planes 0..nx-2 of a state (used after upload / init)
 template <typename real>
 __global__ void __launch_bounds__(256) k_duct_velmax(GridDims g, double rho_u_Tu, const real* q0, const real* q1,
-                                                     const real* q2, const real* q4, unsigned long long* maxbits)
+                                                     const real* q2, const real* q4, double* vpart)
 {
+    __shared__ double vred[4];
     double vm = 0.0;
     long long lim = (long long)(g.nx - 1) * g.sx;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < lim; q += (long long)gridDim.x * blockDim.x) {
@@ -549,7 +629,9 @@ __global__ void __launch_bounds__(256) k_duct_velmax(GridDims g, double rho_u_Tu
         vm = fmax(vm, (double)sqrt(u * u + v * v + w * w));
     }
     for (int o = 32; o > 0; o >>= 1) vm = fmax(vm, __shfl_xor(vm, o));
-    if ((threadIdx.x & 63) == 0) atomic_max_pos(&maxbits[0], vm);
+    if ((threadIdx.x & 63) == 0) vred[threadIdx.x >> 6] = vm;
+    __syncthreads();
+    if (threadIdx.x == 0) vpart[blockIdx.x] = fmax(fmax(vred[0], vred[1]), fmax(vred[2], vred[3]));
 }
 
 // ---------------------------------------------------------------------------
@@ -655,8 +737,9 @@ __global__ void __launch_bounds__(1024) k_duct_sem_norm(long long P, double targ
 template <typename real>
 __global__ void __launch_bounds__(256) k_duct_bc(GridDims g, DuctConsts k, const double* __restrict__ inlet,
                                                  real* q0, real* q1, real* q2, real* q3, real* q4,
-                                                 unsigned long long* maxbits)
+                                                 double* vpart)
 {
+    __shared__ double vred[4];
     const int kk = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     double vm = 0.0;
@@ -677,7 +760,9 @@ __global__ void __launch_bounds__(256) k_duct_bc(GridDims g, DuctConsts k, const
         q0[qn] = q0[qm]; q1[qn] = q1[qm]; q2[qn] = q2[qm]; q3[qn] = q3[qm]; q4[qn] = q4[qm];
     }
     for (int o = 32; o > 0; o >>= 1) vm = fmax(vm, __shfl_xor(vm, o));
-    if ((threadIdx.x & 63) == 0) atomic_max_pos(&maxbits[0], vm);
+    if ((threadIdx.x & 63) == 0) vred[threadIdx.x >> 6] = vm;
+    __syncthreads();
+    if (threadIdx.x == 0) vpart[(long long)blockIdx.y * gridDim.x + blockIdx.x] = fmax(fmax(vred[0], vred[1]), fmax(vred[2], vred[3]));
 }
 
 // ---------------------------------------------------------------------------
@@ -723,19 +808,23 @@ __global__ void __launch_bounds__(256) k_copy(long long n, const real* __restric
 
 // ---------------------------------------------------------------------------
 // statistics: per-cell FP64 accumulators (:3167-3192) + y-z plane profiles.
-// One block per x-plane; the plane sums are reduced in a fixed order.
+// Grid (chunks, nx): each block sums one chunk of one x-plane; the chunk sums
+// are added to the plane profile in a fixed order by k_duct_stats_final.
 // ---------------------------------------------------------------------------
 template <typename real>
 __global__ void __launch_bounds__(256) k_duct_stats(GridDims g, DuctConsts k, const real* q0, const real* q1,
                                                     const real* q2, const real* q3, const real* q4, StatPtrs st,
-                                                    int per_cell, double* prof)
+                                                    int per_cell, double* spart)
 {
     __shared__ double red[10][256];
-    const int i = blockIdx.x;
+    const int i = blockIdx.y;
     const long long P = (long long)g.ny * g.nz;
+    const long long chunk = (P + gridDim.x - 1) / gridDim.x;
+    const long long p0 = (long long)blockIdx.x * chunk;
+    const long long p1 = p0 + chunk < P ? p0 + chunk : P;
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const real dx = (real)g.dx;
-    for (long long p = threadIdx.x; p < P; p += 256) {
+    for (long long p = p0 + threadIdx.x; p < p1; p += 256) {
         long long q = (long long)i * P + p;
         real rho = (real)k.rho_u_Tu / q4[q];
         double u = (double)(q0[q] / rho), v = (double)(q1[q] / rho), w = (double)(q2[q] / rho);
@@ -755,7 +844,17 @@ __global__ void __launch_bounds__(256) k_duct_stats(GridDims g, DuctConsts k, co
             for (int a = 0; a < 10; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x < 10) prof[10 * (long long)i + threadIdx.x] += red[threadIdx.x][0];
+    if (threadIdx.x < 10) spart[((long long)i * gridDim.x + blockIdx.x) * 10 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_duct_stats_final(int nx, int chunks, const double* spart, double* prof)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nx * 10) return;
+    int i = t / 10, a = t % 10;
+    double s = 0.0;
+    for (int c = 0; c < chunks; ++c) s += spart[((long long)i * chunks + c) * 10 + a];
+    prof[t] += s;
 }
 
 // fixed-point time series (u, v, w, G) at npts cells into a ring of cap steps
@@ -900,11 +999,17 @@ int duct_create(les_gpu* c)
     d->b.rho_part = (double*)dalloc_bytes(c, 8 * (size_t)d->filter_blocks);
     d->b.scal = (double*)dalloc_bytes(c, 8 * 8);
     d->b.maxbits = (unsigned long long*)dalloc_bytes(c, 8 * 4);
+    d->bc_blocks = (int)(((c->g.nz + 255) / 256) * c->g.ny);
+    d->b.vpart = (double*)dalloc_bytes(c, 8 * (size_t)(d->filter_blocks + d->bc_blocks + 1024));
+    d->b.dpart = (double*)dalloc_bytes(c, 8 * 2 * 3 * (size_t)d->filter_blocks);
+    d->stat_chunks = 16;
+    d->b.spart = (double*)dalloc_bytes(c, 8 * 10 * (size_t)d->stat_chunks * c->p.nx);
+    if (!d->b.vpart || !d->b.dpart || !d->b.spart) return LES_ERR_HIP;
     d->b.prof = (double*)dalloc_bytes(c, 8 * 10 * (size_t)c->p.nx);
     d->b.mon_part = (double*)dalloc_bytes(c, 8 * 6 * 1024);
     if (!d->b.eddy || !d->b.inlet || !d->b.rho_part || !d->b.scal || !d->b.maxbits || !d->b.prof || !d->b.mon_part)
         return LES_ERR_HIP;
-    if (c->p.stats_start_step >= 0) {
+    if (c->p.stats_start_step >= 0 && c->p.stats_per_cell) {
         for (int a = 0; a < 10; ++a)
             if (!(d->b.stat[a] = (double*)dalloc_bytes(c, 8 * n))) return LES_ERR_HIP;
     }
@@ -945,7 +1050,7 @@ static int duct_velmax_t(les_gpu* c)
     const GridDims& g = c->g;
     hipLaunchKernelGGL(k_duct_velmax<real>, dim3(1024), dim3(256), 0, c->stream, g, d->k.rho_u_Tu,
                        (const real*)d->b.A[0], (const real*)d->b.A[1], (const real*)d->b.A[2], (const real*)d->b.A[4],
-                       d->b.maxbits);
+                       d->b.vpart);
     DLAUNCH_CHECK();
     return LES_OK;
 }
@@ -964,20 +1069,22 @@ static int duct_stage_t(les_gpu* c, int stage, const real* const* qin, real* con
                        d->b.scal);
     DLAUNCH_CHECK();
     hipLaunchKernelGGL(k_duct_sgs<real>, L.grid, L.block, 0, c->stream, g, d->k, d->b.mu_tab, d->b.cp_tab, F[0], F[1],
-                       F[2], F[3], F[4], (real*)d->b.nut, (real*)d->b.st, d->b.maxbits);
+                       F[2], F[3], F[4], (real*)d->b.nut, (real*)d->b.st,
+                       d->b.dpart + 2 * (size_t)stage * d->filter_blocks);
     DLAUNCH_CHECK();
     RhsArgs<real> a;
     a.uf = F[0]; a.vf = F[1]; a.wf = F[2]; a.Tf = F[3]; a.pf = F[4];
     a.nut = (const real*)d->b.nut; a.st = (const real*)d->b.st;
     for (int f = 0; f < DNF; ++f) { a.qin[f] = qin[f]; a.q0[f] = (const real*)d->b.A[f]; a.qout[f] = qout[f]; }
-    a.mu_tab = d->b.mu_tab; a.cp_tab = d->b.cp_tab; a.scal = d->b.scal; a.maxbits = d->b.maxbits;
+    a.mu_tab = d->b.mu_tab; a.cp_tab = d->b.cp_tab; a.scal = d->b.scal; a.vpart = d->b.vpart;
     a.stage = stage;
     a.ca = (stage == 1) ? 0.75 : (1.0 / 3.0);
     a.cb = (stage == 1) ? 0.25 : (2.0 / 3.0);
     a.track_vel = (stage == 2);
     for (int f = 0; f < DNF; ++f) a.Lout[f] = Lout ? Lout[f] : nullptr;
     if (c->timing) DHIPC(hipEventRecord(c->ev[0], c->stream));
-    hipLaunchKernelGGL(k_duct_rhs<real>, L.grid, L.block, 0, c->stream, g, d->k, a);
+    hipLaunchKernelGGL(k_duct_rhs_scalar<real>, L.grid, L.block, 0, c->stream, g, d->k, a);
+    hipLaunchKernelGGL(k_duct_rhs_momentum<real>, L.grid, L.block, 0, c->stream, g, d->k, a);
     DLAUNCH_CHECK();
     if (c->timing) {
         DHIPC(hipEventRecord(c->ev[1], c->stream));
@@ -997,7 +1104,8 @@ static int duct_step_launches(les_gpu* c, bool reinit, bool stats, bool points)
     DuctState* d = c->d;
     const GridDims& g = c->g;
     const long long P = (long long)g.ny * g.nz;
-    hipLaunchKernelGGL(k_duct_dt, dim3(1), dim3(64), 0, c->stream, g.dx, d->k, d->b.scal, d->b.maxbits);
+    hipLaunchKernelGGL(k_duct_dt, dim3(1), dim3(1024), 0, c->stream, g.dx, d->k, d->b.scal, d->b.vpart,
+                       d->filter_blocks + d->bc_blocks, d->b.dpart, 3 * d->filter_blocks);
     DLAUNCH_CHECK();
     if (d->k.nvort > 0) {
         hipLaunchKernelGGL(k_duct_eddy_update, dim3((d->k.nvort + 255) / 256), dim3(256), 0, c->stream, d->k, d->b.scal,
@@ -1015,7 +1123,7 @@ static int duct_step_launches(les_gpu* c, bool reinit, bool stats, bool points)
     if ((e = duct_stage_t<real>(c, 1, (const real* const*)B, B)) != LES_OK) return e;
     if ((e = duct_stage_t<real>(c, 2, (const real* const*)B, A)) != LES_OK) return e;
     hipLaunchKernelGGL(k_duct_bc<real>, dim3((g.nz + 255) / 256, g.ny), dim3(256), 0, c->stream, g, d->k, d->b.inlet,
-                       A[0], A[1], A[2], A[3], A[4], d->b.maxbits);
+                       A[0], A[1], A[2], A[3], A[4], d->b.vpart + d->filter_blocks);
     DLAUNCH_CHECK();
     if (reinit) {
         Launch3 L = launch3(g.nx, g.ny, g.nz);
@@ -1032,9 +1140,11 @@ static int duct_step_launches(les_gpu* c, bool reinit, bool stats, bool points)
     if (stats) {
         StatPtrs st;
         for (int a = 0; a < 10; ++a) st.acc[a] = d->b.stat[a];
-        hipLaunchKernelGGL(k_duct_stats<real>, dim3(g.nx), dim3(256), 0, c->stream, g, d->k, (const real*)A[0],
-                           (const real*)A[1], (const real*)A[2], (const real*)A[3], (const real*)A[4], st,
-                           d->b.stat[0] != nullptr ? 1 : 0, d->b.prof);
+        hipLaunchKernelGGL(k_duct_stats<real>, dim3(d->stat_chunks, g.nx), dim3(256), 0, c->stream, g, d->k,
+                           (const real*)A[0], (const real*)A[1], (const real*)A[2], (const real*)A[3],
+                           (const real*)A[4], st, d->b.stat[0] != nullptr ? 1 : 0, d->b.spart);
+        hipLaunchKernelGGL(k_duct_stats_final, dim3((g.nx * 10 + 255) / 256), dim3(256), 0, c->stream, g.nx,
+                           d->stat_chunks, d->b.spart, d->b.prof);
         DLAUNCH_CHECK();
     }
     if (points) {
@@ -1189,7 +1299,8 @@ static int duct_xfer(les_gpu* c, void* dev, double* host, bool upload)
 static int duct_after_state_change(les_gpu* c)
 {
     DuctState* d = c->d;
-    DHIPC(hipMemsetAsync(d->b.maxbits, 0, 4 * 8, c->stream));
+    DHIPC(hipMemsetAsync(d->b.vpart, 0, 8 * (size_t)(d->filter_blocks + d->bc_blocks), c->stream));
+    DHIPC(hipMemsetAsync(d->b.dpart, 0, 8 * 2 * 3 * (size_t)d->filter_blocks, c->stream));
     int e = (d->precision == LES_PREC_FP32) ? duct_velmax_t<float>(c) : duct_velmax_t<double>(c);
     if (e != LES_OK) return e;
     DHIPC(hipStreamSynchronize(c->stream));
@@ -1236,7 +1347,6 @@ int duct_init_state(les_gpu* c)
     const double* f[5] = {&h[0], &h[n], &h[2 * n], &h[3 * n], &h[4 * n]};
     int e = duct_upload_state(c, f, 5);
     if (e != LES_OK) return e;
-    DHIPC(hipMemsetAsync(d->b.maxbits + 1, 0, 2 * 8, c->stream));
     if (d->k.nvort > 0) {
         hipLaunchKernelGGL(k_duct_eddy_init, dim3((d->k.nvort + 255) / 256), dim3(256), 0, c->stream, d->k, d->b.eddy);
         DLAUNCH_CHECK();

@@ -67,6 +67,7 @@ static void set_reference_values(les_params* p)
         for (int n = 0; n < 7; ++n) p->probe_x[n] = xs[n];
     }
     p->stats_start_step = -1;                           /* dead in the reference (:4382) */
+    p->stats_per_cell = 1;
     p->seed = 0;
 }
 
@@ -104,6 +105,7 @@ static void set_duct_values(les_params* p, int nx, int ny, int nz)
         for (int n = 0; n < 7; ++n) p->probe_x[n] = xs[n];
     }
     p->stats_start_step = 0;
+    p->stats_per_cell = 0;
     p->seed = 0;
 }
 

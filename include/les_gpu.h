@@ -140,7 +140,8 @@ typedef struct les_params {
     /* sampling / statistics */
     int    num_probe_x;          /* flame-region x-stations */
     double probe_x[LES_NPROBE_MAX];
-    int    stats_start_step;     /* FP64 accumulators from this step on (<0 = off) */
+    int    stats_start_step;     /* FP64 statistics from this step on (<0 = off) */
+    int    stats_per_cell;       /* 1: also the per-cell accumulators of :3167-3192 */
 
     uint64_t seed;               /* SEM / noise seed */
 } les_params;

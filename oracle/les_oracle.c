@@ -1451,6 +1451,13 @@ int ora_probe_fetch(ora_ctx* c, les_probe_row* rows, int max_rows, int* n_rows)
     return LES_OK;
 }
 
+int ora_probe_now(ora_ctx* c, int t_loop)
+{
+    if (c->p.mode != LES_MODE_REFERENCE) return LES_ERR_PARAM;
+    probes_ref(c, t_loop);
+    return LES_OK;
+}
+
 int ora_get_step(const ora_ctx* c) { return c->step; }
 void ora_set_step(ora_ctx* c, int step) { c->step = step; }
 

@@ -31,6 +31,8 @@ int  ora_step(ora_ctx* c, int n_steps, les_step_report* rep);
 int  ora_rhs(ora_ctx* c, int t_loop, double* const* out, int n_out);
 int  ora_probe_fetch(ora_ctx* c, les_probe_row* rows, int max_rows, int* n_rows);
 int  ora_get_step(const ora_ctx* c);
+/* Flame-region probe rows of the current state (record_flame_region_data). */
+int  ora_probe_now(ora_ctx* c, int t_loop);
 void ora_set_step(ora_ctx* c, int step);
 /* SEM eddies: n x 7 doubles (x, y, z, radius, su, sv, sw). */
 int  ora_get_vortices(const ora_ctx* c, double* out, int max_n);

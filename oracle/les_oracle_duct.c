@@ -299,22 +299,23 @@ static void duct_rhs_eval(ora_ctx* c, double* const* Q, double* const* out)
                 double pu[7], pv[7], pw[7], pT[7], pp[7];
                 const int di[7] = {-1, 0, 0, 0, 0, 0, 1}, dj[7] = {0, -1, 0, 0, 0, 1, 0}, dk[7] = {0, 0, -1, 0, 1, 0, 0};
                 int npt = (i == 0 || i == nx - 1) ? 1 : 7;
+                const double irut = 1.0 / (P->rho_unburnt * P->inlet_temperature);
                 for (int t = 0; t < npt; ++t) {
                     size_t r = (npt == 1) ? q : QX(c, i + di[t], j + dj[t], k + dk[t]);
                     int slot = (npt == 1) ? 3 : t;
-                    double T = Q[4][r], rho = duct_rho(c, T);
-                    pu[slot] = Q[0][r] / rho; pv[slot] = Q[1][r] / rho; pw[slot] = Q[2][r] / rho;
+                    double T = Q[4][r], ri = T * irut;      /* 1/rho */
+                    pu[slot] = Q[0][r] * ri; pv[slot] = Q[1][r] * ri; pw[slot] = Q[2][r] * ri;
                     pT[slot] = T; pp[slot] = Q[3][r];
                 }
                 if (npt == 1) {
                     d->uf[q] = pu[3]; d->vf[q] = pv[3]; d->wf[q] = pw[3]; d->Tf[q] = pT[3]; d->pf[q] = pp[3];
                 } else {
-                    const double wc = P->filter_weight_center, den = 6.0 + wc;
-                    d->uf[q] = (pu[0] + pu[1] + pu[2] + wc * pu[3] + pu[4] + pu[5] + pu[6]) / den;
-                    d->vf[q] = (pv[0] + pv[1] + pv[2] + wc * pv[3] + pv[4] + pv[5] + pv[6]) / den;
-                    d->wf[q] = (pw[0] + pw[1] + pw[2] + wc * pw[3] + pw[4] + pw[5] + pw[6]) / den;
-                    d->Tf[q] = (pT[0] + pT[1] + pT[2] + wc * pT[3] + pT[4] + pT[5] + pT[6]) / den;
-                    d->pf[q] = (pp[0] + pp[1] + pp[2] + wc * pp[3] + pp[4] + pp[5] + pp[6]) / den;
+                    const double wc = P->filter_weight_center, iden = 1.0 / (6.0 + wc);
+                    d->uf[q] = (pu[0] + pu[1] + pu[2] + wc * pu[3] + pu[4] + pu[5] + pu[6]) * iden;
+                    d->vf[q] = (pv[0] + pv[1] + pv[2] + wc * pv[3] + pv[4] + pv[5] + pv[6]) * iden;
+                    d->wf[q] = (pw[0] + pw[1] + pw[2] + wc * pw[3] + pw[4] + pw[5] + pw[6]) * iden;
+                    d->Tf[q] = (pT[0] + pT[1] + pT[2] + wc * pT[3] + pT[4] + pT[5] + pT[6]) * iden;
+                    d->pf[q] = (pp[0] + pp[1] + pp[2] + wc * pp[3] + pp[4] + pp[5] + pp[6]) * iden;
                 }
             }
 
@@ -338,15 +339,16 @@ static void duct_rhs_eval(ora_ctx* c, double* const* Q, double* const* out)
                 if (i == 0 || i == nx - 1) { d->nu_t[q] = 0.0; d->S_mag[q] = 0.0; d->ST[q] = SL; continue; }
                 if (nu > mnu) mnu = nu;
                 if (al > mal) mal = al;
-                double du_dx = (d->uf[QX(c, i + 1, j, k)] - d->uf[QX(c, i - 1, j, k)]) / (2.0 * DX);
-                double dv_dy = (d->vf[QX(c, i, j + 1, k)] - d->vf[QX(c, i, j - 1, k)]) / (2.0 * DY);
-                double dw_dz = (d->wf[QX(c, i, j, k + 1)] - d->wf[QX(c, i, j, k - 1)]) / (2.0 * DZ);
-                double du_dy = (d->uf[QX(c, i, j + 1, k)] - d->uf[QX(c, i, j - 1, k)]) / (2.0 * DY);
-                double du_dz = (d->uf[QX(c, i, j, k + 1)] - d->uf[QX(c, i, j, k - 1)]) / (2.0 * DZ);
-                double dv_dx = (d->vf[QX(c, i + 1, j, k)] - d->vf[QX(c, i - 1, j, k)]) / (2.0 * DX);
-                double dv_dz = (d->vf[QX(c, i, j, k + 1)] - d->vf[QX(c, i, j, k - 1)]) / (2.0 * DZ);
-                double dw_dx = (d->wf[QX(c, i + 1, j, k)] - d->wf[QX(c, i - 1, j, k)]) / (2.0 * DX);
-                double dw_dy = (d->wf[QX(c, i, j + 1, k)] - d->wf[QX(c, i, j - 1, k)]) / (2.0 * DY);
+                const double hx = 0.5 / DX, hy = 0.5 / DY, hz = 0.5 / DZ;
+                double du_dx = (d->uf[QX(c, i + 1, j, k)] - d->uf[QX(c, i - 1, j, k)]) * hx;
+                double dv_dy = (d->vf[QX(c, i, j + 1, k)] - d->vf[QX(c, i, j - 1, k)]) * hy;
+                double dw_dz = (d->wf[QX(c, i, j, k + 1)] - d->wf[QX(c, i, j, k - 1)]) * hz;
+                double du_dy = (d->uf[QX(c, i, j + 1, k)] - d->uf[QX(c, i, j - 1, k)]) * hy;
+                double du_dz = (d->uf[QX(c, i, j, k + 1)] - d->uf[QX(c, i, j, k - 1)]) * hz;
+                double dv_dx = (d->vf[QX(c, i + 1, j, k)] - d->vf[QX(c, i - 1, j, k)]) * hx;
+                double dv_dz = (d->vf[QX(c, i, j, k + 1)] - d->vf[QX(c, i, j, k - 1)]) * hz;
+                double dw_dx = (d->wf[QX(c, i + 1, j, k)] - d->wf[QX(c, i - 1, j, k)]) * hx;
+                double dw_dy = (d->wf[QX(c, i, j + 1, k)] - d->wf[QX(c, i, j - 1, k)]) * hy;
                 double S12 = 0.5 * (du_dy + dv_dx), S13 = 0.5 * (du_dz + dw_dx), S23 = 0.5 * (dv_dz + dw_dy);
                 double ss = 2.0 * (S12 * S12 + S13 * S13 + S23 * S23) + du_dx * du_dx + dv_dy * dv_dy + dw_dz * dw_dz;
                 double smag = sqrt(2.0 * ss);
@@ -416,7 +418,7 @@ static void duct_rhs_eval(ora_ctx* c, double* const* Q, double* const* out)
                             faces(scheme, v, up >= 0.0, &fp, &t1);
                             faces(scheme, v, um >= 0.0, &t2, &fm);
                         }
-                        double term = (mp * fp - mm * fm) / dd[a];
+                        double term = (mp * fp - mm * fm) * (1.0 / dd[a]);
                         conv[m] = (a == 0) ? term : conv[m] + term;
                     }
                 }
@@ -425,18 +427,18 @@ static void duct_rhs_eval(ora_ctx* c, double* const* Q, double* const* out)
                 double diff[3];
                 for (int m = 0; m < 3; ++m) {
                     const double* U = vel[m];
-                    double lx = (U[nb[0][1]] - 2.0 * U[q] + U[nb[0][0]]) / (DX * DX);
-                    double ly = (U[nb[1][1]] - 2.0 * U[q] + U[nb[1][0]]) / (DY * DY);
-                    double lz = (U[nb[2][1]] - 2.0 * U[q] + U[nb[2][0]]) / (DZ * DZ);
+                    double lx = (U[nb[0][1]] - 2.0 * U[q] + U[nb[0][0]]) * (1.0 / (DX * DX));
+                    double ly = (U[nb[1][1]] - 2.0 * U[q] + U[nb[1][0]]) * (1.0 / (DY * DY));
+                    double lz = (U[nb[2][1]] - 2.0 * U[q] + U[nb[2][0]]) * (1.0 / (DZ * DZ));
                     diff[m] = mu_eff * (lx + ly + lz);
                 }
                 double grav = (rf0 - d->rho_avg) * G_GRAV;
                 double g0 = ora_duct_progress(d->pf[q], DX);
                 double fx = 0.0, fy = 0.0, fz = 0.0;
                 if (g0 > 0.01 && g0 < 0.99) {
-                    double gx = (ora_duct_progress(d->pf[nb[0][1]], DX) - ora_duct_progress(d->pf[nb[0][0]], DX)) / (2.0 * DX);
-                    double gy = (ora_duct_progress(d->pf[nb[1][1]], DX) - ora_duct_progress(d->pf[nb[1][0]], DX)) / (2.0 * DY);
-                    double gz = (ora_duct_progress(d->pf[nb[2][1]], DX) - ora_duct_progress(d->pf[nb[2][0]], DX)) / (2.0 * DZ);
+                    double gx = (ora_duct_progress(d->pf[nb[0][1]], DX) - ora_duct_progress(d->pf[nb[0][0]], DX)) * (0.5 / DX);
+                    double gy = (ora_duct_progress(d->pf[nb[1][1]], DX) - ora_duct_progress(d->pf[nb[1][0]], DX)) * (0.5 / DY);
+                    double gz = (ora_duct_progress(d->pf[nb[2][1]], DX) - ora_duct_progress(d->pf[nb[2][0]], DX)) * (0.5 / DZ);
                     double df = g0 - ctr;
                     double win = exp(-(df * df) / (2.0 * wdt * wdt));
                     double common = P->flame_accel_strength * rf0 * win;
@@ -459,27 +461,28 @@ static void duct_rhs_eval(ora_ctx* c, double* const* Q, double* const* out)
                 double cv[2], sg[2], lap[2], grad[3];
                 for (int s = 0; s < 2; ++s) {
                     const double* F = S2[s];
-                    const double sc = (s == 0) ? P->sc_t : P->pr_t;
+                    const double isc2 = 0.5 / ((s == 0) ? P->sc_t : P->pr_t);
                     double acc = 0.0, sacc = 0.0, lacc = 0.0;
                     for (int a = 0; a < 3; ++a) {
+                        const double id = 1.0 / dd[a], id2 = 1.0 / (dd[a] * dd[a]), hid = 0.5 / dd[a];
                         double v[7];
                         line7(c, F, i, j, k, a, v);
                         double ua = vel[a][q];
                         double fp, fm;
                         if (a == 0 && xband) {
-                            double t = (ua >= 0.0) ? ua * (v[3] - v[2]) / dd[a] : ua * (v[4] - v[3]) / dd[a];
+                            double t = (ua >= 0.0) ? ua * (v[3] - v[2]) * id : ua * (v[4] - v[3]) * id;
                             acc = (a == 0) ? t : acc + t;
                         } else {
                             faces(scheme, v, ua >= 0.0, &fp, &fm);
-                            double t = ua * (fp - fm) / dd[a];
+                            double t = ua * (fp - fm) * id;
                             acc = (a == 0) ? t : acc + t;
                         }
-                        double gp = (ntp[a] + nu0) / (2.0 * sc), gm = (nu0 + ntm[a]) / (2.0 * sc);
-                        double t2 = (gp * (v[4] - v[3]) / dd[a] - gm * (v[3] - v[2]) / dd[a]) / dd[a];
+                        double gp = (ntp[a] + nu0) * isc2, gm = (nu0 + ntm[a]) * isc2;
+                        double t2 = (gp * (v[4] - v[3]) - gm * (v[3] - v[2])) * id2;
                         sacc = (a == 0) ? t2 : sacc + t2;
-                        double t3 = (v[4] - 2.0 * v[3] + v[2]) / (dd[a] * dd[a]);
+                        double t3 = (v[4] - 2.0 * v[3] + v[2]) * id2;
                         lacc = (a == 0) ? t3 : lacc + t3;
-                        if (s == 0) grad[a] = (v[4] - v[2]) / (2.0 * dd[a]);
+                        if (s == 0) grad[a] = (v[4] - v[2]) * hid;
                     }
                     cv[s] = acc; sg[s] = sacc; lap[s] = lacc;
                 }
@@ -599,9 +602,11 @@ static void duct_stats(ora_ctx* c)
             double rho = duct_rho(c, d->q[4][q]);
             double u = d->q[0][q] / rho, v = d->q[1][q] / rho, w = d->q[2][q] / rho;
             double g = ora_duct_progress(d->q[3][q], c->dx);
+            if (c->p.stats_per_cell) {
             c->stat[0][q] += u; c->stat[1][q] += v; c->stat[2][q] += w; c->stat[3][q] += g;
             c->stat[4][q] += u * u; c->stat[5][q] += v * v; c->stat[6][q] += w * w;
             c->stat[7][q] += u * v; c->stat[8][q] += u * w; c->stat[9][q] += v * w;
+            }
             acc[0] += u; acc[1] += v; acc[2] += w; acc[3] += g;
             acc[4] += u * u; acc[5] += v * v; acc[6] += w * w; acc[7] += u * v; acc[8] += u * w; acc[9] += v * w;
         }

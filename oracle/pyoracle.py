@@ -55,6 +55,27 @@ def lib():
         L.ora_boundary.argtypes = [C.c_void_p]
         L.ora_rand_stream.argtypes = [C.c_uint, C.c_int, P(C.c_int)]
         L.ora_get_stats.argtypes = [C.c_void_p, P(P(C.c_double)), P(C.c_int)]
+        L.ora_probe_now.argtypes = [C.c_void_p, C.c_int]
+        L.ora_teno5.argtypes = [C.c_double] * 5
+        L.ora_teno5.restype = C.c_double
+        L.ora_superbee.argtypes = [C.c_double]
+        L.ora_superbee.restype = C.c_double
+        L.ora_weno5_face.argtypes = [C.c_double] * 5
+        L.ora_weno5_face.restype = C.c_double
+        L.ora_minmod.argtypes = [C.c_double] * 2
+        L.ora_minmod.restype = C.c_double
+        L.ora_muscl_face.argtypes = [C.c_double] * 3
+        L.ora_muscl_face.restype = C.c_double
+        L.ora_weno5_gradient.argtypes = [P(C.c_double), C.c_double]
+        L.ora_weno5_gradient.restype = C.c_double
+        L.ora_duct_table.argtypes = [P(C.c_double), C.c_double, C.c_double, C.c_int, C.c_double]
+        L.ora_duct_table.restype = C.c_double
+        L.ora_duct_progress.argtypes = [C.c_double, C.c_double]
+        L.ora_duct_progress.restype = C.c_double
+        L.ora_duct_reinit.argtypes = [C.c_void_p, C.c_int]
+        L.ora_duct_sem.argtypes = [C.c_void_p, C.c_double]
+        L.ora_duct_set_eddies.argtypes = [C.c_void_p, P(C.c_double), C.c_int]
+        L.ora_duct_set_dt_state.argtypes = [C.c_void_p, C.c_double, C.c_double]
         _lib = L
     return _lib
 
@@ -183,6 +204,14 @@ class Oracle:
     def filter_and_props(self, fields):
         arrs = [np.ascontiguousarray(f, dtype=np.float64).reshape(-1) for f in fields]
         assert lib().ora_filter_and_props(self.h, _ptrs(arrs)) == 0
+
+    def probe_now(self, t_loop):
+        assert lib().ora_probe_now(self.h, t_loop) == 0
+
+    def raw_field(self, name, count):
+        out = np.empty(count)
+        assert lib().ora_get_field(self.h, name.encode(), out.ctypes.data_as(C.POINTER(C.c_double))) == 0
+        return out
 
     def stats(self):
         out = [np.empty(self.n) for _ in range(10)]

@@ -70,6 +70,8 @@ void calculate_RHS(int t_loop, const ref_grid u_in, const ref_grid v_in, const r
                    ref_grid* dYk_dt);
 
 int les_reference_main(int argc, char* argv[]);
+void record_flame_region_data(int t_loop, const ref_grid u, const ref_grid v, const ref_grid w, const ref_grid G,
+                              const ref_grid nu_t, const ref_grid S_mag);
 
 static unsigned g_seed = 0;
 static int g_steps = 3;
@@ -164,6 +166,9 @@ static void rhs_on_inputs(const char* dir, int t_rhs)
     int attr_d[REF_NX * REF_NY * REF_NZ];
     memcpy(attr_d, attribute, sizeof(attr_d));
     dump("attribute", -1, attr_d, sizeof(attr_d));
+    /* flame-region probe rows of the loaded state (appended to the solver's
+     * probe CSV files with timestep 999999) */
+    record_flame_region_data(999999, u, v, w, G, nu_t, S_mag);
 }
 
 int les_ref_printf(const char* fmt, ...)
