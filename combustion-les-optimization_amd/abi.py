@@ -127,6 +127,35 @@ class ProbeRow(C.Structure):
     ]
 
 
+LES_TILE_FLAT = 0
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [
+        ("max_slots", C.c_int),
+        ("pool_cells", C.c_longlong),
+        ("max_nx", C.c_int), ("max_ny", C.c_int), ("max_nz", C.c_int), ("max_vortices", C.c_int),
+        ("precision", C.c_int),
+        ("tile_k", C.c_int),
+        ("xcd_map", C.c_int),
+        ("autotune", C.c_int),
+        ("point_cap", C.c_int),
+        ("stats_per_cell", C.c_int),
+        ("table_t0", C.c_double), ("table_dt", C.c_double),
+        ("table_n", C.c_int),
+    ]
+
+
+class SlotInfo(C.Structure):
+    _fields_ = [
+        ("active", C.c_int), ("done", C.c_int),
+        ("step", C.c_int), ("n_steps", C.c_int),
+        ("time", C.c_double), ("dt", C.c_double), ("max_velocity", C.c_double),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("cells", C.c_longlong),
+    ]
+
+
 def declare_common(lib):
     """Declare the parameter/mixture entry points shared by both libraries."""
     lib.les_params_default.argtypes = [C.POINTER(Params), C.c_int]

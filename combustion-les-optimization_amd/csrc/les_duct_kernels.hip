@@ -6,24 +6,36 @@
 // Reductions (domain-mean density, SEM plane moments, statistics, DT maxima)
 // are FP64 in both forms.
 //
-// Per RK stage three kernels run on one stream:
-//   k_duct_filter  state -> filtered primitives u, v, w, T, phi (+ block sums of rho)
-//   k_duct_sgs     filtered velocity -> nu_t, S_T (+ exact max nu / alpha for DT)
-//   k_duct_rhs     RHS of the 5 state fields fused with the SSP-RK3 combine
-// A step adds: DT (1 thread), eddy update (1 block), inflow plane, its
-// normalisation (1 block), inflow/outflow planes, optional reinitialisation
-// and statistics.  No step needs the host: DT and every reduction stay on the
-// device, so a whole step is captured once into a hipGraph and replayed.
+// One engine context holds a pool of SLOTS; each slot is one run with its own
+// grid, parameters, time step and step counter.  Field storage is shared
+// arenas in HBM (first-fit cell ranges per slot) and every stencil kernel is a
+// persistent loop over a device-side TILE LIST (first-fit tile ranges per
+// slot), so one launch advances every active run: the ensemble and the served
+// request stream use the same kernels as a single run (a pool of one).  All
+// per-run state the kernels need (descriptor, dt, counters) lives on the
+// device, so one time step of the whole pool is captured once into a hipGraph
+// and replayed; admitting or retiring a run only rewrites device tables.
+//
+// Per RK stage:
+//   k_duct_filter      state -> filtered primitives u, v, w, T, phi (+ per-tile sums of rho)
+//   k_duct_rho_final   per-slot domain-mean density (fixed-order reduction)
+//   k_duct_sgs         nu_t, S_T (+ per-tile max nu / alpha for the next DT)
+//   k_duct_rhs_scalar  phi and T: RHS fused with the SSP-RK3 combine
+//   k_duct_rhs_momentum rho*u, rho*v, rho*w: RHS fused with the combine
+// Per step: DT, eddy update, inflow plane, its normalisation, inflow/outflow
+// planes, reinitialisation (slots due), statistics, point series, step end.
 #include "les_engine.hpp"
 
 // The duct scheme is this engine's own specification: its kernels may fuse
-// multiply-adds (the reference-scheme kernels above keep strict IEEE order).
+// multiply-adds (the reference-scheme kernels keep strict IEEE order).
 #pragma clang fp contract(fast)
 
 #define DNF 5
+#define DUCT_NPTS 64
+#define STAT_CHUNKS 16
 
 // ---------------------------------------------------------------------------
-// device-side parameters of the duct scheme
+// device-side parameters of the duct scheme (one set per slot)
 // ---------------------------------------------------------------------------
 struct DuctConsts {
     double rho_u_Tu;        // rho_unburnt * T_u  (rho = rho_u_Tu / T)
@@ -39,55 +51,64 @@ struct DuctConsts {
     int table_n;
     int scheme;             // LES_ADV_WENO5 or LES_ADV_MUSCL_MINMOD
     int nvort;
+    int reinit_interval, reinit_iters;
+    int stats_start, stats_per_cell;
     double L;               // eddy length scale
     double ly, lz;
     unsigned long long seed;
 };
 
-// Device buffers of one duct run.  Fields are stored as `real`.
-struct DuctBufs {
-    void* A[DNF];      // q^n, overwritten by q^{n+1} in the last stage
-    void* B[DNF];      // RK stage state
-    void* F[DNF];      // filtered u, v, w, T, phi
-    void* nut;         // nu_t
-    void* st;          // S_T
+// Run descriptor: written by the host when a run is admitted or retired.
+struct SlotDesc {
+    GridDims g;
+    long long off;          // first cell in the field arenas
+    long long poff;         // first value in the plane arena (3 planes)
+    long long xoff;         // first value in the profile arena (10 per x-plane)
+    long long eoff;         // first eddy in the eddy arena
+    long long ptoff;        // first value in the point-series arena
+    int tile0, ntiles;      // tile range in the tile list
+    int tiles_j, tiles_k;   // tiles of one x-plane
+    int active;             // 1 while admitted
+    int n_steps;            // steps requested
+    int point_idx[DUCT_NPTS];
+    DuctConsts k;
+};
+
+// Run state the kernels update (never written by the host while a step runs).
+struct SlotDyn {
+    double dt, rho_avg, maxvel, time;
+    int step;               // steps done
+    int done;               // step >= n_steps
+    int pad[2];
+};
+
+// Pool-wide device pointers handed to every kernel.
+struct Pool {
+    void* A[DNF];           // q^n, overwritten by q^{n+1} in the last stage
+    void* B[DNF];           // RK stage state / scratch
+    void* F[DNF];           // filtered u, v, w, T, phi
+    void* nut;
+    void* st;
+    double* stat[10];       // per-cell FP64 accumulators (optional arena)
+    const SlotDesc* desc;
+    SlotDyn* dyn;
+    const int* tile_slot;   // tile -> slot (-1: free)
+    const int* ntile;       // tile high-water mark
     double* cp_tab;
     double* mu_tab;
-    double* eddy;      // nvort x 7
-    double* inlet;     // 3 planes
-    double* rho_part;  // block partial sums of rho(T_f)
-    double* scal;      // [0] dt, [1] rho_avg, [2] maxvel, [3] last dt, [4] proj residual
-    unsigned long long* maxbits;   // unused by the duct step (kept for upload paths)
-    double* vpart;     // per-block max |u| of the new state (RHS stage 3, then inflow plane)
-    double* dpart;     // per-block max nu, alpha: [3 stages][blocks][2]
-    double* spart;     // statistics plane partial sums: [nx][chunks][10]
-    double* stat[10];  // per-cell FP64 accumulators
-    double* prof;      // nx x 10 FP64 plane accumulators
-    double* points;    // point time series ring: cap x npts x 4
-    int* point_idx;    // flat cell index of each point
-    double* mon_part;
+    double* eddy;           // 7 doubles per eddy
+    double* inlet;          // 3 planes per slot
+    double* prof;           // 10 FP64 per x-plane per slot
+    double* points;         // ring: cap x 64 x 4 per slot
+    double* rho_part;       // per tile
+    double* vpart;          // per tile: max |u| of the new state
+    double* dpart;          // [3][tile][2]: max nu, alpha per stage
+    double* bcpart;         // [slot][ny_max][nzb]: max |u| of the inflow plane
+    double* spart;          // [slot][nx_max][STAT_CHUNKS][10]
+    int tile_cap;
+    int ny_max, nz_max, nx_max, nzb;
+    int point_cap;          // steps per point-series ring
 };
-
-struct DuctState {
-    DuctConsts k;
-    DuctBufs b;
-    int precision;
-    int npts = 0;
-    int point_cap = 0;
-    int point_count = 0;
-    int filter_blocks = 0;
-    int bc_blocks = 0;
-    int stat_chunks = 0;
-    int prof_count = 0;
-    size_t esize = 8;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    bool graph_ok = false;
-    bool graph_stats = false;
-    int use_graph = 1;
-    double* stage64 = nullptr;   // FP64 staging for FP32 state transfers
-};
-
 // ---------------------------------------------------------------------------
 // helpers shared by host-side checks and kernels
 // ---------------------------------------------------------------------------
@@ -175,467 +196,639 @@ __device__ __forceinline__ real face_minus(int scheme, const real* v, bool posit
                     : face_left<real>(scheme, v[5], v[4], v[3], v[2], v[1]);
 }
 
+
 // ---------------------------------------------------------------------------
-// K1: primitives + box filter (:1307-1470 structure), block sums of rho(T_f)
+// persistent tile loop (XCD-aware) and tile decoding
 // ---------------------------------------------------------------------------
-template <typename real>
-__global__ void __launch_bounds__(256) k_duct_filter(GridDims g, DuctConsts k, const real* __restrict__ q0,
-                                                     const real* __restrict__ q1, const real* __restrict__ q2,
-                                                     const real* __restrict__ q3, const real* __restrict__ q4,
-                                                     real* __restrict__ uf, real* __restrict__ vf,
-                                                     real* __restrict__ wf, real* __restrict__ Tf,
-                                                     real* __restrict__ pf, double* __restrict__ rho_part)
+#define DUCT_CAS __attribute__((address_space(4)))
+
+// One tile per block.  Blocks are dispatched round-robin over the 8 XCDs
+// (block b runs on XCD b % 8); with xcd_map the blocks of one XCD take one
+// contiguous range of the tile list, so tiles that share stencil halos
+// (neighbours in j, k and i) are read through the same L2.  The grid is the
+// tile high-water mark rounded up to a multiple of 8.
+__device__ __forceinline__ int tile_of_block(const Pool& P, int xcd_map)
 {
-    __shared__ double red[256];
-    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    double rsum = 0.0;
-    if (kk < g.nz && j < g.ny) {
-        const long long q = lidx(g, i, j, kk);
-        const real rut = (real)k.rho_u_Tu, irut = (real)(1.0 / k.rho_u_Tu);
-        real fu, fv, fw, fT, fp;
-        if (i == 0 || i == g.nx - 1) {
-            real T = q4[q], ri = T * irut;          // 1/rho
-            fu = q0[q] * ri; fv = q1[q] * ri; fw = q2[q] * ri; fT = T; fp = q3[q];
-        } else {
-            // neighbour order (-x, -y, -z, centre, +z, +y, +x) as :1329-1331
-            long long r[7];
-            r[0] = q - g.sx;
-            r[1] = lidx(g, i, wrapi(j - 1, g.ny), kk);
-            r[2] = lidx(g, i, j, wrapi(kk - 1, g.nz));
-            r[3] = q;
-            r[4] = lidx(g, i, j, wrapi(kk + 1, g.nz));
-            r[5] = lidx(g, i, wrapi(j + 1, g.ny), kk);
-            r[6] = q + g.sx;
-            real pu[7], pv[7], pw[7], pT[7], pp[7];
-#pragma unroll
-            for (int t = 0; t < 7; ++t) {
-                real T = q4[r[t]], ri = T * irut;
-                pu[t] = q0[r[t]] * ri; pv[t] = q1[r[t]] * ri; pw[t] = q2[r[t]] * ri;
-                pT[t] = T; pp[t] = q3[r[t]];
-            }
-            const real wc = (real)k.wc, iden = (real)(1.0 / (6.0 + k.wc));
-            fu = (pu[0] + pu[1] + pu[2] + wc * pu[3] + pu[4] + pu[5] + pu[6]) * iden;
-            fv = (pv[0] + pv[1] + pv[2] + wc * pv[3] + pv[4] + pv[5] + pv[6]) * iden;
-            fw = (pw[0] + pw[1] + pw[2] + wc * pw[3] + pw[4] + pw[5] + pw[6]) * iden;
-            fT = (pT[0] + pT[1] + pT[2] + wc * pT[3] + pT[4] + pT[5] + pT[6]) * iden;
-            fp = (pp[0] + pp[1] + pp[2] + wc * pp[3] + pp[4] + pp[5] + pp[6]) * iden;
-        }
-        uf[q] = fu; vf[q] = fv; wf[q] = fw; Tf[q] = fT; pf[q] = fp;
-        rsum = (double)(rut / fT);
-        (void)irut;
+    int b = blockIdx.x;
+    if (xcd_map && (gridDim.x & 7) == 0) b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    return b < *(const DUCT_CAS int*)P.ntile ? b : -1;
+}
+
+// Loads through the constant address space: run descriptors, the tile list
+// and the run state are not written while a stencil kernel runs, so loads
+// with a block-uniform address become scalar loads into SGPRs.
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const DUCT_CAS T*)p;
+#else
+    return *p;      // host pass of a device function: never executed
+#endif
+}
+
+// The parts of a run descriptor a stencil kernel uses, held in SGPRs.
+struct SlotView {
+    GridDims g;
+    DuctConsts k;
+    long long off;
+};
+
+__device__ __forceinline__ SlotView slot_view(const Pool& P, int s)
+{
+    const SlotDesc* D = P.desc + s;
+    SlotView v;
+    v.g = ldc(&D->g);
+    v.k = ldc(&D->k);
+    v.off = ldc(&D->off);
+    return v;
+}
+
+struct TilePos {
+    int s, i, j, kk;
+    bool in;
+};
+
+// Decodes tile t for this thread; returns false (uniformly for the block) if
+// the tile is free or its run is inactive or finished.  TK > 0: a tile is
+// TK cells in k by 256/TK rows in j of one x-plane; TK == 0 (flat): 256
+// consecutive cells of one x-plane (no idle lanes when nz is not a multiple
+// of the tile width).
+template <int TK>
+__device__ __forceinline__ bool tile_decode(const Pool& P, int t, TilePos& tp)
+{
+    const int s = ldc(P.tile_slot + t);
+    if (s < 0) return false;
+    const SlotDesc* D = P.desc + s;
+    if (!ldc(&D->active) || ldc(&P.dyn[s].done)) return false;
+    const int lt = t - ldc(&D->tile0);
+    const int tiles_k = ldc(&D->tiles_k);
+    const int ny = ldc(&D->g.ny), nz = ldc(&D->g.nz);
+    tp.s = s;
+    if constexpr (TK == 0) {
+        tp.i = lt / tiles_k;
+        const int p = (lt - tp.i * tiles_k) * 256 + (int)threadIdx.x;
+        tp.in = p < ny * nz;
+        tp.j = p / nz;
+        tp.kk = p - tp.j * nz;
+    } else {
+        constexpr int TJ = 256 / TK;
+        const int tiles_j = ldc(&D->tiles_j);
+        const int kb = lt % tiles_k;
+        const int rest = lt / tiles_k;
+        const int jb = rest % tiles_j;
+        tp.i = rest / tiles_j;
+        tp.kk = kb * TK + (int)(threadIdx.x % TK);
+        tp.j = jb * TJ + (int)(threadIdx.x / TK);
+        tp.in = tp.kk < nz && tp.j < ny;
     }
-    red[tid] = rsum;
+    return true;
+}
+
+__device__ __forceinline__ int line_idx(const GridDims& g, int bi, int j, int kk, int d, int o)
+{
+    if (d == 0) return bi + o * (int)g.sx + j * g.nz + kk;
+    if (d == 1) return bi + wrapi(j + o, g.ny) * g.nz + kk;
+    return bi + j * g.nz + wrapi(kk + o, g.nz);
+}
+
+__device__ __forceinline__ double block_sum256(double v, double* red)
+{
+    red[threadIdx.x] = v;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) red[tid] += red[tid + s];
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (tid == 0) {
-        long long b = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        rho_part[b] = red[0];
+    double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ void block_max2_256(double& a, double& b, double* red)
+{
+    for (int o = 32; o > 0; o >>= 1) { a = fmax(a, __shfl_xor(a, o)); b = fmax(b, __shfl_xor(b, o)); }
+    if ((threadIdx.x & 63) == 0) { red[threadIdx.x >> 6] = a; red[4 + (threadIdx.x >> 6)] = b; }
+    __syncthreads();
+    a = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    b = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// K1: primitives + box filter (:1307-1470 structure), per-tile sums of rho
+// ---------------------------------------------------------------------------
+template <typename real, int TK>
+__global__ void __launch_bounds__(256) k_duct_filter(Pool P, int qin_is_B, int xcd_map)
+{
+    __shared__ double red[256];
+    const int t = tile_of_block(P, xcd_map);
+    TilePos tp;
+    if (t < 0 || !tile_decode<TK>(P, t, tp)) return;
+    {
+        const SlotView D = slot_view(P, tp.s);
+        const GridDims g = D.g;
+        const DuctConsts& k = D.k;
+        void* const* Q = qin_is_B ? P.B : P.A;
+        const real* q0 = (const real*)Q[0] + D.off;
+        const real* q1 = (const real*)Q[1] + D.off;
+        const real* q2 = (const real*)Q[2] + D.off;
+        const real* q3 = (const real*)Q[3] + D.off;
+        const real* q4 = (const real*)Q[4] + D.off;
+        double rsum = 0.0;
+        if (tp.in) {
+            const int i = tp.i, j = tp.j, kk = tp.kk;
+            const int q = (int)lidx(g, i, j, kk);
+            const real rut = (real)k.rho_u_Tu, irut = (real)(1.0 / k.rho_u_Tu);
+            real fu, fv, fw, fT, fp;
+            if (i == 0 || i == g.nx - 1) {
+                real T = q4[q], ri = T * irut;          // 1/rho
+                fu = q0[q] * ri; fv = q1[q] * ri; fw = q2[q] * ri; fT = T; fp = q3[q];
+            } else {
+                // neighbour order (-x, -y, -z, centre, +z, +y, +x) as :1329-1331
+                int r[7];
+                r[0] = q - (int)g.sx;
+                r[1] = (int)lidx(g, i, wrapi(j - 1, g.ny), kk);
+                r[2] = (int)lidx(g, i, j, wrapi(kk - 1, g.nz));
+                r[3] = q;
+                r[4] = (int)lidx(g, i, j, wrapi(kk + 1, g.nz));
+                r[5] = (int)lidx(g, i, wrapi(j + 1, g.ny), kk);
+                r[6] = q + (int)g.sx;
+                real pu[7], pv[7], pw[7], pT[7], pp[7];
+#pragma unroll
+                for (int u = 0; u < 7; ++u) {
+                    real T = q4[r[u]], ri = T * irut;
+                    pu[u] = q0[r[u]] * ri; pv[u] = q1[r[u]] * ri; pw[u] = q2[r[u]] * ri;
+                    pT[u] = T; pp[u] = q3[r[u]];
+                }
+                const real wc = (real)k.wc, iden = (real)(1.0 / (6.0 + k.wc));
+                fu = (pu[0] + pu[1] + pu[2] + wc * pu[3] + pu[4] + pu[5] + pu[6]) * iden;
+                fv = (pv[0] + pv[1] + pv[2] + wc * pv[3] + pv[4] + pv[5] + pv[6]) * iden;
+                fw = (pw[0] + pw[1] + pw[2] + wc * pw[3] + pw[4] + pw[5] + pw[6]) * iden;
+                fT = (pT[0] + pT[1] + pT[2] + wc * pT[3] + pT[4] + pT[5] + pT[6]) * iden;
+                fp = (pp[0] + pp[1] + pp[2] + wc * pp[3] + pp[4] + pp[5] + pp[6]) * iden;
+            }
+            real* F0 = (real*)P.F[0] + D.off;
+            real* F1 = (real*)P.F[1] + D.off;
+            real* F2 = (real*)P.F[2] + D.off;
+            real* F3 = (real*)P.F[3] + D.off;
+            real* F4 = (real*)P.F[4] + D.off;
+            F0[q] = fu; F1[q] = fv; F2[q] = fw; F3[q] = fT; F4[q] = fp;
+            rsum = (double)(rut / fT);
+        }
+        double tot = block_sum256(rsum, red);
+        if (threadIdx.x == 0) P.rho_part[t] = tot;
     }
 }
 
-// deterministic finalisation of the block sums: rho_avg = sum / n
-__global__ void __launch_bounds__(1024) k_duct_rho_final(long long n, int nparts, const double* __restrict__ part,
-                                                         double* __restrict__ scal)
+// per-slot domain-mean density: fixed-order reduction of the slot's tile sums
+__global__ void __launch_bounds__(1024) k_duct_rho_final(Pool P)
 {
     __shared__ double red[1024];
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += 1024) s += part[b];
-    red[threadIdx.x] = s;
+    const int s = blockIdx.x;
+    const SlotDesc& D = P.desc[s];
+    if (!D.active || P.dyn[s].done) return;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < D.ntiles; b += 1024) v += P.rho_part[D.tile0 + b];
+    red[threadIdx.x] = v;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) scal[1] = red[0] / (double)n;
+    if (threadIdx.x == 0) P.dyn[s].rho_avg = red[0] / (double)D.g.n;
 }
 
 // ---------------------------------------------------------------------------
-// K2: properties, nu_t (:1569-1633), S_T (:1913-1965); exact DT maxima
+// K2: properties, nu_t (:1569-1633), S_T (:1913-1965); per-tile DT maxima
 // ---------------------------------------------------------------------------
-template <typename real>
-__global__ void __launch_bounds__(256) k_duct_sgs(GridDims g, DuctConsts k, const double* __restrict__ mu_tab,
-                                                  const double* __restrict__ cp_tab, const real* __restrict__ uf,
-                                                  const real* __restrict__ vf, const real* __restrict__ wf,
-                                                  const real* __restrict__ Tf, const real* __restrict__ pf,
-                                                  real* __restrict__ nut, real* __restrict__ st,
-                                                  double* __restrict__ dpart)
+template <typename real, int TK>
+__global__ void __launch_bounds__(256) k_duct_sgs(Pool P, int stage, int xcd_map)
 {
-    __shared__ double red[2][4];
-    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    double mn = 0.0, ma = 0.0;
-    if (kk < g.nz && j < g.ny) {
-        const long long q = lidx(g, i, j, kk);
-        const real SL = (real)k.s_l;
-        if (i == 0 || i == g.nx - 1) {
-            nut[q] = (real)0.0; st[q] = SL;
-        } else {
-            const real DX = (real)g.dx, DY = (real)g.dy, DZ = (real)g.dz;
-            real T = Tf[q], rf = (real)k.rho_u_Tu / T;
-            real mu = table_lookup<real>(mu_tab, k, T);
-            real cp = table_lookup<real>(cp_tab, k, T);
-            real nu = mu / rf;
-            real lam = mu * cp / (real)k.prandtl;
-            real al = lam / (rf * cp);
-            mn = (double)nu; ma = (double)al;
-            long long xp = q + g.sx, xm = q - g.sx;
-            long long yp = lidx(g, i, wrapi(j + 1, g.ny), kk), ym = lidx(g, i, wrapi(j - 1, g.ny), kk);
-            long long zp = lidx(g, i, j, wrapi(kk + 1, g.nz)), zm = lidx(g, i, j, wrapi(kk - 1, g.nz));
-            const real hx = (real)(0.5 / g.dx), hy = (real)(0.5 / g.dy), hz = (real)(0.5 / g.dz);
-            real du_dx = (uf[xp] - uf[xm]) * hx;
-            real dv_dy = (vf[yp] - vf[ym]) * hy;
-            real dw_dz = (wf[zp] - wf[zm]) * hz;
-            real du_dy = (uf[yp] - uf[ym]) * hy;
-            real du_dz = (uf[zp] - uf[zm]) * hz;
-            real dv_dx = (vf[xp] - vf[xm]) * hx;
-            real dv_dz = (vf[zp] - vf[zm]) * hz;
-            real dw_dx = (wf[xp] - wf[xm]) * hx;
-            real dw_dy = (wf[yp] - wf[ym]) * hy;
-            real S12 = (real)0.5 * (du_dy + dv_dx), S13 = (real)0.5 * (du_dz + dw_dx), S23 = (real)0.5 * (dv_dz + dw_dy);
-            real ss = (real)2.0 * (S12 * S12 + S13 * S13 + S23 * S23) + du_dx * du_dx + dv_dy * dv_dy + dw_dz * dw_dz;
-            real smag = sqrt((real)2.0 * ss);
-            real div = du_dx + dv_dy + dw_dz;
-            real gg = progress<real>(pf[q], DX);
-            real cs = (real)k.cs_u + ((real)k.cs_b - (real)k.cs_u) * gg;
-            cs += (real)k.cs_f * (real)4.0 * gg * ((real)1.0 - gg);
-            nut[q] = (cs * DX) * (cs * DX) * smag;
-            // SSR-SGS flame speed
-            real num = ss - (div * div) * ((real)1.0 / (real)3.0);
-            real X = (pow(DX, (real)4.0) / (nu * nu)) * num;
-            real d_eta;
-            if (X < (real)LES_SMALL) d_eta = (real)1.0;
-            else {
-                real px = pow(X, (real)0.75);
-                real alg = (real)0.2187 * px;
-                d_eta = (real)0.2340 * px * (alg > (real)1.0 ? log(alg) : (real)0.0);
+    __shared__ double red[8];
+    const int t = tile_of_block(P, xcd_map);
+    TilePos tp;
+    if (t < 0 || !tile_decode<TK>(P, t, tp)) return;
+    {
+        const SlotView D = slot_view(P, tp.s);
+        const GridDims g = D.g;
+        const DuctConsts& k = D.k;
+        double mn = 0.0, ma = 0.0;
+        if (tp.in) {
+            const int i = tp.i, j = tp.j, kk = tp.kk;
+            const int q = (int)lidx(g, i, j, kk);
+            const real* uf = (const real*)P.F[0] + D.off;
+            const real* vf = (const real*)P.F[1] + D.off;
+            const real* wf = (const real*)P.F[2] + D.off;
+            const real* Tf = (const real*)P.F[3] + D.off;
+            const real* pf = (const real*)P.F[4] + D.off;
+            real* nut = (real*)P.nut + D.off;
+            real* stv = (real*)P.st + D.off;
+            const real SL = (real)k.s_l;
+            if (i == 0 || i == g.nx - 1) {
+                nut[q] = (real)0.0; stv[q] = SL;
+            } else {
+                const real DX = (real)g.dx;
+                real T = Tf[q], rf = (real)k.rho_u_Tu / T;
+                real mu = table_lookup<real>(P.mu_tab, k, T);
+                real cp = table_lookup<real>(P.cp_tab, k, T);
+                real nu = mu / rf;
+                real lam = mu * cp / (real)k.prandtl;
+                real al = lam / (rf * cp);
+                mn = (double)nu; ma = (double)al;
+                const int xp = q + (int)g.sx, xm = q - (int)g.sx;
+                const int yp = (int)lidx(g, i, wrapi(j + 1, g.ny), kk), ym = (int)lidx(g, i, wrapi(j - 1, g.ny), kk);
+                const int zp = (int)lidx(g, i, j, wrapi(kk + 1, g.nz)), zm = (int)lidx(g, i, j, wrapi(kk - 1, g.nz));
+                const real hx = (real)(0.5 / g.dx), hy = (real)(0.5 / g.dy), hz = (real)(0.5 / g.dz);
+                real du_dx = (uf[xp] - uf[xm]) * hx;
+                real dv_dy = (vf[yp] - vf[ym]) * hy;
+                real dw_dz = (wf[zp] - wf[zm]) * hz;
+                real du_dy = (uf[yp] - uf[ym]) * hy;
+                real du_dz = (uf[zp] - uf[zm]) * hz;
+                real dv_dx = (vf[xp] - vf[xm]) * hx;
+                real dv_dz = (vf[zp] - vf[zm]) * hz;
+                real dw_dx = (wf[xp] - wf[xm]) * hx;
+                real dw_dy = (wf[yp] - wf[ym]) * hy;
+                real S12 = (real)0.5 * (du_dy + dv_dx), S13 = (real)0.5 * (du_dz + dw_dx), S23 = (real)0.5 * (dv_dz + dw_dy);
+                real ss = (real)2.0 * (S12 * S12 + S13 * S13 + S23 * S23) + du_dx * du_dx + dv_dy * dv_dy + dw_dz * dw_dz;
+                real smag = sqrt((real)2.0 * ss);
+                real div = du_dx + dv_dy + dw_dz;
+                real gg = progress<real>(pf[q], DX);
+                real cs = (real)k.cs_u + ((real)k.cs_b - (real)k.cs_u) * gg;
+                cs += (real)k.cs_f * (real)4.0 * gg * ((real)1.0 - gg);
+                nut[q] = (cs * DX) * (cs * DX) * smag;
+                // SSR-SGS flame speed
+                real num = ss - (div * div) * ((real)1.0 / (real)3.0);
+                real X = (pow(DX, (real)4.0) / (nu * nu)) * num;
+                real d_eta;
+                if (X < (real)LES_SMALL) d_eta = (real)1.0;
+                else {
+                    real px = pow(X, (real)0.75);
+                    real alg = (real)0.2187 * px;
+                    d_eta = (real)0.2340 * px * (alg > (real)1.0 ? log(alg) : (real)0.0);
+                }
+                real eta = DX / (d_eta + (real)LES_SMALL);
+                real dF = (real)k.nu_unburnt / (SL + (real)LES_SMALL);
+                real asf = (real)8.0 * exp((real)10.0 * dF / ((real)8.0 * eta + (real)LES_SMALL));
+                real wr = pow(asf * eta / DX, (real)2.0 - (real)k.d3);
+                real te = ((real)k.tau * SL > (real)LES_SMALL) ? DX * fmax((real)0.0, div) / ((real)k.tau * SL) : (real)0.0;
+                stv[q] = fmax(SL, SL * fmax((real)0.0, wr + te));
             }
-            real eta = DX / (d_eta + (real)LES_SMALL);
-            real dF = (real)k.nu_unburnt / (SL + (real)LES_SMALL);
-            real asf = (real)8.0 * exp((real)10.0 * dF / ((real)8.0 * eta + (real)LES_SMALL));
-            real wr = pow(asf * eta / DX, (real)2.0 - (real)k.d3);
-            real te = ((real)k.tau * SL > (real)LES_SMALL) ? DX * fmax((real)0.0, div) / ((real)k.tau * SL) : (real)0.0;
-            st[q] = fmax(SL, SL * fmax((real)0.0, wr + te));
+        }
+        block_max2_256(mn, ma, red);
+        if (threadIdx.x == 0) {
+            double* dp = P.dpart + 2 * ((long long)stage * P.tile_cap + t);
+            dp[0] = mn;
+            dp[1] = ma;
         }
     }
-    // block max (wavefront shuffles, then LDS across the 4 wavefronts); one
-    // slot per block, reduced by k_duct_dt: no atomics, order-independent
-    for (int o = 32; o > 0; o >>= 1) {
-        mn = fmax(mn, __shfl_xor(mn, o));
-        ma = fmax(ma, __shfl_xor(ma, o));
-    }
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    if ((tid & 63) == 0) { red[0][tid >> 6] = mn; red[1][tid >> 6] = ma; }
-    __syncthreads();
-    if (tid == 0) {
-        long long b = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        dpart[2 * b] = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
-        dpart[2 * b + 1] = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
-    }
 }
 
 // ---------------------------------------------------------------------------
-// K3: RHS (:2552-2955 structure) fused with the SSP-RK3 combine
+// K3: RHS fused with the SSP-RK3 combine
 // ---------------------------------------------------------------------------
 template <typename real>
 struct RhsArgs {
-    const real *uf, *vf, *wf, *Tf, *pf, *nut, *st;
-    const real* qin[DNF];
-    const real* q0[DNF];
-    real* qout[DNF];
-    const double* mu_tab;
-    const double* cp_tab;
-    const double* scal;
-    double* vpart;
+    int qin_is_B;        // stage input: B (stages 2, 3) or A (stage 1)
+    int qout_is_B;       // stage output: B (stages 1, 2) or A (stage 3)
     int stage;
-    double ca, cb;            // q_out = ca*q0 + cb*(q_in + dt*L) for stages 1, 2
-    int track_vel;            // last stage: max |u| of the new state
-    real* Lout[DNF];          // optional: the RHS itself (testing hook)
+    double ca, cb;       // q_out = ca*q0 + cb*(q_in + dt*L) for stages 2, 3
+    int track_vel;       // last stage: per-tile max |u| of the new state
+    int xcd_map;
+    real* Lout[DNF];     // optional: the RHS itself, slot-local indices (testing hook)
 };
 
-// Flat index of the point at offset o (-3..3) from cell (i, j, k) along axis d;
-// y and z wrap (periodic duct), x never leaves the domain for the offsets used.
-__device__ __forceinline__ int line_idx(const GridDims& g, int base_i, int j, int kk, int d, int o)
+// scalar RHS (phi: G equation :2706-2818; T :2912-2955) + SSP-RK3 combine
+template <typename real, int TK>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_duct_rhs_scalar(Pool P, RhsArgs<real> a)
 {
-    if (d == 0) return base_i + o * (int)g.sx + j * g.nz + kk;
-    if (d == 1) return base_i + wrapi(j + o, g.ny) * g.nz + kk;
-    return base_i + j * g.nz + wrapi(kk + o, g.nz);
-}
-
-// K3a: scalar RHS (phi: G equation :2706-2818; T :2912-2955) + SSP-RK3 combine
-template <typename real>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_duct_rhs_scalar(GridDims g, DuctConsts k, RhsArgs<real> a)
-{
-    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (kk >= g.nz || j >= g.ny) return;
-    const int bi = i * (int)g.sx;
-    const int q = bi + j * g.nz + kk;
-    if (i == 0 || i == g.nx - 1) {
-        a.qout[3][q] = a.qin[3][q];
-        a.qout[4][q] = a.qin[4][q];
-        return;
-    }
-    const real DX = (real)g.dx, DY = (real)g.dy, DZ = (real)g.dz;
-    const real rut = (real)k.rho_u_Tu;
-    const bool xband = (i < 3 || i > g.nx - 4);
-    const int scheme = k.scheme;
-    const real* vel[3] = {a.uf, a.vf, a.wf};
-    const real T0 = a.Tf[q];
-    const real nu0 = a.nut[q];
-    const int xp = q + (int)g.sx, xm = q - (int)g.sx;
-    const int yp = line_idx(g, bi, j, kk, 1, 1), ym = line_idx(g, bi, j, kk, 1, -1);
-    const int zp = line_idx(g, bi, j, kk, 2, 1), zm = line_idx(g, bi, j, kk, 2, -1);
-    const real* S2[2] = {a.pf, a.Tf};
-    real cv[2], sg[2], lap[2], grad[3];
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        const real* F = S2[s];
-        const real isc2 = (real)(0.5 / ((s == 0) ? k.sc_t : k.pr_t));
-        real acc = (real)0, sacc = (real)0, lacc = (real)0;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double dd64 = d == 0 ? g.dx : (d == 1 ? g.dy : g.dz);
-            const real id = (real)(1.0 / dd64), id2 = (real)(1.0 / (dd64 * dd64)), hid = (real)(0.5 / dd64);
-            const int qp = d == 0 ? xp : (d == 1 ? yp : zp);
-            const int qm = d == 0 ? xm : (d == 1 ? ym : zm);
-            real ua = vel[d][q];
-            real vm1 = F[qm], v0 = F[q], vp1 = F[qp];
-            real t;
-            if (d == 0 && xband) {
-                t = (ua >= (real)0.0) ? ua * (v0 - vm1) * id : ua * (vp1 - v0) * id;
-            } else {
-                real v[7];
-#pragma unroll
-                for (int o = 0; o < 7; ++o) v[o] = F[line_idx(g, bi, j, kk, d, o - 3)];
-                bool pos = ua >= (real)0.0;
-                real fp = face_plus<real>(scheme, v, pos);
-                real fm = face_minus<real>(scheme, v, pos);
-                t = ua * (fp - fm) * id;
-            }
-            acc = (d == 0) ? t : acc + t;
-            real ntp = a.nut[qp], ntm = a.nut[qm];
-            real gp = (ntp + nu0) * isc2, gm = (nu0 + ntm) * isc2;
-            real t2 = (gp * (vp1 - v0) - gm * (v0 - vm1)) * id2;
-            sacc = (d == 0) ? t2 : sacc + t2;
-            real t3 = (vp1 - (real)2.0 * v0 + vm1) * id2;
-            lacc = (d == 0) ? t3 : lacc + t3;
-            if (s == 0) grad[d] = (vp1 - vm1) * hid;
-        }
-        cv[s] = acc; sg[s] = sacc; lap[s] = lacc;
-    }
-    const real g0 = progress<real>(a.pf[q], DX);
-    const real wdt = (real)k.accel_width, ctr = (real)k.accel_center;
-    real gmag = sqrt(grad[0] * grad[0] + grad[1] * grad[1] + grad[2] * grad[2]);
-    real art = (real)0;
-    if (g0 > (real)0.1 && g0 < (real)0.99) {
-        real df = g0 - ctr;
-        real win = exp(-(df * df) / ((real)2.0 * wdt * wdt));
-        art = (real)5.0 * (nu0 + (real)1.0e-5) * win * lap[0];
-    }
-    real L3 = -cv[0] + a.st[q] * gmag + sg[0] + art;
-    const real rf0 = rut / T0;
-    real mu = table_lookup<real>(a.mu_tab, k, T0);
-    real cp = table_lookup<real>(a.cp_tab, k, T0);
-    real lam = mu * cp / (real)k.prandtl;
-    real al = lam / (rf0 * cp);
-    const real Tu = (real)k.Tu, Tb = (real)k.Tb;
-    real Ttg = Tu + g0 * (Tb - Tu);
-    real L4 = -cv[1] + al * lap[1] + sg[1] + (real)k.relaxation * (Ttg - T0);
-    if (a.Lout[0]) { a.Lout[3][q] = L3; a.Lout[4][q] = L4; }
-    const real DT = (real)a.scal[0];
-    real y3 = a.qin[3][q] + DT * L3, y4 = a.qin[4][q] + DT * L4;
-    real v3, v4;
-    if (a.stage == 0) { v3 = y3; v4 = y4; }
-    else { v3 = (real)a.ca * a.q0[3][q] + (real)a.cb * y3; v4 = (real)a.ca * a.q0[4][q] + (real)a.cb * y4; }
-    v4 = fmax((real)200.0, fmin((real)3000.0, v4));
-    a.qout[3][q] = v3;
-    a.qout[4][q] = v4;
-}
-
-// K3b: momentum RHS (:2552-2691 structure) + SSP-RK3 combine; in the last stage
-// also the per-block max |u| of the new state (its T is written by K3a first)
-template <typename real>
-__global__ void __launch_bounds__(256) k_duct_rhs_momentum(GridDims g, DuctConsts k, RhsArgs<real> a)
-{
-    __shared__ double vred[4];
-    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    double vmax = 0.0;
-    if (kk < g.nz && j < g.ny) {
+    const int t = tile_of_block(P, a.xcd_map);
+    TilePos tp;
+    if (t < 0 || !tile_decode<TK>(P, t, tp)) return;
+    {
+        if (!tp.in) return;
+        const SlotView D = slot_view(P, tp.s);
+        const GridDims g = D.g;
+        const DuctConsts& k = D.k;
+        const int i = tp.i, j = tp.j, kk = tp.kk;
         const int bi = i * (int)g.sx;
         const int q = bi + j * g.nz + kk;
+        void* const* QI = a.qin_is_B ? P.B : P.A;
+        void* const* QO = a.qout_is_B ? P.B : P.A;
+        const real* qin3 = (const real*)QI[3] + D.off;
+        const real* qin4 = (const real*)QI[4] + D.off;
+        real* qo3 = (real*)QO[3] + D.off;
+        real* qo4 = (real*)QO[4] + D.off;
         if (i == 0 || i == g.nx - 1) {
+            qo3[q] = qin3[q];
+            qo4[q] = qin4[q];
+            return;
+        }
+        const real* uf = (const real*)P.F[0] + D.off;
+        const real* vf = (const real*)P.F[1] + D.off;
+        const real* wf = (const real*)P.F[2] + D.off;
+        const real* Tf = (const real*)P.F[3] + D.off;
+        const real* pf = (const real*)P.F[4] + D.off;
+        const real* nut = (const real*)P.nut + D.off;
+        const real DX = (real)g.dx;
+        const real rut = (real)k.rho_u_Tu;
+        const bool xband = (i < 3 || i > g.nx - 4);
+        const int scheme = k.scheme;
+        const real* vel[3] = {uf, vf, wf};
+        const real T0 = Tf[q];
+        const real nu0 = nut[q];
+        const int xp = q + (int)g.sx, xm = q - (int)g.sx;
+        const int yp = line_idx(g, bi, j, kk, 1, 1), ym = line_idx(g, bi, j, kk, 1, -1);
+        const int zp = line_idx(g, bi, j, kk, 2, 1), zm = line_idx(g, bi, j, kk, 2, -1);
+        const real* S2[2] = {pf, Tf};
+        real cv[2], sg[2], lap[2], grad[3];
 #pragma unroll
-            for (int f = 0; f < 3; ++f) a.qout[f][q] = a.qin[f][q];
-        } else {
-            const real DX = (real)g.dx, DY = (real)g.dy, DZ = (real)g.dz;
-            const real rut = (real)k.rho_u_Tu;
-            const bool xband = (i < 3 || i > g.nx - 4);
-            const int scheme = k.scheme;
-            const real* vel[3] = {a.uf, a.vf, a.wf};
-            const real T0 = a.Tf[q];
-            const real rf0 = rut / T0;
-            real conv[3] = {(real)0, (real)0, (real)0};
-#pragma unroll 1
+        for (int s = 0; s < 2; ++s) {
+            const real* Fs = S2[s];
+            const real isc2 = (real)(0.5 / ((s == 0) ? k.sc_t : k.pr_t));
+            real acc = (real)0, sacc = (real)0, lacc = (real)0;
+#pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const real id = (real)(1.0 / (d == 0 ? g.dx : (d == 1 ? g.dy : g.dz)));
-                const int qp = line_idx(g, bi, j, kk, d, 1), qm = line_idx(g, bi, j, kk, d, -1);
-                const real* U = vel[d];
-                real up = (real)0.5 * (U[qp] + U[q]);
-                real um = (real)0.5 * (U[q] + U[qm]);
-                real rp = (real)0.5 * (rut / a.Tf[qp] + rf0);
-                real rm = (real)0.5 * (rf0 + rut / a.Tf[qm]);
-                real mp = rp * up, mm = rm * um;
-                const bool band = (d == 0 && xband);
+                const double dd64 = d == 0 ? g.dx : (d == 1 ? g.dy : g.dz);
+                const real id = (real)(1.0 / dd64), id2 = (real)(1.0 / (dd64 * dd64)), hid = (real)(0.5 / dd64);
+                const int qp = d == 0 ? xp : (d == 1 ? yp : zp);
+                const int qm = d == 0 ? xm : (d == 1 ? ym : zm);
+                real ua = vel[d][q];
+                real vm1 = Fs[qm], v0 = Fs[q], vp1 = Fs[qp];
+                real tt;
+                if (d == 0 && xband) {
+                    tt = (ua >= (real)0.0) ? ua * (v0 - vm1) * id : ua * (vp1 - v0) * id;
+                } else {
+                    real v[7];
+#pragma unroll
+                    for (int o = 0; o < 7; ++o) v[o] = Fs[line_idx(g, bi, j, kk, d, o - 3)];
+                    bool pos = ua >= (real)0.0;
+                    real fp = face_plus<real>(scheme, v, pos);
+                    real fm = face_minus<real>(scheme, v, pos);
+                    tt = ua * (fp - fm) * id;
+                }
+                acc = (d == 0) ? tt : acc + tt;
+                real ntp = nut[qp], ntm = nut[qm];
+                real gp = (ntp + nu0) * isc2, gm = (nu0 + ntm) * isc2;
+                real t2 = (gp * (vp1 - v0) - gm * (v0 - vm1)) * id2;
+                sacc = (d == 0) ? t2 : sacc + t2;
+                real t3 = (vp1 - (real)2.0 * v0 + vm1) * id2;
+                lacc = (d == 0) ? t3 : lacc + t3;
+                if (s == 0) grad[d] = (vp1 - vm1) * hid;
+            }
+            cv[s] = acc; sg[s] = sacc; lap[s] = lacc;
+        }
+        const real g0 = progress<real>(pf[q], DX);
+        const real wdt = (real)k.accel_width, ctr = (real)k.accel_center;
+        real gmag = sqrt(grad[0] * grad[0] + grad[1] * grad[1] + grad[2] * grad[2]);
+        real art = (real)0;
+        if (g0 > (real)0.1 && g0 < (real)0.99) {
+            real df = g0 - ctr;
+            real win = exp(-(df * df) / ((real)2.0 * wdt * wdt));
+            art = (real)5.0 * (nu0 + (real)1.0e-5) * win * lap[0];
+        }
+        const real* stv = (const real*)P.st + D.off;
+        real L3 = -cv[0] + stv[q] * gmag + sg[0] + art;
+        const real rf0 = rut / T0;
+        real mu = table_lookup<real>(P.mu_tab, k, T0);
+        real cp = table_lookup<real>(P.cp_tab, k, T0);
+        real lam = mu * cp / (real)k.prandtl;
+        real al = lam / (rf0 * cp);
+        const real Tu = (real)k.Tu, Tb = (real)k.Tb;
+        real Ttg = Tu + g0 * (Tb - Tu);
+        real L4 = -cv[1] + al * lap[1] + sg[1] + (real)k.relaxation * (Ttg - T0);
+        if (a.Lout[0]) { a.Lout[3][q] = L3; a.Lout[4][q] = L4; }
+        const real DT = (real)ldc(&P.dyn[tp.s].dt);
+        real y3 = qin3[q] + DT * L3, y4 = qin4[q] + DT * L4;
+        real v3, v4;
+        if (a.stage == 0) { v3 = y3; v4 = y4; }
+        else {
+            const real* qa3 = (const real*)P.A[3] + D.off;
+            const real* qa4 = (const real*)P.A[4] + D.off;
+            v3 = (real)a.ca * qa3[q] + (real)a.cb * y3;
+            v4 = (real)a.ca * qa4[q] + (real)a.cb * y4;
+        }
+        v4 = fmax((real)200.0, fmin((real)3000.0, v4));
+        qo3[q] = v3;
+        qo4[q] = v4;
+    }
+}
+
+// momentum RHS (:2552-2691 structure) + SSP-RK3 combine; in the last stage also
+// the per-tile max |u| of the new state (its T was written by the scalar kernel)
+template <typename real, int TK>
+__global__ void __launch_bounds__(256) k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
+{
+    __shared__ double vred[8];
+    const int t = tile_of_block(P, a.xcd_map);
+    TilePos tp;
+    if (t < 0 || !tile_decode<TK>(P, t, tp)) return;
+    {
+        const SlotView D = slot_view(P, tp.s);
+        const GridDims g = D.g;
+        const DuctConsts& k = D.k;
+        double vmax = 0.0;
+        if (tp.in) {
+            const int i = tp.i, j = tp.j, kk = tp.kk;
+            const int bi = i * (int)g.sx;
+            const int q = bi + j * g.nz + kk;
+            void* const* QI = a.qin_is_B ? P.B : P.A;
+            void* const* QO = a.qout_is_B ? P.B : P.A;
+            const real* qi[3] = {(const real*)QI[0] + D.off, (const real*)QI[1] + D.off, (const real*)QI[2] + D.off};
+            real* qo[3] = {(real*)QO[0] + D.off, (real*)QO[1] + D.off, (real*)QO[2] + D.off};
+            if (i == 0 || i == g.nx - 1) {
+#pragma unroll
+                for (int f = 0; f < 3; ++f) qo[f][q] = qi[f][q];
+            } else {
+                const real* uf = (const real*)P.F[0] + D.off;
+                const real* vf = (const real*)P.F[1] + D.off;
+                const real* wf = (const real*)P.F[2] + D.off;
+                const real* Tf = (const real*)P.F[3] + D.off;
+                const real* pf = (const real*)P.F[4] + D.off;
+                const real* nut = (const real*)P.nut + D.off;
+                const real DX = (real)g.dx;
+                const real rut = (real)k.rho_u_Tu;
+                const bool xband = (i < 3 || i > g.nx - 4);
+                const int scheme = k.scheme;
+                const real* vel[3] = {uf, vf, wf};
+                const real T0 = Tf[q];
+                const real rf0 = rut / T0;
+                real conv[3] = {(real)0, (real)0, (real)0};
+#pragma unroll 1
+                for (int d = 0; d < 3; ++d) {
+                    const real id = (real)(1.0 / (d == 0 ? g.dx : (d == 1 ? g.dy : g.dz)));
+                    const int qp = line_idx(g, bi, j, kk, d, 1), qm = line_idx(g, bi, j, kk, d, -1);
+                    const real* U = vel[d];
+                    real up = (real)0.5 * (U[qp] + U[q]);
+                    real um = (real)0.5 * (U[q] + U[qm]);
+                    real rp = (real)0.5 * (rut / Tf[qp] + rf0);
+                    real rm = (real)0.5 * (rf0 + rut / Tf[qm]);
+                    real mp = rp * up, mm = rm * um;
+                    const bool band = (d == 0 && xband);
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) {
+                        const real* V = vel[m];
+                        real fp, fm;
+                        if (band) {
+                            real v2 = V[qm], v3 = V[q], v4 = V[qp];
+                            fp = (up >= (real)0.0) ? v3 : v4;
+                            fm = (um >= (real)0.0) ? v2 : v3;
+                        } else {
+                            real v[7];
+#pragma unroll
+                            for (int o = 0; o < 7; ++o) v[o] = V[line_idx(g, bi, j, kk, d, o - 3)];
+                            fp = face_plus<real>(scheme, v, up >= (real)0.0);
+                            fm = face_minus<real>(scheme, v, um >= (real)0.0);
+                        }
+                        real term = (mp * fp - mm * fm) * id;
+                        conv[m] = (d == 0) ? term : conv[m] + term;
+                    }
+                }
+                real mu = table_lookup<real>(P.mu_tab, k, T0);
+                real nu0 = nut[q];
+                real mu_eff = mu + rf0 * nu0;
+                const int xp = q + (int)g.sx, xm = q - (int)g.sx;
+                const int yp = line_idx(g, bi, j, kk, 1, 1), ym = line_idx(g, bi, j, kk, 1, -1);
+                const int zp = line_idx(g, bi, j, kk, 2, 1), zm = line_idx(g, bi, j, kk, 2, -1);
+                const real ix2 = (real)(1.0 / (g.dx * g.dx)), iy2 = (real)(1.0 / (g.dy * g.dy)), iz2 = (real)(1.0 / (g.dz * g.dz));
+                real diff[3];
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
-                    const real* V = vel[m];
-                    real fp, fm;
-                    if (band) {
-                        real v2 = V[qm], v3 = V[q], v4 = V[qp];
-                        fp = (up >= (real)0.0) ? v3 : v4;
-                        fm = (um >= (real)0.0) ? v2 : v3;
-                    } else {
-                        real v[7];
+                    const real* U = vel[m];
+                    real uc = U[q];
+                    real lx = (U[xp] - (real)2.0 * uc + U[xm]) * ix2;
+                    real ly = (U[yp] - (real)2.0 * uc + U[ym]) * iy2;
+                    real lz = (U[zp] - (real)2.0 * uc + U[zm]) * iz2;
+                    diff[m] = mu_eff * (lx + ly + lz);
+                }
+                const real rho_avg = (real)ldc(&P.dyn[tp.s].rho_avg);
+                real grav = (rf0 - rho_avg) * (real)LES_G_GRAV;
+                real g0 = progress<real>(pf[q], DX);
+                real fx = (real)0, fy = (real)0, fz = (real)0;
+                const real wdt = (real)k.accel_width, ctr = (real)k.accel_center, Fm = (real)k.max_force;
+                if (g0 > (real)0.01 && g0 < (real)0.99) {
+                    real gx = (progress<real>(pf[xp], DX) - progress<real>(pf[xm], DX)) * (real)(0.5 / g.dx);
+                    real gy = (progress<real>(pf[yp], DX) - progress<real>(pf[ym], DX)) * (real)(0.5 / g.dy);
+                    real gz = (progress<real>(pf[zp], DX) - progress<real>(pf[zm], DX)) * (real)(0.5 / g.dz);
+                    real df = g0 - ctr;
+                    real win = exp(-(df * df) / ((real)2.0 * wdt * wdt));
+                    real common = (real)k.accel_strength * rf0 * win;
+                    fx = common * gx * (real)k.weight_x;
+                    fy = common * gy;
+                    fz = common * gz;
+                }
+                fx = fmax(-Fm, fmin(Fm, fx));
+                fy = fmax(-Fm, fmin(Fm, fy));
+                fz = fmax(-Fm, fmin(Fm, fz));
+                real Lq[3];
+                Lq[0] = -conv[0] + diff[0] + grav + fx;
+                Lq[1] = -conv[1] + diff[1] + fy;
+                Lq[2] = -conv[2] + diff[2] + fz;
+                if (a.Lout[0]) {
 #pragma unroll
-                        for (int o = 0; o < 7; ++o) v[o] = V[line_idx(g, bi, j, kk, d, o - 3)];
-                        fp = face_plus<real>(scheme, v, up >= (real)0.0);
-                        fm = face_minus<real>(scheme, v, um >= (real)0.0);
-                    }
-                    real term = (mp * fp - mm * fm) * id;
-                    conv[m] = (d == 0) ? term : conv[m] + term;
+                    for (int f = 0; f < 3; ++f) a.Lout[f][q] = Lq[f];
+                }
+                const real DT = (real)ldc(&P.dyn[tp.s].dt);
+                real out[3];
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    real y = qi[f][q] + DT * Lq[f];
+                    if (a.stage == 0) out[f] = y;
+                    else out[f] = (real)a.ca * ((const real*)P.A[f] + D.off)[q] + (real)a.cb * y;
+                    qo[f][q] = out[f];
+                }
+                if (a.track_vel) {
+                    real rho = rut / ((const real*)QO[4] + D.off)[q];
+                    real uu = out[0] / rho, vv = out[1] / rho, ww = out[2] / rho;
+                    vmax = (double)sqrt(uu * uu + vv * vv + ww * ww);
                 }
             }
-            real mu = table_lookup<real>(a.mu_tab, k, T0);
-            real nu0 = a.nut[q];
-            real mu_eff = mu + rf0 * nu0;
-            const int xp = q + (int)g.sx, xm = q - (int)g.sx;
-            const int yp = line_idx(g, bi, j, kk, 1, 1), ym = line_idx(g, bi, j, kk, 1, -1);
-            const int zp = line_idx(g, bi, j, kk, 2, 1), zm = line_idx(g, bi, j, kk, 2, -1);
-            const real ix2 = (real)(1.0 / (g.dx * g.dx)), iy2 = (real)(1.0 / (g.dy * g.dy)), iz2 = (real)(1.0 / (g.dz * g.dz));
-            real diff[3];
-#pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                const real* U = vel[m];
-                real uc = U[q];
-                real lx = (U[xp] - (real)2.0 * uc + U[xm]) * ix2;
-                real ly = (U[yp] - (real)2.0 * uc + U[ym]) * iy2;
-                real lz = (U[zp] - (real)2.0 * uc + U[zm]) * iz2;
-                diff[m] = mu_eff * (lx + ly + lz);
-            }
-            const real rho_avg = (real)a.scal[1];
-            real grav = (rf0 - rho_avg) * (real)LES_G_GRAV;
-            const real* P = a.pf;
-            real g0 = progress<real>(P[q], DX);
-            real fx = (real)0, fy = (real)0, fz = (real)0;
-            const real wdt = (real)k.accel_width, ctr = (real)k.accel_center, Fm = (real)k.max_force;
-            if (g0 > (real)0.01 && g0 < (real)0.99) {
-                real gx = (progress<real>(P[xp], DX) - progress<real>(P[xm], DX)) * (real)(0.5 / g.dx);
-                real gy = (progress<real>(P[yp], DX) - progress<real>(P[ym], DX)) * (real)(0.5 / g.dy);
-                real gz = (progress<real>(P[zp], DX) - progress<real>(P[zm], DX)) * (real)(0.5 / g.dz);
-                real df = g0 - ctr;
-                real win = exp(-(df * df) / ((real)2.0 * wdt * wdt));
-                real common = (real)k.accel_strength * rf0 * win;
-                fx = common * gx * (real)k.weight_x;
-                fy = common * gy;
-                fz = common * gz;
-            }
-            fx = fmax(-Fm, fmin(Fm, fx));
-            fy = fmax(-Fm, fmin(Fm, fy));
-            fz = fmax(-Fm, fmin(Fm, fz));
-            real Lq[3];
-            Lq[0] = -conv[0] + diff[0] + grav + fx;
-            Lq[1] = -conv[1] + diff[1] + fy;
-            Lq[2] = -conv[2] + diff[2] + fz;
-            if (a.Lout[0]) {
-#pragma unroll
-                for (int f = 0; f < 3; ++f) a.Lout[f][q] = Lq[f];
-            }
-            const real DT = (real)a.scal[0];
-            real out[3];
-#pragma unroll
-            for (int f = 0; f < 3; ++f) {
-                real y = a.qin[f][q] + DT * Lq[f];
-                out[f] = (a.stage == 0) ? y : (real)a.ca * a.q0[f][q] + (real)a.cb * y;
-                a.qout[f][q] = out[f];
-            }
-            if (a.track_vel) {
-                real rho = rut / a.qout[4][q];
-                real uu = out[0] / rho, vv = out[1] / rho, ww = out[2] / rho;
-                vmax = (double)sqrt(uu * uu + vv * vv + ww * ww);
-            }
         }
-    }
-    if (a.track_vel) {
-        for (int o = 32; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o));
-        const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-        if ((tid & 63) == 0) vred[tid >> 6] = vmax;
-        __syncthreads();
-        if (tid == 0) {
-            long long b = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-            a.vpart[b] = fmax(fmax(vred[0], vred[1]), fmax(vred[2], vred[3]));
+        if (a.track_vel) {
+            double dummy = 0.0;
+            block_max2_256(vmax, dummy, vred);
+            if (threadIdx.x == 0) P.vpart[t] = vmax;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// DT (:4399-4411 structure): one block reduces the per-block maxima of the
-// previous step (max |u| of the new state, max nu / alpha over its 3 stages)
+// DT per slot (:4399-4411 structure): one block per slot reduces the per-tile
+// maxima of the previous step (max |u| of the new state, max nu / alpha over
+// its 3 stages) and the inflow-plane maxima.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_duct_dt(double dx, DuctConsts k, double* scal, const double* vpart,
-                                                  int nv, const double* dpart, int nd)
+__global__ void __launch_bounds__(1024) k_duct_dt(Pool P)
 {
     __shared__ double r[3][1024];
+    const int s = blockIdx.x;
+    const SlotDesc& D = P.desc[s];
+    if (!D.active || P.dyn[s].done) return;
+    const DuctConsts& k = D.k;
     double mv = 0.0, mn = 0.0, ma = 0.0;
-    for (int b = threadIdx.x; b < nv; b += 1024) mv = fmax(mv, vpart[b]);
-    for (int b = threadIdx.x; b < nd; b += 1024) { mn = fmax(mn, dpart[2 * b]); ma = fmax(ma, dpart[2 * b + 1]); }
+    for (int b = threadIdx.x; b < D.ntiles; b += 1024) {
+        const int t = D.tile0 + b;
+        mv = fmax(mv, P.vpart[t]);
+        for (int stg = 0; stg < 3; ++stg) {
+            const double* dp = P.dpart + 2 * ((long long)stg * P.tile_cap + t);
+            mn = fmax(mn, dp[0]);
+            ma = fmax(ma, dp[1]);
+        }
+    }
+    const int nbc = P.ny_max * P.nzb;
+    for (int b = threadIdx.x; b < nbc; b += 1024) mv = fmax(mv, P.bcpart[(long long)s * nbc + b]);
     r[0][threadIdx.x] = mv; r[1][threadIdx.x] = mn; r[2][threadIdx.x] = ma;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w)
-            for (int a = 0; a < 3; ++a) r[a][threadIdx.x] = fmax(r[a][threadIdx.x], r[a][threadIdx.x + w]);
+            for (int c = 0; c < 3; ++c) r[c][threadIdx.x] = fmax(r[c][threadIdx.x], r[c][threadIdx.x + w]);
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
     mv = r[0][0]; mn = r[1][0]; ma = r[2][0];
+    const double dx = D.g.dx;
     double maxvel = mv > LES_SMALL ? mv : k.U;
     double dtc = k.courant * dx / (maxvel + LES_SMALL);
     double dtn = k.courant_diff * (dx * dx) / (mn + LES_SMALL);
     double dta = k.courant_diff * (dx * dx) / (ma + LES_SMALL);
-    scal[0] = fmin(dtc, fmin(dtn, dta));
-    scal[2] = maxvel;
+    P.dyn[s].dt = fmin(dtc, fmin(dtn, dta));
+    P.dyn[s].maxvel = maxvel;
 }
 
-// max |u| over planes 0..nx-2 of a state (used after upload / init)
-template <typename real>
-__global__ void __launch_bounds__(256) k_duct_velmax(GridDims g, double rho_u_Tu, const real* q0, const real* q1,
-                                                     const real* q2, const real* q4, double* vpart)
+// max |u| per tile of a state in A (after an upload or init)
+template <typename real, int TK>
+__global__ void __launch_bounds__(256) k_duct_velmax(Pool P, int only_slot, int xcd_map)
 {
-    __shared__ double vred[4];
-    double vm = 0.0;
-    long long lim = (long long)(g.nx - 1) * g.sx;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < lim; q += (long long)gridDim.x * blockDim.x) {
-        real rho = (real)rho_u_Tu / q4[q];
-        real u = q0[q] / rho, v = q1[q] / rho, w = q2[q] / rho;
-        vm = fmax(vm, (double)sqrt(u * u + v * v + w * w));
+    __shared__ double red[8];
+    const int t = tile_of_block(P, xcd_map);
+    TilePos tp;
+    if (t < 0 || !tile_decode<TK>(P, t, tp)) return;
+    {
+        if (tp.s != only_slot) return;
+        const SlotView D = slot_view(P, tp.s);
+        double vm = 0.0;
+        if (tp.in && tp.i <= D.g.nx - 2) {
+            const int q = (int)lidx(D.g, tp.i, tp.j, tp.kk);
+            real rho = (real)D.k.rho_u_Tu / ((const real*)P.A[4] + D.off)[q];
+            real u = ((const real*)P.A[0] + D.off)[q] / rho;
+            real v = ((const real*)P.A[1] + D.off)[q] / rho;
+            real w = ((const real*)P.A[2] + D.off)[q] / rho;
+            vm = (double)sqrt(u * u + v * v + w * w);
+        }
+        double dummy = 0.0;
+        block_max2_256(vm, dummy, red);
+        if (threadIdx.x == 0) {
+            P.vpart[t] = vm;
+            for (int stg = 0; stg < 3; ++stg) {
+                double* dp = P.dpart + 2 * ((long long)stg * P.tile_cap + t);
+                dp[0] = 0.0;
+                dp[1] = 0.0;
+            }
+        }
     }
-    for (int o = 32; o > 0; o >>= 1) vm = fmax(vm, __shfl_xor(vm, o));
-    if ((threadIdx.x & 63) == 0) vred[threadIdx.x >> 6] = vm;
-    __syncthreads();
-    if (threadIdx.x == 0) vpart[blockIdx.x] = fmax(fmax(vred[0], vred[1]), fmax(vred[2], vred[3]));
 }
 
 // ---------------------------------------------------------------------------
-// SEM: eddy advection/recycling (1 block), Gaussian plane, normalisation
+// SEM: eddy advection/recycling, Gaussian plane, normalisation
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void eddy_draw_dev(const DuctConsts& k, double* e, int idx, unsigned long long gen, bool initial)
 {
@@ -648,39 +841,49 @@ __device__ __forceinline__ void eddy_draw_dev(const DuctConsts& k, double* e, in
     e[6] = (double)gen;
 }
 
-__global__ void k_duct_eddy_init(DuctConsts k, double* eddy)
+__global__ void k_duct_eddy_init(Pool P, int s)
 {
+    const SlotDesc& D = P.desc[s];
     int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= k.nvort) return;
-    eddy_draw_dev(k, eddy + 7 * (long long)e, e, 0ull, true);
+    if (e >= D.k.nvort) return;
+    eddy_draw_dev(D.k, P.eddy + 7 * (D.eoff + e), e, 0ull, true);
 }
 
-__global__ void k_duct_eddy_update(DuctConsts k, const double* scal, double* eddy)
+__global__ void k_duct_eddy_update(Pool P)
 {
+    const int s = blockIdx.y;
+    const SlotDesc& D = P.desc[s];
+    if (!D.active || P.dyn[s].done) return;
     int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= k.nvort) return;
-    double* E = eddy + 7 * (long long)e;
-    E[0] += k.U * scal[0];
-    if (E[0] > k.L) {
-        E[0] -= 2.0 * k.L;
-        eddy_draw_dev(k, E, e, (unsigned long long)E[6] + 1ull, false);
+    if (e >= D.k.nvort) return;
+    double* E = P.eddy + 7 * (D.eoff + e);
+    E[0] += D.k.U * P.dyn[s].dt;
+    if (E[0] > D.k.L) {
+        E[0] -= 2.0 * D.k.L;
+        eddy_draw_dev(D.k, E, e, (unsigned long long)E[6] + 1ull, false);
     }
 }
 
-// One thread per inflow cell; the eddy table is staged through LDS in chunks.
-__global__ void __launch_bounds__(256) k_duct_sem_plane(GridDims g, DuctConsts k, const double* __restrict__ eddy,
-                                                        double* __restrict__ inlet)
+// One thread per inflow cell; the slot's eddy table is staged through LDS.
+__global__ void __launch_bounds__(256) k_duct_sem_plane(Pool P)
 {
     __shared__ double se[256 * 7];
-    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.z;
+    const SlotDesc& D = P.desc[s];
+    if (!D.active || P.dyn[s].done) return;
+    const GridDims& g = D.g;
+    const DuctConsts& k = D.k;
     const int j = blockIdx.y;
-    const long long P = (long long)g.ny * g.nz;
+    if (j >= g.ny) return;
+    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long PL = (long long)g.ny * g.nz;
     const double y = j * g.dy, z = kk * g.dz, inv_l2 = 1.0 / (k.L * k.L);
+    const double* eddy = P.eddy + 7 * D.eoff;
     double su = 0.0, sv = 0.0, sw = 0.0;
     for (int base = 0; base < k.nvort; base += 256) {
         int cnt = min(256, k.nvort - base);
         __syncthreads();
-        for (int t = threadIdx.x; t < cnt * 7; t += blockDim.x) se[t] = eddy[7 * (long long)base + t];
+        for (int u = threadIdx.x; u < cnt * 7; u += blockDim.x) se[u] = eddy[7 * (long long)base + u];
         __syncthreads();
         if (kk < g.nz) {
             for (int e = 0; e < cnt; ++e) {
@@ -696,28 +899,34 @@ __global__ void __launch_bounds__(256) k_duct_sem_plane(GridDims g, DuctConsts k
         }
     }
     if (kk < g.nz) {
+        double* in = P.inlet + D.poff;
         long long p = (long long)j * g.nz + kk;
-        inlet[p] = su; inlet[P + p] = sv; inlet[2 * P + p] = sw;
+        in[p] = su; in[PL + p] = sv; in[2 * PL + p] = sw;
     }
 }
 
-// mean removal and rms rescale of the three planes, one 1024-thread block each
-__global__ void __launch_bounds__(1024) k_duct_sem_norm(long long P, double target, double* __restrict__ inlet)
+// mean removal and rms rescale, one 1024-thread block per (component, slot)
+__global__ void __launch_bounds__(1024) k_duct_sem_norm(Pool P)
 {
     __shared__ double red[1024];
-    double* f = inlet + blockIdx.x * P;
-    double s = 0.0;
-    for (long long p = threadIdx.x; p < P; p += 1024) s += f[p];
-    red[threadIdx.x] = s;
+    const int s = blockIdx.y;
+    const SlotDesc& D = P.desc[s];
+    if (!D.active || P.dyn[s].done) return;
+    const long long PL = (long long)D.g.ny * D.g.nz;
+    const double target = D.k.U * D.k.intensity;
+    double* f = P.inlet + D.poff + blockIdx.x * PL;
+    double sm = 0.0;
+    for (long long p = threadIdx.x; p < PL; p += 1024) sm += f[p];
+    red[threadIdx.x] = sm;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    double mean = red[0] / (double)P;
+    double mean = red[0] / (double)PL;
     __syncthreads();
     double qq = 0.0;
-    for (long long p = threadIdx.x; p < P; p += 1024) {
+    for (long long p = threadIdx.x; p < PL; p += 1024) {
         double v = f[p] - mean;
         f[p] = v;
         qq += v * v;
@@ -728,26 +937,35 @@ __global__ void __launch_bounds__(1024) k_duct_sem_norm(long long P, double targ
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    double rms = sqrt(red[0] / (double)P);
+    double rms = sqrt(red[0] / (double)PL);
     double sc = (rms > LES_SMALL) ? target / rms : 0.0;
-    for (long long p = threadIdx.x; p < P; p += 1024) f[p] *= sc;
+    for (long long p = threadIdx.x; p < PL; p += 1024) f[p] *= sc;
 }
 
 // inflow plane from the SEM planes, zero-gradient outflow plane
 template <typename real>
-__global__ void __launch_bounds__(256) k_duct_bc(GridDims g, DuctConsts k, const double* __restrict__ inlet,
-                                                 real* q0, real* q1, real* q2, real* q3, real* q4,
-                                                 double* vpart)
+__global__ void __launch_bounds__(256) k_duct_bc(Pool P)
 {
-    __shared__ double vred[4];
-    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double red[8];
+    const int s = blockIdx.z;
+    const SlotDesc& D = P.desc[s];
+    if (!D.active || P.dyn[s].done) return;
+    const GridDims& g = D.g;
+    const DuctConsts& k = D.k;
     const int j = blockIdx.y;
+    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
     double vm = 0.0;
-    if (kk < g.nz) {
-        const long long P = (long long)g.ny * g.nz;
+    if (j < g.ny && kk < g.nz) {
+        real* q0 = (real*)P.A[0] + D.off;
+        real* q1 = (real*)P.A[1] + D.off;
+        real* q2 = (real*)P.A[2] + D.off;
+        real* q3 = (real*)P.A[3] + D.off;
+        real* q4 = (real*)P.A[4] + D.off;
+        const double* in = P.inlet + D.poff;
+        const long long PL = (long long)g.ny * g.nz;
         const long long p = (long long)j * g.nz + kk;
         const double rho_u = k.rho_u_Tu / k.Tu;
-        const double u = k.U + inlet[p], v = inlet[P + p], w = inlet[2 * P + p];
+        const double u = k.U + in[p], v = in[PL + p], w = in[2 * PL + p];
         q0[p] = (real)(rho_u * u);
         q1[p] = (real)(rho_u * v);
         q2[p] = (real)(rho_u * w);
@@ -759,144 +977,199 @@ __global__ void __launch_bounds__(256) k_duct_bc(GridDims g, DuctConsts k, const
         const long long qn = (long long)(g.nx - 1) * g.sx + p, qm = qn - g.sx;
         q0[qn] = q0[qm]; q1[qn] = q1[qm]; q2[qn] = q2[qm]; q3[qn] = q3[qm]; q4[qn] = q4[qm];
     }
-    for (int o = 32; o > 0; o >>= 1) vm = fmax(vm, __shfl_xor(vm, o));
-    if ((threadIdx.x & 63) == 0) vred[threadIdx.x >> 6] = vm;
-    __syncthreads();
-    if (threadIdx.x == 0) vpart[(long long)blockIdx.y * gridDim.x + blockIdx.x] = fmax(fmax(vred[0], vred[1]), fmax(vred[2], vred[3]));
+    double dummy = 0.0;
+    block_max2_256(vm, dummy, red);
+    if (threadIdx.x == 0)
+        P.bcpart[(long long)s * P.ny_max * P.nzb + (long long)blockIdx.y * P.nzb + blockIdx.x] = vm;
 }
 
 // ---------------------------------------------------------------------------
-// level-set reinitialisation (Sussman-Smereka-Osher, first-order Godunov)
+// level-set reinitialisation (Sussman-Smereka-Osher, first-order Godunov) of
+// the slots due this step; phi0 in B[3], iterate A[3] -> B[4] -> A[3]
 // ---------------------------------------------------------------------------
-template <typename real>
-__global__ void __launch_bounds__(256) k_duct_reinit(GridDims g, const real* __restrict__ phi0,
-                                                     const real* __restrict__ phi, real* __restrict__ out)
+__device__ __forceinline__ bool reinit_due(const SlotView& D, int step, int iter)
 {
-    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (kk >= g.nz || j >= g.ny) return;
-    const long long q = lidx(g, i, j, kk);
-    if (i == 0 || i == g.nx - 1) { out[q] = phi[q]; return; }
-    const real dx = (real)g.dx, dy = (real)g.dy, dz = (real)g.dz, dtau = (real)0.3 * dx;
-    real p0 = phi0[q];
-    real s = p0 / sqrt(p0 * p0 + dx * dx);
-    real c = phi[q];
-    real ax = (c - phi[q - g.sx]) / dx, bx = (phi[q + g.sx] - c) / dx;
-    real ay = (c - phi[lidx(g, i, wrapi(j - 1, g.ny), kk)]) / dy, by = (phi[lidx(g, i, wrapi(j + 1, g.ny), kk)] - c) / dy;
-    real az = (c - phi[lidx(g, i, j, wrapi(kk - 1, g.nz))]) / dz, bz = (phi[lidx(g, i, j, wrapi(kk + 1, g.nz))] - c) / dz;
-    real gx, gy, gz;
-    if (p0 > (real)0.0) {
-        real a1 = fmax(ax, (real)0), b1 = fmin(bx, (real)0), a2 = fmax(ay, (real)0), b2 = fmin(by, (real)0);
-        real a3 = fmax(az, (real)0), b3 = fmin(bz, (real)0);
-        gx = fmax(a1 * a1, b1 * b1); gy = fmax(a2 * a2, b2 * b2); gz = fmax(a3 * a3, b3 * b3);
-    } else {
-        real a1 = fmin(ax, (real)0), b1 = fmax(bx, (real)0), a2 = fmin(ay, (real)0), b2 = fmax(by, (real)0);
-        real a3 = fmin(az, (real)0), b3 = fmax(bz, (real)0);
-        gx = fmax(a1 * a1, b1 * b1); gy = fmax(a2 * a2, b2 * b2); gz = fmax(a3 * a3, b3 * b3);
+    return D.k.reinit_interval > 0 && (step + 1) % D.k.reinit_interval == 0 && iter < D.k.reinit_iters;
+}
+
+template <typename real, int TK>
+__global__ void __launch_bounds__(256) k_duct_reinit(Pool P, int phase, int iter, int xcd_map)
+{
+    const int t = tile_of_block(P, xcd_map);
+    TilePos tp;
+    if (t < 0 || !tile_decode<TK>(P, t, tp)) return;
+    {
+        const SlotView D = slot_view(P, tp.s);
+        if (!reinit_due(D, ldc(&P.dyn[tp.s].step), iter) || !tp.in) return;
+        const GridDims g = D.g;
+        const int i = tp.i, j = tp.j, kk = tp.kk;
+        const int q = (int)lidx(g, i, j, kk);
+        real* phi = (real*)P.A[3] + D.off;
+        real* phi0 = (real*)P.B[3] + D.off;
+        real* tmp = (real*)P.B[4] + D.off;
+        if (phase == 0) {             // save phi0 (first iteration only)
+            if (iter == 0) phi0[q] = phi[q];
+            return;
+        }
+        if (phase == 2) {             // copy back
+            phi[q] = tmp[q];
+            return;
+        }
+        if (i == 0 || i == g.nx - 1) { tmp[q] = phi[q]; return; }
+        const real dx = (real)g.dx, dy = (real)g.dy, dz = (real)g.dz, dtau = (real)0.3 * dx;
+        real p0 = phi0[q];
+        real sgn = p0 / sqrt(p0 * p0 + dx * dx);
+        real c = phi[q];
+        real ax = (c - phi[q - (int)g.sx]) / dx, bx = (phi[q + (int)g.sx] - c) / dx;
+        real ay = (c - phi[lidx(g, i, wrapi(j - 1, g.ny), kk)]) / dy, by = (phi[lidx(g, i, wrapi(j + 1, g.ny), kk)] - c) / dy;
+        real az = (c - phi[lidx(g, i, j, wrapi(kk - 1, g.nz))]) / dz, bz = (phi[lidx(g, i, j, wrapi(kk + 1, g.nz))] - c) / dz;
+        real gx, gy, gz;
+        if (p0 > (real)0.0) {
+            real a1 = fmax(ax, (real)0), b1 = fmin(bx, (real)0), a2 = fmax(ay, (real)0), b2 = fmin(by, (real)0);
+            real a3 = fmax(az, (real)0), b3 = fmin(bz, (real)0);
+            gx = fmax(a1 * a1, b1 * b1); gy = fmax(a2 * a2, b2 * b2); gz = fmax(a3 * a3, b3 * b3);
+        } else {
+            real a1 = fmin(ax, (real)0), b1 = fmax(bx, (real)0), a2 = fmin(ay, (real)0), b2 = fmax(by, (real)0);
+            real a3 = fmin(az, (real)0), b3 = fmax(bz, (real)0);
+            gx = fmax(a1 * a1, b1 * b1); gy = fmax(a2 * a2, b2 * b2); gz = fmax(a3 * a3, b3 * b3);
+        }
+        real gm = sqrt(gx + gy + gz);
+        tmp[q] = c - dtau * sgn * (gm - (real)1.0);
     }
-    real gm = sqrt(gx + gy + gz);
-    out[q] = c - dtau * s * (gm - (real)1.0);
-}
-
-template <typename real>
-__global__ void __launch_bounds__(256) k_copy(long long n, const real* __restrict__ a, real* __restrict__ b)
-{
-    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) b[q] = a[q];
 }
 
 // ---------------------------------------------------------------------------
 // statistics: per-cell FP64 accumulators (:3167-3192) + y-z plane profiles.
-// Grid (chunks, nx): each block sums one chunk of one x-plane; the chunk sums
-// are added to the plane profile in a fixed order by k_duct_stats_final.
+// Grid (chunks, nx_max, slots); chunk sums are added in a fixed order.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ bool stats_due(const SlotDesc& D, const SlotDyn& y)
+{
+    return D.k.stats_start >= 0 && y.step >= D.k.stats_start;
+}
+
 template <typename real>
-__global__ void __launch_bounds__(256) k_duct_stats(GridDims g, DuctConsts k, const real* q0, const real* q1,
-                                                    const real* q2, const real* q3, const real* q4, StatPtrs st,
-                                                    int per_cell, double* spart)
+__global__ void __launch_bounds__(256) k_duct_stats(Pool P)
 {
     __shared__ double red[10][256];
+    const int s = blockIdx.z;
+    const SlotDesc& D = P.desc[s];
+    if (!D.active || P.dyn[s].done || !stats_due(D, P.dyn[s])) return;
+    const GridDims& g = D.g;
     const int i = blockIdx.y;
-    const long long P = (long long)g.ny * g.nz;
-    const long long chunk = (P + gridDim.x - 1) / gridDim.x;
+    if (i >= g.nx) return;
+    const DuctConsts& k = D.k;
+    const long long PL = (long long)g.ny * g.nz;
+    const long long chunk = (PL + gridDim.x - 1) / gridDim.x;
     const long long p0 = (long long)blockIdx.x * chunk;
-    const long long p1 = p0 + chunk < P ? p0 + chunk : P;
+    const long long p1 = p0 + chunk < PL ? p0 + chunk : PL;
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const real dx = (real)g.dx;
+    const real rut = (real)k.rho_u_Tu;
+    const real* A0 = (const real*)P.A[0] + D.off;
+    const real* A1 = (const real*)P.A[1] + D.off;
+    const real* A2 = (const real*)P.A[2] + D.off;
+    const real* A3 = (const real*)P.A[3] + D.off;
+    const real* A4 = (const real*)P.A[4] + D.off;
+    const bool per_cell = k.stats_per_cell && P.stat[0];
     for (long long p = p0 + threadIdx.x; p < p1; p += 256) {
-        long long q = (long long)i * P + p;
-        real rho = (real)k.rho_u_Tu / q4[q];
-        double u = (double)(q0[q] / rho), v = (double)(q1[q] / rho), w = (double)(q2[q] / rho);
-        double c = (double)progress<real>(q3[q], dx);
+        long long q = (long long)i * PL + p;
+        real rho = rut / A4[q];
+        double u = (double)(A0[q] / rho), v = (double)(A1[q] / rho), w = (double)(A2[q] / rho);
+        double c = (double)progress<real>(A3[q], dx);
         if (per_cell) {
-            st.acc[0][q] += u; st.acc[1][q] += v; st.acc[2][q] += w; st.acc[3][q] += c;
-            st.acc[4][q] += u * u; st.acc[5][q] += v * v; st.acc[6][q] += w * w;
-            st.acc[7][q] += u * v; st.acc[8][q] += u * w; st.acc[9][q] += v * w;
+            long long qa = D.off + q;
+            P.stat[0][qa] += u; P.stat[1][qa] += v; P.stat[2][qa] += w; P.stat[3][qa] += c;
+            P.stat[4][qa] += u * u; P.stat[5][qa] += v * v; P.stat[6][qa] += w * w;
+            P.stat[7][qa] += u * v; P.stat[8][qa] += u * w; P.stat[9][qa] += v * w;
         }
         acc[0] += u; acc[1] += v; acc[2] += w; acc[3] += c;
         acc[4] += u * u; acc[5] += v * v; acc[6] += w * w; acc[7] += u * v; acc[8] += u * w; acc[9] += v * w;
     }
     for (int a = 0; a < 10; ++a) red[a][threadIdx.x] = acc[a];
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            for (int a = 0; a < 10; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + s];
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int a = 0; a < 10; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x < 10) spart[((long long)i * gridDim.x + blockIdx.x) * 10 + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 10)
+        P.spart[(((long long)s * P.nx_max + i) * STAT_CHUNKS + blockIdx.x) * 10 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void k_duct_stats_final(int nx, int chunks, const double* spart, double* prof)
+__global__ void k_duct_stats_final(Pool P)
 {
+    const int s = blockIdx.y;
+    const SlotDesc& D = P.desc[s];
+    if (!D.active || P.dyn[s].done || !stats_due(D, P.dyn[s])) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nx * 10) return;
+    if (t >= D.g.nx * 10) return;
     int i = t / 10, a = t % 10;
-    double s = 0.0;
-    for (int c = 0; c < chunks; ++c) s += spart[((long long)i * chunks + c) * 10 + a];
-    prof[t] += s;
+    double sm = 0.0;
+    for (int c = 0; c < STAT_CHUNKS; ++c) sm += P.spart[(((long long)s * P.nx_max + i) * STAT_CHUNKS + c) * 10 + a];
+    P.prof[D.xoff + t] += sm;
 }
 
-// fixed-point time series (u, v, w, G) at npts cells into a ring of cap steps
+// fixed-point series (u, v, w, G) of each slot into its ring, then the step end
 template <typename real>
-__global__ void k_duct_points(DuctConsts k, double dx, const int* idx, int npts, const real* q0, const real* q1,
-                              const real* q2, const real* q3, const real* q4, double* ring, int slot)
+__global__ void k_duct_points(Pool P)
 {
-    int t = threadIdx.x;
-    if (t >= npts) return;
-    long long q = idx[t];
-    real rho = (real)k.rho_u_Tu / q4[q];
-    double* o = ring + ((long long)slot * npts + t) * 4;
-    o[0] = (double)(q0[q] / rho); o[1] = (double)(q1[q] / rho); o[2] = (double)(q2[q] / rho);
-    o[3] = (double)progress<real>(q3[q], (real)dx);
+    const int s = blockIdx.x;
+    const SlotDesc& D = P.desc[s];
+    if (!D.active || P.dyn[s].done) return;
+    const int t = threadIdx.x;
+    if (t >= DUCT_NPTS) return;
+    const int q = D.point_idx[t];
+    real rho = (real)D.k.rho_u_Tu / ((const real*)P.A[4] + D.off)[q];
+    const int slot = P.dyn[s].step % P.point_cap;
+    double* o = P.points + D.ptoff + ((long long)slot * DUCT_NPTS + t) * 4;
+    o[0] = (double)(((const real*)P.A[0] + D.off)[q] / rho);
+    o[1] = (double)(((const real*)P.A[1] + D.off)[q] / rho);
+    o[2] = (double)(((const real*)P.A[2] + D.off)[q] / rho);
+    o[3] = (double)progress<real>(((const real*)P.A[3] + D.off)[q], (real)D.g.dx);
 }
 
-// monitor reductions of the duct state (total mass, T and G range, non-finite)
+__global__ void k_duct_step_end(Pool P, int nslots)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const SlotDesc& D = P.desc[s];
+    SlotDyn& y = P.dyn[s];
+    if (!D.active || y.done) return;
+    y.time += y.dt;
+    y.step += 1;
+    if (D.n_steps > 0 && y.step >= D.n_steps) y.done = 1;
+}
+
+// monitor reductions of one slot (total mass, T and G range, non-finite)
 template <typename real>
-__global__ void __launch_bounds__(256) k_duct_monitor(GridDims g, DuctConsts k, const real* q0, const real* q3,
-                                                      const real* q4, double* part)
+__global__ void __launch_bounds__(256) k_duct_monitor(Pool P, int s, double* part)
 {
     __shared__ double sh[6][256];
+    const SlotDesc& D = P.desc[s];
+    const GridDims& g = D.g;
     double tm = 0, mnT = 1e9, mxT = -1e9, mnG = 1e9, mxG = -1e9, bad = 0;
     const double dV = g.dx * g.dy * g.dz;
+    const real* A0 = (const real*)P.A[0] + D.off;
+    const real* A3 = (const real*)P.A[3] + D.off;
+    const real* A4 = (const real*)P.A[4] + D.off;
     for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < g.n; q += (long long)gridDim.x * 256) {
-        double T = (double)q4[q];
-        tm += k.rho_u_Tu / T * dV;
+        double T = (double)A4[q];
+        tm += D.k.rho_u_Tu / T * dV;
         mnT = fmin(mnT, T); mxT = fmax(mxT, T);
-        double c = (double)progress<real>(q3[q], (real)g.dx);
+        double c = (double)progress<real>(A3[q], (real)g.dx);
         mnG = fmin(mnG, c); mxG = fmax(mxG, c);
-        if (!isfinite(T) || !isfinite((double)q0[q]) || !isfinite((double)q3[q])) bad = 1.0;
+        if (!isfinite(T) || !isfinite((double)A0[q]) || !isfinite((double)A3[q])) bad = 1.0;
     }
     sh[0][threadIdx.x] = tm; sh[1][threadIdx.x] = mnT; sh[2][threadIdx.x] = mxT;
     sh[3][threadIdx.x] = mnG; sh[4][threadIdx.x] = mxG; sh[5][threadIdx.x] = bad;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int w = 128; w > 0; w >>= 1) {
         int t = threadIdx.x;
-        if (t < s) {
-            sh[0][t] += sh[0][t + s];
-            sh[1][t] = fmin(sh[1][t], sh[1][t + s]); sh[2][t] = fmax(sh[2][t], sh[2][t + s]);
-            sh[3][t] = fmin(sh[3][t], sh[3][t + s]); sh[4][t] = fmax(sh[4][t], sh[4][t + s]);
-            sh[5][t] = fmax(sh[5][t], sh[5][t + s]);
+        if (t < w) {
+            sh[0][t] += sh[0][t + w];
+            sh[1][t] = fmin(sh[1][t], sh[1][t + w]); sh[2][t] = fmax(sh[2][t], sh[2][t + w]);
+            sh[3][t] = fmin(sh[3][t], sh[3][t + w]); sh[4][t] = fmax(sh[4][t], sh[4][t + w]);
+            sh[5][t] = fmax(sh[5][t], sh[5][t + w]);
         }
         __syncthreads();
     }
@@ -904,8 +1177,25 @@ __global__ void __launch_bounds__(256) k_duct_monitor(GridDims g, DuctConsts k, 
         for (int c = 0; c < 6; ++c) part[6 * blockIdx.x + c] = sh[c][0];
 }
 
+template <typename real>
+__global__ void k_to_real(long long n, const double* src, real* dst)
+{
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[q] = (real)src[q];
+}
+
+template <typename real>
+__global__ void k_to_double(long long n, const real* src, double* dst)
+{
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[q] = (double)src[q];
+}
+
+// host code below keeps IEEE operation order
+#pragma clang fp contract(off)
+
 // ---------------------------------------------------------------------------
-// host side of the duct mode
+// host side
 // ---------------------------------------------------------------------------
 #define DHIPC(call)                                                                        \
     do {                                                                                   \
@@ -919,11 +1209,112 @@ __global__ void __launch_bounds__(256) k_duct_monitor(GridDims g, DuctConsts k, 
 
 #define DLAUNCH_CHECK() DHIPC(hipGetLastError())
 
-static DuctConsts make_duct_consts(const les_gpu* c)
+// Calls f with the tile width as a compile-time constant.
+template <typename F>
+static int tk_dispatch(int tk, F&& f)
+{
+    switch (tk) {
+    case 0: return f(std::integral_constant<int, 0>());
+    case 32: return f(std::integral_constant<int, 32>());
+    case 128: return f(std::integral_constant<int, 128>());
+    default: return f(std::integral_constant<int, 64>());
+    }
+}
+
+static bool tk_valid(int tk) { return tk == 0 || tk == 32 || tk == 64 || tk == 128; }
+
+// Tiles of one run: TK > 0 covers TK x (256/TK) cells of an x-plane, flat
+// tiles 256 consecutive cells of an x-plane.
+static void run_tiles(int tk, int nx, int ny, int nz, int& tiles_j, int& tiles_k, long long& ntiles)
+{
+    if (tk == 0) {
+        tiles_j = 1;
+        tiles_k = (int)(((long long)ny * nz + 255) / 256);
+    } else {
+        const int tj = 256 / tk;
+        tiles_j = (ny + tj - 1) / tj;
+        tiles_k = (nz + tk - 1) / tk;
+    }
+    ntiles = (long long)nx * tiles_j * tiles_k;
+}
+
+// First-fit allocator of ranges in [0, cap).
+struct RangeAlloc {
+    long long cap = 0;
+    std::vector<std::pair<long long, long long>> free_;   // (start, length), sorted
+    void reset(long long c) { cap = c; free_.clear(); free_.push_back({0, c}); }
+    bool alloc(long long n, long long& start)
+    {
+        for (size_t a = 0; a < free_.size(); ++a)
+            if (free_[a].second >= n) {
+                start = free_[a].first;
+                free_[a].first += n;
+                free_[a].second -= n;
+                if (free_[a].second == 0) free_.erase(free_.begin() + a);
+                return true;
+            }
+        return false;
+    }
+    void release(long long start, long long n)
+    {
+        size_t a = 0;
+        while (a < free_.size() && free_[a].first < start) ++a;
+        free_.insert(free_.begin() + a, {start, n});
+        // merge neighbours
+        for (size_t b = 0; b + 1 < free_.size();) {
+            if (free_[b].first + free_[b].second == free_[b + 1].first) {
+                free_[b].second += free_[b + 1].second;
+                free_.erase(free_.begin() + b + 1);
+            } else ++b;
+        }
+    }
+    long long largest() const
+    {
+        long long m = 0;
+        for (auto& f : free_) m = f.second > m ? f.second : m;
+        return m;
+    }
+};
+
+struct DuctState {
+    int precision = LES_PREC_FP64;
+    size_t esize = 8;
+    int max_slots = 1;
+    long long pool_cells = 0;
+    int tile_cap = 0;
+    int nx_max = 0, ny_max = 0, nz_max = 0, nvort_max = 0, nzb = 0;
+    int point_cap = 1024;
+    int stats_cells = 0;
+    int cus = 256;
+    // launch configuration (autotuned)
+    int tk = 64;
+    int grid = 8;                  // tile high-water mark rounded up to a multiple of 8
+    int xcd_map = 1;
+    double tuned_ms = 0.0;
+    Pool P;
+    SlotDesc* d_desc = nullptr;
+    SlotDyn* d_dyn = nullptr;
+    int* d_tile_slot = nullptr;
+    int* d_ntile = nullptr;
+    std::vector<SlotDesc> desc;
+    std::vector<int> tile_slot;
+    std::vector<int> taken;        // slot admitted (its state may not be set yet)
+    std::vector<int> host_step;
+    std::vector<int> point_base;   // host step index of the oldest unfetched point sample
+    RangeAlloc cells, tiles;
+    int ntile_hwm = 0;
+    double* stage64 = nullptr;
+    double* mon_part = nullptr;
+    int max_reinit_iters = 0;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};   // [0] plain step, [1] with reinitialisation
+    bool graph_ok[2] = {false, false};
+    int use_graph = 1;
+};
+
+static DuctConsts make_duct_consts(const les_params& p)
 {
     DuctConsts k;
     memset(&k, 0, sizeof(k));
-    const les_params& p = c->p;
     k.rho_u_Tu = p.rho_unburnt * p.inlet_temperature;
     k.Tu = p.inlet_temperature; k.Tb = p.burnt_temperature;
     k.U = p.inlet_velocity; k.intensity = p.turbulence_intensity;
@@ -939,18 +1330,12 @@ static DuctConsts make_duct_consts(const les_gpu* c)
     k.table_t0 = p.table_t0; k.table_dt = p.table_dt; k.table_n = p.table_n;
     k.scheme = p.advection;
     k.nvort = p.num_vortices;
+    k.reinit_interval = p.reinit_interval; k.reinit_iters = p.reinit_iters;
+    k.stats_start = p.stats_start_step; k.stats_per_cell = p.stats_per_cell;
     k.L = p.turbulent_length_scale;
     k.ly = p.ly; k.lz = p.lz;
     k.seed = (unsigned long long)p.seed;
     return k;
-}
-
-void duct_params_changed(les_gpu* c)
-{
-    if (c->d) {
-        c->d->k = make_duct_consts(c);
-        c->d->graph_ok = false;   // constants are baked into the captured graph
-    }
 }
 
 static void* dalloc_bytes(les_gpu* c, size_t bytes)
@@ -962,129 +1347,216 @@ static void* dalloc_bytes(les_gpu* c, size_t bytes)
     return p;
 }
 
-int duct_create(les_gpu* c)
+static void grid_dims(const les_params& p, GridDims& g)
 {
-    if (c->p.integrator != LES_INT_SSPRK3 ||
-        (c->p.advection != LES_ADV_WENO5 && c->p.advection != LES_ADV_MUSCL_MINMOD))
-        return LES_ERR_PARAM;
+    g.nx = p.nx; g.ny = p.ny; g.nz = p.nz;
+    g.n = (long long)p.nx * p.ny * p.nz;
+    g.sx = (long long)p.ny * p.nz;
+    g.sy = p.nz;
+    g.dx = p.lx / (double)p.nx;
+    g.dy = p.ly / (double)p.ny;
+    g.dz = p.lz / (double)p.nz;
+}
+
+// Allocates the pool of one context.  cfg limits come from the batch config
+// (a single-run context is a pool of one slot sized to its grid).
+static int pool_create(les_gpu* c, const les_batch_config& cfg)
+{
     DuctState* d = new (std::nothrow) DuctState();
     if (!d) return LES_ERR_HIP;
     c->d = d;
-    d->precision = c->p.precision;
-    d->esize = (d->precision == LES_PREC_FP32) ? 4 : 8;
-    d->k = make_duct_consts(c);
-    const size_t n = (size_t)c->g.n, nb = n * d->esize;
-    for (int f = 0; f < DNF; ++f) {
-        if (!(d->b.A[f] = dalloc_bytes(c, nb)) || !(d->b.B[f] = dalloc_bytes(c, nb)) || !(d->b.F[f] = dalloc_bytes(c, nb)))
-            return LES_ERR_HIP;
+    d->precision = cfg.precision;
+    d->esize = (cfg.precision == LES_PREC_FP32) ? 4 : 8;
+    d->max_slots = cfg.max_slots;
+    d->pool_cells = cfg.pool_cells;
+    d->nx_max = cfg.max_nx; d->ny_max = cfg.max_ny; d->nz_max = cfg.max_nz;
+    d->nvort_max = cfg.max_vortices > 0 ? cfg.max_vortices : 1;
+    d->point_cap = cfg.point_cap > 0 ? cfg.point_cap : 1024;
+    d->stats_cells = cfg.stats_per_cell;
+    d->nzb = (d->nz_max + 255) / 256;
+    d->tk = cfg.tile_k;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        d->cus = prop.multiProcessorCount;
+    d->xcd_map = cfg.xcd_map;
+    // tiles of all resident runs, for any tile width: per run at most
+    // n/256 + nx*ny/2 + nx*nz/32 + nx (ceil bounds of each tile shape)
+    {
+        long long cap = d->pool_cells / 256 +
+                        (long long)d->max_slots * d->nx_max * (d->ny_max / 2 + d->nz_max / 32 + 2) + 64;
+        if (cap > (1LL << 30)) return LES_ERR_PARAM;
+        d->tile_cap = (int)cap;
     }
-    if (!(d->b.nut = dalloc_bytes(c, nb)) || !(d->b.st = dalloc_bytes(c, nb))) return LES_ERR_HIP;
-    const int tn = c->p.table_n;
+    const size_t nb = (size_t)d->pool_cells * d->esize;
+    Pool& P = d->P;
+    memset(&P, 0, sizeof(P));
+    for (int f = 0; f < DNF; ++f)
+        if (!(P.A[f] = dalloc_bytes(c, nb)) || !(P.B[f] = dalloc_bytes(c, nb)) || !(P.F[f] = dalloc_bytes(c, nb)))
+            return LES_ERR_HIP;
+    if (!(P.nut = dalloc_bytes(c, nb)) || !(P.st = dalloc_bytes(c, nb))) return LES_ERR_HIP;
+    if (d->stats_cells)
+        for (int a = 0; a < 10; ++a)
+            if (!(P.stat[a] = (double*)dalloc_bytes(c, 8 * (size_t)d->pool_cells))) return LES_ERR_HIP;
+    const int tn = cfg.table_n;
     std::vector<double> cp(tn), mu(tn);
     for (int t = 0; t < tn; ++t) {
-        double T = c->p.table_t0 + t * c->p.table_dt;
+        double T = cfg.table_t0 + t * cfg.table_dt;
         cp[t] = 1000.0 + 0.2 * (T - 300.0);
         mu[t] = 1.8e-5 * pow(T / 300.0, 0.7);
     }
-    d->b.cp_tab = (double*)dalloc_bytes(c, tn * 8);
-    d->b.mu_tab = (double*)dalloc_bytes(c, tn * 8);
-    if (!d->b.cp_tab || !d->b.mu_tab) return LES_ERR_HIP;
-    DHIPC(hipMemcpy(d->b.cp_tab, cp.data(), tn * 8, hipMemcpyHostToDevice));
-    DHIPC(hipMemcpy(d->b.mu_tab, mu.data(), tn * 8, hipMemcpyHostToDevice));
-    const size_t P = (size_t)c->p.ny * c->p.nz;
-    d->b.eddy = (double*)dalloc_bytes(c, 7 * 8 * (size_t)(c->p.num_vortices > 0 ? c->p.num_vortices : 1));
-    d->b.inlet = (double*)dalloc_bytes(c, 3 * 8 * P);
-    Launch3 L = launch3(c->g.nx, c->g.ny, c->g.nz);
-    d->filter_blocks = (int)(L.grid.x * L.grid.y * L.grid.z);
-    d->b.rho_part = (double*)dalloc_bytes(c, 8 * (size_t)d->filter_blocks);
-    d->b.scal = (double*)dalloc_bytes(c, 8 * 8);
-    d->b.maxbits = (unsigned long long*)dalloc_bytes(c, 8 * 4);
-    d->bc_blocks = (int)(((c->g.nz + 255) / 256) * c->g.ny);
-    d->b.vpart = (double*)dalloc_bytes(c, 8 * (size_t)(d->filter_blocks + d->bc_blocks + 1024));
-    d->b.dpart = (double*)dalloc_bytes(c, 8 * 2 * 3 * (size_t)d->filter_blocks);
-    d->stat_chunks = 16;
-    d->b.spart = (double*)dalloc_bytes(c, 8 * 10 * (size_t)d->stat_chunks * c->p.nx);
-    if (!d->b.vpart || !d->b.dpart || !d->b.spart) return LES_ERR_HIP;
-    d->b.prof = (double*)dalloc_bytes(c, 8 * 10 * (size_t)c->p.nx);
-    d->b.mon_part = (double*)dalloc_bytes(c, 8 * 6 * 1024);
-    if (!d->b.eddy || !d->b.inlet || !d->b.rho_part || !d->b.scal || !d->b.maxbits || !d->b.prof || !d->b.mon_part)
+    P.cp_tab = (double*)dalloc_bytes(c, 8 * (size_t)tn);
+    P.mu_tab = (double*)dalloc_bytes(c, 8 * (size_t)tn);
+    if (!P.cp_tab || !P.mu_tab) return LES_ERR_HIP;
+    DHIPC(hipMemcpy(P.cp_tab, cp.data(), 8 * (size_t)tn, hipMemcpyHostToDevice));
+    DHIPC(hipMemcpy(P.mu_tab, mu.data(), 8 * (size_t)tn, hipMemcpyHostToDevice));
+    const size_t S = (size_t)d->max_slots;
+    P.eddy = (double*)dalloc_bytes(c, 8 * 7 * S * d->nvort_max);
+    P.inlet = (double*)dalloc_bytes(c, 8 * 3 * S * d->ny_max * d->nz_max);
+    P.prof = (double*)dalloc_bytes(c, 8 * 10 * S * d->nx_max);
+    P.points = (double*)dalloc_bytes(c, 8 * 4 * DUCT_NPTS * S * d->point_cap);
+    P.rho_part = (double*)dalloc_bytes(c, 8 * (size_t)d->tile_cap);
+    P.vpart = (double*)dalloc_bytes(c, 8 * (size_t)d->tile_cap);
+    P.dpart = (double*)dalloc_bytes(c, 8 * 2 * 3 * (size_t)d->tile_cap);
+    P.bcpart = (double*)dalloc_bytes(c, 8 * S * d->ny_max * d->nzb);
+    P.spart = (double*)dalloc_bytes(c, 8 * 10 * S * d->nx_max * STAT_CHUNKS);
+    d->d_desc = (SlotDesc*)dalloc_bytes(c, sizeof(SlotDesc) * S);
+    d->d_dyn = (SlotDyn*)dalloc_bytes(c, sizeof(SlotDyn) * S);
+    d->d_tile_slot = (int*)dalloc_bytes(c, 4 * (size_t)d->tile_cap);
+    d->d_ntile = (int*)dalloc_bytes(c, 64);
+    d->mon_part = (double*)dalloc_bytes(c, 8 * 6 * 1024);
+    if (!P.eddy || !P.inlet || !P.prof || !P.points || !P.rho_part || !P.vpart || !P.dpart || !P.bcpart ||
+        !P.spart || !d->d_desc || !d->d_dyn || !d->d_tile_slot || !d->d_ntile || !d->mon_part)
         return LES_ERR_HIP;
-    if (c->p.stats_start_step >= 0 && c->p.stats_per_cell) {
-        for (int a = 0; a < 10; ++a)
-            if (!(d->b.stat[a] = (double*)dalloc_bytes(c, 8 * n))) return LES_ERR_HIP;
-    }
-    // 64 fixed probe points: 4 x-stations x 4 y x 4 z, evenly spaced
-    d->npts = 64;
-    d->point_cap = 4096;
-    std::vector<int> pidx(64);
+    if (d->precision == LES_PREC_FP32 && !(d->stage64 = (double*)dalloc_bytes(c, 8 * (size_t)d->pool_cells)))
+        return LES_ERR_HIP;
+    P.desc = d->d_desc; P.dyn = d->d_dyn; P.tile_slot = d->d_tile_slot; P.ntile = d->d_ntile;
+    P.tile_cap = d->tile_cap;
+    P.ny_max = d->ny_max; P.nz_max = d->nz_max; P.nx_max = d->nx_max; P.nzb = d->nzb;
+    P.point_cap = d->point_cap;
+    d->desc.assign(S, SlotDesc());
+    for (auto& x : d->desc) memset(&x, 0, sizeof(x));
+    d->tile_slot.assign(d->tile_cap, -1);
+    d->taken.assign(S, 0);
+    d->host_step.assign(S, 0);
+    d->point_base.assign(S, 0);
+    d->cells.reset(d->pool_cells);
+    d->tiles.reset(d->tile_cap);
+    std::vector<int> neg(d->tile_cap, -1);
+    DHIPC(hipMemcpy(d->d_tile_slot, neg.data(), 4 * (size_t)d->tile_cap, hipMemcpyHostToDevice));
+    return LES_OK;
+}
+
+// Uploads the descriptor table, tile list and tile high-water mark.
+static int pool_sync_tables(les_gpu* c)
+{
+    DuctState* d = c->d;
+    DHIPC(hipStreamSynchronize(c->stream));     // no step may be reading the tables
+    int hwm = 0;
+    for (int t = 0; t < d->tile_cap; ++t) if (d->tile_slot[t] >= 0) hwm = t + 1;
+    d->ntile_hwm = hwm;
+    const int grid = hwm > 0 ? (hwm + 7) / 8 * 8 : 8;
+    if (grid != d->grid) { d->grid = grid; d->graph_ok[0] = d->graph_ok[1] = false; }
+    DHIPC(hipMemcpyAsync(d->d_desc, d->desc.data(), sizeof(SlotDesc) * d->max_slots, hipMemcpyHostToDevice, c->stream));
+    DHIPC(hipMemcpyAsync(d->d_tile_slot, d->tile_slot.data(), 4 * (size_t)d->tile_cap, hipMemcpyHostToDevice, c->stream));
+    DHIPC(hipMemcpyAsync(d->d_ntile, &d->ntile_hwm, 4, hipMemcpyHostToDevice, c->stream));
+    DHIPC(hipStreamSynchronize(c->stream));
+    int maxit = 0;
+    for (auto& x : d->desc)
+        if (x.active && x.k.reinit_interval > 0 && x.k.reinit_iters > maxit) maxit = x.k.reinit_iters;
+    if (maxit != d->max_reinit_iters) { d->graph_ok[0] = d->graph_ok[1] = false; }
+    d->max_reinit_iters = maxit;
+    return LES_OK;
+}
+
+// Admits run p into a free slot (first-fit cells and tiles); the slot starts
+// empty (state zero) until init or upload.
+static int pool_admit(les_gpu* c, const les_params& p, int n_steps, int* slot_out)
+{
+    DuctState* d = c->d;
+    if (p.mode != LES_MODE_DUCT || p.integrator != LES_INT_SSPRK3 ||
+        (p.advection != LES_ADV_WENO5 && p.advection != LES_ADV_MUSCL_MINMOD))
+        return LES_ERR_PARAM;
+    if (p.precision != d->precision || p.nx > d->nx_max || p.ny > d->ny_max || p.nz > d->nz_max ||
+        p.num_vortices > d->nvort_max || p.reinit_iters < 0 || p.reinit_iters > 64)
+        return LES_ERR_ADMISSION;
+    int s = -1;
+    for (int a = 0; a < d->max_slots; ++a) if (!d->taken[a]) { s = a; break; }
+    if (s < 0) return LES_ERR_ADMISSION;
+    GridDims g;
+    grid_dims(p, g);
+    int tiles_j, tiles_k;
+    long long ntiles;
+    run_tiles(d->tk, p.nx, p.ny, p.nz, tiles_j, tiles_k, ntiles);
+    long long c0 = 0, t0 = 0;
+    if (!d->cells.alloc(g.n, c0)) return LES_ERR_ADMISSION;
+    if (!d->tiles.alloc(ntiles, t0)) { d->cells.release(c0, g.n); return LES_ERR_ADMISSION; }
+    SlotDesc& D = d->desc[s];
+    memset(&D, 0, sizeof(D));
+    D.g = g;
+    D.off = c0;
+    D.poff = (long long)s * 3 * d->ny_max * d->nz_max;
+    D.xoff = (long long)s * 10 * d->nx_max;
+    D.eoff = (long long)s * d->nvort_max;
+    D.ptoff = (long long)s * 4 * DUCT_NPTS * d->point_cap;
+    D.tile0 = (int)t0; D.ntiles = (int)ntiles; D.tiles_j = tiles_j; D.tiles_k = tiles_k;
+    D.active = 0;               // set once the run has a state (init or upload)
+    D.n_steps = n_steps;
     int t = 0;
     for (int a = 0; a < 4; ++a)
-        for (int b2 = 0; b2 < 4; ++b2)
+        for (int b = 0; b < 4; ++b)
             for (int e = 0; e < 4; ++e) {
-                int ii = (a + 1) * c->p.nx / 5, jj = (2 * b2 + 1) * c->p.ny / 8, kk = (2 * e + 1) * c->p.nz / 8;
-                pidx[t++] = (int)(((long long)ii * c->p.ny + jj) * c->p.nz + kk);
+                int ii = (a + 1) * p.nx / 5, jj = (2 * b + 1) * p.ny / 8, kk = (2 * e + 1) * p.nz / 8;
+                D.point_idx[t++] = (int)(((long long)ii * p.ny + jj) * p.nz + kk);
             }
-    d->b.point_idx = (int*)dalloc_bytes(c, 64 * 4);
-    d->b.points = (double*)dalloc_bytes(c, 8 * 4 * 64 * (size_t)d->point_cap);
-    if (!d->b.point_idx || !d->b.points) return LES_ERR_HIP;
-    DHIPC(hipMemcpy(d->b.point_idx, pidx.data(), 64 * 4, hipMemcpyHostToDevice));
-    if (d->precision == LES_PREC_FP32 && !(d->stage64 = (double*)dalloc_bytes(c, 8 * n))) return LES_ERR_HIP;
-    c->attr_h.assign(n, 3);
+    D.k = make_duct_consts(p);
+    for (long long a = 0; a < ntiles; ++a) d->tile_slot[t0 + a] = s;
+    d->host_step[s] = 0;
+    d->point_base[s] = 0;
+    d->taken[s] = 1;
+    SlotDyn y;
+    memset(&y, 0, sizeof(y));
+    DHIPC(hipMemcpyAsync(d->d_dyn + s, &y, sizeof(y), hipMemcpyHostToDevice, c->stream));
+    int e = pool_sync_tables(c);
+    if (e != LES_OK) return e;
+    *slot_out = s;
     return LES_OK;
 }
 
-void duct_destroy(les_gpu* c)
-{
-    if (!c->d) return;
-    if (c->d->gexec) (void)hipGraphExecDestroy(c->d->gexec);
-    if (c->d->graph) (void)hipGraphDestroy(c->d->graph);
-    delete c->d;
-    c->d = nullptr;
-}
-
-// ---- typed launches ----
-template <typename real>
-static int duct_velmax_t(les_gpu* c)
+static int pool_release(les_gpu* c, int s)
 {
     DuctState* d = c->d;
-    const GridDims& g = c->g;
-    hipLaunchKernelGGL(k_duct_velmax<real>, dim3(1024), dim3(256), 0, c->stream, g, d->k.rho_u_Tu,
-                       (const real*)d->b.A[0], (const real*)d->b.A[1], (const real*)d->b.A[2], (const real*)d->b.A[4],
-                       d->b.vpart);
-    DLAUNCH_CHECK();
-    return LES_OK;
+    if (s < 0 || s >= d->max_slots || !d->taken[s]) return LES_ERR_PARAM;
+    SlotDesc& D = d->desc[s];
+    d->cells.release(D.off, D.g.n);
+    d->tiles.release(D.tile0, D.ntiles);
+    for (int a = 0; a < D.ntiles; ++a) d->tile_slot[D.tile0 + a] = -1;
+    D.active = 0;
+    d->taken[s] = 0;
+    return pool_sync_tables(c);
 }
 
-template <typename real>
-static int duct_stage_t(les_gpu* c, int stage, const real* const* qin, real* const* qout, real* const* Lout = nullptr)
+// ---- launches ----
+template <typename real, int TK>
+static int stage_launch(les_gpu* c, int stage, real* const* Lout)
 {
     DuctState* d = c->d;
-    const GridDims& g = c->g;
-    Launch3 L = launch3(g.nx, g.ny, g.nz);
-    real** F = (real**)d->b.F;
-    hipLaunchKernelGGL(k_duct_filter<real>, L.grid, L.block, 0, c->stream, g, d->k, qin[0], qin[1], qin[2], qin[3],
-                       qin[4], F[0], F[1], F[2], F[3], F[4], d->b.rho_part);
-    DLAUNCH_CHECK();
-    hipLaunchKernelGGL(k_duct_rho_final, dim3(1), dim3(1024), 0, c->stream, g.n, d->filter_blocks, d->b.rho_part,
-                       d->b.scal);
-    DLAUNCH_CHECK();
-    hipLaunchKernelGGL(k_duct_sgs<real>, L.grid, L.block, 0, c->stream, g, d->k, d->b.mu_tab, d->b.cp_tab, F[0], F[1],
-                       F[2], F[3], F[4], (real*)d->b.nut, (real*)d->b.st,
-                       d->b.dpart + 2 * (size_t)stage * d->filter_blocks);
+    const int G = d->grid;
+    const int qin_B = stage > 0, qout_B = stage < 2;
+    hipLaunchKernelGGL((k_duct_filter<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, qin_B, d->xcd_map);
+    hipLaunchKernelGGL(k_duct_rho_final, dim3(d->max_slots), dim3(1024), 0, c->stream, d->P);
+    hipLaunchKernelGGL((k_duct_sgs<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, stage, d->xcd_map);
     DLAUNCH_CHECK();
     RhsArgs<real> a;
-    a.uf = F[0]; a.vf = F[1]; a.wf = F[2]; a.Tf = F[3]; a.pf = F[4];
-    a.nut = (const real*)d->b.nut; a.st = (const real*)d->b.st;
-    for (int f = 0; f < DNF; ++f) { a.qin[f] = qin[f]; a.q0[f] = (const real*)d->b.A[f]; a.qout[f] = qout[f]; }
-    a.mu_tab = d->b.mu_tab; a.cp_tab = d->b.cp_tab; a.scal = d->b.scal; a.vpart = d->b.vpart;
-    a.stage = stage;
+    a.qin_is_B = qin_B; a.qout_is_B = qout_B; a.stage = stage;
     a.ca = (stage == 1) ? 0.75 : (1.0 / 3.0);
     a.cb = (stage == 1) ? 0.25 : (2.0 / 3.0);
     a.track_vel = (stage == 2);
+    a.xcd_map = d->xcd_map;
     for (int f = 0; f < DNF; ++f) a.Lout[f] = Lout ? Lout[f] : nullptr;
     if (c->timing) DHIPC(hipEventRecord(c->ev[0], c->stream));
-    hipLaunchKernelGGL(k_duct_rhs_scalar<real>, L.grid, L.block, 0, c->stream, g, d->k, a);
-    hipLaunchKernelGGL(k_duct_rhs_momentum<real>, L.grid, L.block, 0, c->stream, g, d->k, a);
+    hipLaunchKernelGGL((k_duct_rhs_scalar<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, a);
+    hipLaunchKernelGGL((k_duct_rhs_momentum<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, a);
     DLAUNCH_CHECK();
     if (c->timing) {
         DHIPC(hipEventRecord(c->ev[1], c->stream));
@@ -1097,193 +1569,108 @@ static int duct_stage_t(les_gpu* c, int stage, const real* const* qin, real* con
     return LES_OK;
 }
 
-// One full time step, all launches on c->stream (capturable).
-template <typename real>
-static int duct_step_launches(les_gpu* c, bool reinit, bool stats, bool points)
+// One engine step of every active, unfinished slot; all launches on c->stream.
+template <typename real, int TK>
+static int step_launches(les_gpu* c, bool reinit)
 {
     DuctState* d = c->d;
-    const GridDims& g = c->g;
-    const long long P = (long long)g.ny * g.nz;
-    hipLaunchKernelGGL(k_duct_dt, dim3(1), dim3(1024), 0, c->stream, g.dx, d->k, d->b.scal, d->b.vpart,
-                       d->filter_blocks + d->bc_blocks, d->b.dpart, 3 * d->filter_blocks);
+    const int S = d->max_slots;
+    hipLaunchKernelGGL(k_duct_dt, dim3(S), dim3(1024), 0, c->stream, d->P);
+    hipLaunchKernelGGL(k_duct_eddy_update, dim3((d->nvort_max + 255) / 256, S), dim3(256), 0, c->stream, d->P);
+    hipLaunchKernelGGL(k_duct_sem_plane, dim3(d->nzb, d->ny_max, S), dim3(256), 0, c->stream, d->P);
+    hipLaunchKernelGGL(k_duct_sem_norm, dim3(3, S), dim3(1024), 0, c->stream, d->P);
     DLAUNCH_CHECK();
-    if (d->k.nvort > 0) {
-        hipLaunchKernelGGL(k_duct_eddy_update, dim3((d->k.nvort + 255) / 256), dim3(256), 0, c->stream, d->k, d->b.scal,
-                           d->b.eddy);
-        DLAUNCH_CHECK();
+    for (int s = 0; s < 3; ++s) {
+        int e = stage_launch<real, TK>(c, s, nullptr);
+        if (e != LES_OK) return e;
     }
-    hipLaunchKernelGGL(k_duct_sem_plane, dim3((g.nz + 255) / 256, g.ny), dim3(256), 0, c->stream, g, d->k, d->b.eddy,
-                       d->b.inlet);
-    hipLaunchKernelGGL(k_duct_sem_norm, dim3(3), dim3(1024), 0, c->stream, P, d->k.U * d->k.intensity, d->b.inlet);
-    DLAUNCH_CHECK();
-    real** A = (real**)d->b.A;
-    real** B = (real**)d->b.B;
-    int e;
-    if ((e = duct_stage_t<real>(c, 0, (const real* const*)A, B)) != LES_OK) return e;
-    if ((e = duct_stage_t<real>(c, 1, (const real* const*)B, B)) != LES_OK) return e;
-    if ((e = duct_stage_t<real>(c, 2, (const real* const*)B, A)) != LES_OK) return e;
-    hipLaunchKernelGGL(k_duct_bc<real>, dim3((g.nz + 255) / 256, g.ny), dim3(256), 0, c->stream, g, d->k, d->b.inlet,
-                       A[0], A[1], A[2], A[3], A[4], d->b.vpart + d->filter_blocks);
+    hipLaunchKernelGGL((k_duct_bc<real>), dim3(d->nzb, d->ny_max, S), dim3(256), 0, c->stream, d->P);
     DLAUNCH_CHECK();
     if (reinit) {
-        Launch3 L = launch3(g.nx, g.ny, g.nz);
-        int nbk = (int)((g.n + 255) / 256);
-        // phi0 kept in B[3]; ping-pong between A[3] and B[4] (B is dead after the step)
-        hipLaunchKernelGGL(k_copy<real>, dim3(nbk), dim3(256), 0, c->stream, g.n, (const real*)A[3], B[3]);
-        for (int it = 0; it < c->p.reinit_iters; ++it) {
-            hipLaunchKernelGGL(k_duct_reinit<real>, L.grid, L.block, 0, c->stream, g, (const real*)B[3],
-                               (const real*)A[3], B[4]);
-            hipLaunchKernelGGL(k_copy<real>, dim3(nbk), dim3(256), 0, c->stream, g.n, (const real*)B[4], A[3]);
+        hipLaunchKernelGGL((k_duct_reinit<real, TK>), dim3(d->grid), dim3(256), 0, c->stream, d->P, 0, 0, d->xcd_map);
+        for (int it = 0; it < d->max_reinit_iters; ++it) {
+            hipLaunchKernelGGL((k_duct_reinit<real, TK>), dim3(d->grid), dim3(256), 0, c->stream, d->P, 1, it,
+                               d->xcd_map);
+            hipLaunchKernelGGL((k_duct_reinit<real, TK>), dim3(d->grid), dim3(256), 0, c->stream, d->P, 2, it,
+                               d->xcd_map);
         }
         DLAUNCH_CHECK();
     }
-    if (stats) {
-        StatPtrs st;
-        for (int a = 0; a < 10; ++a) st.acc[a] = d->b.stat[a];
-        hipLaunchKernelGGL(k_duct_stats<real>, dim3(d->stat_chunks, g.nx), dim3(256), 0, c->stream, g, d->k,
-                           (const real*)A[0], (const real*)A[1], (const real*)A[2], (const real*)A[3],
-                           (const real*)A[4], st, d->b.stat[0] != nullptr ? 1 : 0, d->b.spart);
-        hipLaunchKernelGGL(k_duct_stats_final, dim3((g.nx * 10 + 255) / 256), dim3(256), 0, c->stream, g.nx,
-                           d->stat_chunks, d->b.spart, d->b.prof);
-        DLAUNCH_CHECK();
-    }
-    if (points) {
-        int slot = d->point_count % d->point_cap;
-        hipLaunchKernelGGL(k_duct_points<real>, dim3(1), dim3(64), 0, c->stream, d->k, g.dx, d->b.point_idx, d->npts,
-                           (const real*)A[0], (const real*)A[1], (const real*)A[2], (const real*)A[3],
-                           (const real*)A[4], d->b.points, slot);
-        DLAUNCH_CHECK();
-    }
-    return LES_OK;
-}
-
-template <typename real>
-static int duct_monitor_t(les_gpu* c, les_step_report* rep)
-{
-    DuctState* d = c->d;
-    const GridDims& g = c->g;
-    int nb = (int)((g.n + 255) / 256);
-    if (nb > 1024) nb = 1024;
-    hipLaunchKernelGGL(k_duct_monitor<real>, dim3(nb), dim3(256), 0, c->stream, g, d->k, (const real*)d->b.A[0],
-                       (const real*)d->b.A[3], (const real*)d->b.A[4], d->b.mon_part);
+    hipLaunchKernelGGL((k_duct_stats<real>), dim3(STAT_CHUNKS, d->nx_max, S), dim3(256), 0, c->stream, d->P);
+    hipLaunchKernelGGL(k_duct_stats_final, dim3((d->nx_max * 10 + 255) / 256, S), dim3(256), 0, c->stream, d->P);
+    hipLaunchKernelGGL((k_duct_points<real>), dim3(S), dim3(64), 0, c->stream, d->P);
+    hipLaunchKernelGGL(k_duct_step_end, dim3((S + 255) / 256), dim3(256), 0, c->stream, d->P, S);
     DLAUNCH_CHECK();
-    std::vector<double> h((size_t)nb * 6);
-    double sc[4];
-    DHIPC(hipMemcpyAsync(h.data(), d->b.mon_part, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    DHIPC(hipMemcpyAsync(sc, d->b.scal, 4 * 8, hipMemcpyDeviceToHost, c->stream));
-    DHIPC(hipStreamSynchronize(c->stream));
-    double tm = 0, mnT = 1e9, mxT = -1e9, mnG = 1e9, mxG = -1e9, bad = 0;
-    for (int b = 0; b < nb; ++b) {
-        const double* q = &h[6 * (size_t)b];
-        tm += q[0]; mnT = fmin(mnT, q[1]); mxT = fmax(mxT, q[2]); mnG = fmin(mnG, q[3]); mxG = fmax(mxG, q[4]);
-        bad = fmax(bad, q[5]);
-    }
-    rep->dt = sc[0]; rep->max_velocity = sc[2];
-    rep->total_mass = tm; rep->min_T = mnT; rep->max_T = mxT; rep->min_G = mnG; rep->max_G = mxG;
-    rep->min_Y_fuel = 0.0; rep->nonfinite = bad > 0.0; rep->proj_residual = 0.0;
     return LES_OK;
 }
 
-template <typename real>
-static int duct_steps_t(les_gpu* c, int n_steps, les_step_report* rep)
+static int pool_step_launches(les_gpu* c, bool reinit)
+{
+    return tk_dispatch(c->d->tk, [&](auto K) {
+        return c->d->precision == LES_PREC_FP32 ? step_launches<float, decltype(K)::value>(c, reinit)
+                                                : step_launches<double, decltype(K)::value>(c, reinit);
+    });
+}
+
+// Advances the pool by n engine steps: graph replay, one graph with and one
+// without the reinitialisation launches (chosen from the host's step mirror).
+static int pool_steps(les_gpu* c, int n)
 {
     DuctState* d = c->d;
-    const bool stats_on = c->p.stats_start_step >= 0;
-    for (int s = 0; s < n_steps; ++s) {
-        const int t = c->step;
-        const bool reinit = c->p.reinit_interval > 0 && (t + 1) % c->p.reinit_interval == 0;
-        const bool stats = stats_on && t >= c->p.stats_start_step;
-        // a plain step (no reinitialisation) replays a captured graph
-        bool plain = !reinit && !c->timing && d->use_graph;
-        if (plain && d->graph_ok && (stats != d->graph_stats)) plain = false;
-        if (plain && !d->graph_ok) {
-            d->graph_stats = stats;
-            if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-                int e = duct_step_launches<real>(c, false, stats, false);
-                hipGraph_t gr = nullptr;
-                hipError_t ce = hipStreamEndCapture(c->stream, &gr);
-                if (e == LES_OK && ce == hipSuccess && gr) {
-                    if (d->gexec) (void)hipGraphExecDestroy(d->gexec);
-                    if (d->graph) (void)hipGraphDestroy(d->graph);
-                    d->graph = gr;
-                    if (hipGraphInstantiate(&d->gexec, gr, nullptr, nullptr, 0) == hipSuccess) d->graph_ok = true;
-                } else if (gr) {
-                    (void)hipGraphDestroy(gr);
+    for (int a = 0; a < n; ++a) {
+        bool reinit = false;
+        for (int s = 0; s < d->max_slots; ++s) {
+            const SlotDesc& D = d->desc[s];
+            if (!D.active || (D.n_steps > 0 && d->host_step[s] >= D.n_steps)) continue;
+            if (D.k.reinit_interval > 0 && (d->host_step[s] + 1) % D.k.reinit_interval == 0) reinit = true;
+        }
+        const int gi = reinit ? 1 : 0;
+        bool done = false;
+        if (d->use_graph && !c->timing) {
+            if (!d->graph_ok[gi]) {
+                if (d->gexec[gi]) { (void)hipGraphExecDestroy(d->gexec[gi]); d->gexec[gi] = nullptr; }
+                if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                    int e = pool_step_launches(c, reinit);
+                    hipGraph_t gr = nullptr;
+                    hipError_t ce = hipStreamEndCapture(c->stream, &gr);
+                    if (e == LES_OK && ce == hipSuccess && gr &&
+                        hipGraphInstantiate(&d->gexec[gi], gr, nullptr, nullptr, 0) == hipSuccess)
+                        d->graph_ok[gi] = true;
+                    if (gr) (void)hipGraphDestroy(gr);
                 }
                 (void)hipGetLastError();
+                if (!d->graph_ok[gi]) d->use_graph = 0;
             }
-            if (!d->graph_ok) { d->use_graph = 0; plain = false; }
+            if (d->graph_ok[gi]) {
+                DHIPC(hipGraphLaunch(d->gexec[gi], c->stream));
+                done = true;
+            }
         }
-        if (plain && d->graph_ok) {
-            DHIPC(hipGraphLaunch(d->gexec, c->stream));
-            int slot = d->point_count % d->point_cap;
-            hipLaunchKernelGGL(k_duct_points<real>, dim3(1), dim3(64), 0, c->stream, d->k, c->g.dx, d->b.point_idx,
-                               d->npts, (const real*)d->b.A[0], (const real*)d->b.A[1], (const real*)d->b.A[2],
-                               (const real*)d->b.A[3], (const real*)d->b.A[4], d->b.points, slot);
-            DLAUNCH_CHECK();
-        } else {
-            int e = duct_step_launches<real>(c, reinit, stats, true);
+        if (!done) {
+            int e = pool_step_launches(c, reinit);
             if (e != LES_OK) return e;
         }
-        d->point_count++;
-        if (stats) { c->stat_count++; d->prof_count++; }
-        c->step++;
-    }
-    if (rep) {
-        rep->step = c->step - 1;
-        return duct_monitor_t<real>(c, rep);
-    }
-    return LES_OK;
-}
-
-int duct_step(les_gpu* c, int n_steps, les_step_report* rep)
-{
-    DHIPC(hipEventRecord(c->ev[2], c->stream));
-    int e = (c->d->precision == LES_PREC_FP32) ? duct_steps_t<float>(c, n_steps, nullptr)
-                                               : duct_steps_t<double>(c, n_steps, nullptr);
-    if (e != LES_OK) return e;
-    DHIPC(hipEventRecord(c->ev[3], c->stream));
-    DHIPC(hipEventSynchronize(c->ev[3]));
-    float ms = 0.f;
-    DHIPC(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-    c->step_ms = n_steps > 0 ? ms / n_steps : 0.0;
-    if (rep) {
-        rep->step = c->step - 1;
-        e = (c->d->precision == LES_PREC_FP32) ? duct_monitor_t<float>(c, rep) : duct_monitor_t<double>(c, rep);
-        if (e != LES_OK) return e;
-        if (rep->nonfinite) {
-            snprintf(c->err, sizeof(c->err), "non-finite value in the state at step %d", rep->step);
-            return LES_ERR_NAN;
+        for (int s = 0; s < d->max_slots; ++s) {
+            const SlotDesc& D = d->desc[s];
+            if (!D.active || (D.n_steps > 0 && d->host_step[s] >= D.n_steps)) continue;
+            d->host_step[s]++;
         }
     }
     return LES_OK;
 }
 
-// ---- state exchange (fields rho*u, rho*v, rho*w, phi, T as double) ----
-template <typename real>
-__global__ void k_to_real(long long n, const double* src, real* dst)
-{
-    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) dst[q] = (real)src[q];
-}
-
-template <typename real>
-__global__ void k_to_double(long long n, const real* src, double* dst)
-{
-    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) dst[q] = (double)src[q];
-}
-
-static int duct_xfer(les_gpu* c, void* dev, double* host, bool upload)
+// ---- state exchange of one slot (fields rho*u, rho*v, rho*w, phi, T as double) ----
+static int slot_xfer(les_gpu* c, int s, void* arena, double* host, bool upload)
 {
     DuctState* d = c->d;
-    const size_t n = (size_t)c->g.n;
+    const SlotDesc& D = d->desc[s];
+    const size_t n = (size_t)D.g.n;
+    char* dev = (char*)arena + (size_t)D.off * d->esize;
     if (d->precision == LES_PREC_FP64) {
         if (upload) DHIPC(hipMemcpyAsync(dev, host, n * 8, hipMemcpyHostToDevice, c->stream));
         else DHIPC(hipMemcpyAsync(host, dev, n * 8, hipMemcpyDeviceToHost, c->stream));
         return LES_OK;
     }
-    // FP32 storage: convert through the context's FP64 staging buffer
     double* tmp = d->stage64;
     int nb = (int)((n + 255) / 256);
     if (upload) {
@@ -1293,106 +1680,216 @@ static int duct_xfer(les_gpu* c, void* dev, double* host, bool upload)
         hipLaunchKernelGGL(k_to_double<float>, dim3(nb), dim3(256), 0, c->stream, (long long)n, (const float*)dev, tmp);
         DHIPC(hipMemcpyAsync(host, tmp, n * 8, hipMemcpyDeviceToHost, c->stream));
     }
-    return LES_OK;
-}
-
-static int duct_after_state_change(les_gpu* c)
-{
-    DuctState* d = c->d;
-    DHIPC(hipMemsetAsync(d->b.vpart, 0, 8 * (size_t)(d->filter_blocks + d->bc_blocks), c->stream));
-    DHIPC(hipMemsetAsync(d->b.dpart, 0, 8 * 2 * 3 * (size_t)d->filter_blocks, c->stream));
-    int e = (d->precision == LES_PREC_FP32) ? duct_velmax_t<float>(c) : duct_velmax_t<double>(c);
-    if (e != LES_OK) return e;
     DHIPC(hipStreamSynchronize(c->stream));
     return LES_OK;
 }
 
-int duct_upload_state(les_gpu* c, const double* const* f, int nf)
+// After new state in slot s: DT maxima of the state (|u|), no diffusive history.
+static int slot_state_changed(les_gpu* c, int s)
 {
+    DuctState* d = c->d;
+    const SlotDesc& D = d->desc[s];
+    DHIPC(hipMemsetAsync(d->P.bcpart + (size_t)s * d->ny_max * d->nzb, 0, 8 * (size_t)d->ny_max * d->nzb, c->stream));
+    tk_dispatch(d->tk, [&](auto K) {
+        constexpr int TK = decltype(K)::value;
+        if (d->precision == LES_PREC_FP32)
+            hipLaunchKernelGGL((k_duct_velmax<float, TK>), dim3(d->grid), dim3(256), 0, c->stream, d->P, s, d->xcd_map);
+        else
+            hipLaunchKernelGGL((k_duct_velmax<double, TK>), dim3(d->grid), dim3(256), 0, c->stream, d->P, s, d->xcd_map);
+        return 0;
+    });
+    DLAUNCH_CHECK();
+    DHIPC(hipStreamSynchronize(c->stream));
+    (void)D;
+    return LES_OK;
+}
+
+// New state of slot s (the step counter is kept, as the oracle's set_state
+// keeps its step); the run becomes active if it was only admitted.
+static int slot_upload(les_gpu* c, int s, const double* const* f, int nf)
+{
+    DuctState* d = c->d;
     if (nf < DNF) { snprintf(c->err, sizeof(c->err), "duct mode state has 5 fields"); return LES_ERR_PARAM; }
+    if (!d->desc[s].active) {
+        d->desc[s].active = 1;
+        int e = pool_sync_tables(c);
+        if (e != LES_OK) return e;
+    }
     for (int a = 0; a < DNF; ++a) {
-        int e = duct_xfer(c, c->d->b.A[a], (double*)f[a], true);
+        int e = slot_xfer(c, s, d->P.A[a], (double*)f[a], true);
         if (e != LES_OK) return e;
     }
-    return duct_after_state_change(c);
+    return slot_state_changed(c, s);
 }
 
-int duct_download_state(les_gpu* c, double* const* f, int nf)
-{
-    for (int a = 0; a < nf && a < DNF; ++a) {
-        int e = duct_xfer(c, c->d->b.A[a], f[a], false);
-        if (e != LES_OK) return e;
-    }
-    DHIPC(hipStreamSynchronize(c->stream));
-    return LES_OK;
-}
-
-int duct_init_state(les_gpu* c)
+// initial state of a slot: plug flow at U, planar flame at flame_x0 (host
+// computed: runs once per run), Gaussian eddies, cleared statistics
+static int slot_init(les_gpu* c, int s, const les_params& p)
 {
     DuctState* d = c->d;
-    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
-    const size_t n = (size_t)c->g.n;
+    const SlotDesc& D = d->desc[s];
+    const size_t n = (size_t)D.g.n;
     std::vector<double> h(5 * n);
-    const double U = c->p.inlet_velocity, Tu = c->p.inlet_temperature, Tb = c->p.burnt_temperature;
-    for (int i = 0; i < nx; ++i) {
-        double phi = i * c->g.dx - c->p.flame_x0;
-        double cc = 0.5 * (1.0 + tanh(phi / (2.0 * c->g.dx)));
+    const double U = p.inlet_velocity, Tu = p.inlet_temperature, Tb = p.burnt_temperature;
+    for (int i = 0; i < p.nx; ++i) {
+        double phi = i * D.g.dx - p.flame_x0;
+        double cc = 0.5 * (1.0 + tanh(phi / (2.0 * D.g.dx)));
         double T = Tu + cc * (Tb - Tu);
-        double rho = c->p.rho_unburnt * Tu / T;
-        for (size_t p = 0; p < (size_t)ny * nz; ++p) {
-            size_t q = (size_t)i * ny * nz + p;
+        double rho = p.rho_unburnt * Tu / T;
+        for (size_t q = (size_t)i * p.ny * p.nz; q < (size_t)(i + 1) * p.ny * p.nz; ++q) {
             h[q] = rho * U; h[n + q] = 0.0; h[2 * n + q] = 0.0; h[3 * n + q] = phi; h[4 * n + q] = T;
         }
     }
+    SlotDyn y;
+    memset(&y, 0, sizeof(y));
+    DHIPC(hipMemcpyAsync(d->d_dyn + s, &y, sizeof(y), hipMemcpyHostToDevice, c->stream));
+    d->host_step[s] = 0;
+    d->point_base[s] = 0;
     const double* f[5] = {&h[0], &h[n], &h[2 * n], &h[3 * n], &h[4 * n]};
-    int e = duct_upload_state(c, f, 5);
+    int e = slot_upload(c, s, f, 5);
     if (e != LES_OK) return e;
-    if (d->k.nvort > 0) {
-        hipLaunchKernelGGL(k_duct_eddy_init, dim3((d->k.nvort + 255) / 256), dim3(256), 0, c->stream, d->k, d->b.eddy);
+    if (D.k.nvort > 0) {
+        hipLaunchKernelGGL(k_duct_eddy_init, dim3((D.k.nvort + 255) / 256), dim3(256), 0, c->stream, d->P, s);
         DLAUNCH_CHECK();
     }
-    DHIPC(hipMemsetAsync(d->b.inlet, 0, 3 * 8 * (size_t)ny * nz, c->stream));
-    DHIPC(hipMemsetAsync(d->b.prof, 0, 10 * 8 * (size_t)nx, c->stream));
-    if (d->b.stat[0])
-        for (int a = 0; a < 10; ++a) DHIPC(hipMemsetAsync(d->b.stat[a], 0, n * 8, c->stream));
+    DHIPC(hipMemsetAsync(d->P.inlet + D.poff, 0, 8 * 3 * (size_t)D.g.ny * D.g.nz, c->stream));
+    DHIPC(hipMemsetAsync(d->P.prof + D.xoff, 0, 8 * 10 * (size_t)D.g.nx, c->stream));
+    if (d->P.stat[0])
+        for (int a = 0; a < 10; ++a) DHIPC(hipMemsetAsync(d->P.stat[a] + D.off, 0, 8 * n, c->stream));
     DHIPC(hipStreamSynchronize(c->stream));
-    c->step = 0;
-    c->stat_count = 0;
-    d->prof_count = 0;
-    d->point_count = 0;
-    c->probe.clear();
     return LES_OK;
 }
 
-int duct_download_field(les_gpu* c, const char* name, double* out)
+template <typename real>
+static int slot_monitor_t(les_gpu* c, int s, les_step_report* rep)
 {
     DuctState* d = c->d;
-    const size_t n = (size_t)c->g.n;
+    const SlotDesc& D = d->desc[s];
+    int nb = (int)((D.g.n + 255) / 256);
+    if (nb > 1024) nb = 1024;
+    hipLaunchKernelGGL((k_duct_monitor<real>), dim3(nb), dim3(256), 0, c->stream, d->P, s, d->mon_part);
+    DLAUNCH_CHECK();
+    std::vector<double> h((size_t)nb * 6);
+    SlotDyn y;
+    DHIPC(hipMemcpyAsync(h.data(), d->mon_part, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    DHIPC(hipMemcpyAsync(&y, d->d_dyn + s, sizeof(y), hipMemcpyDeviceToHost, c->stream));
+    DHIPC(hipStreamSynchronize(c->stream));
+    double tm = 0, mnT = 1e9, mxT = -1e9, mnG = 1e9, mxG = -1e9, bad = 0;
+    for (int b = 0; b < nb; ++b) {
+        const double* q = &h[6 * (size_t)b];
+        tm += q[0]; mnT = fmin(mnT, q[1]); mxT = fmax(mxT, q[2]); mnG = fmin(mnG, q[3]); mxG = fmax(mxG, q[4]);
+        bad = fmax(bad, q[5]);
+    }
+    rep->step = y.step - 1;
+    rep->dt = y.dt; rep->max_velocity = y.maxvel;
+    rep->total_mass = tm; rep->min_T = mnT; rep->max_T = mxT; rep->min_G = mnG; rep->max_G = mxG;
+    rep->min_Y_fuel = 0.0; rep->nonfinite = bad > 0.0; rep->proj_residual = 0.0;
+    return LES_OK;
+}
+
+static int slot_monitor(les_gpu* c, int s, les_step_report* rep)
+{
+    return c->d->precision == LES_PREC_FP32 ? slot_monitor_t<float>(c, s, rep) : slot_monitor_t<double>(c, s, rep);
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: times each (tile width, XCD mapping) candidate on a scratch pool
+// holding one run of the probe grid and keeps the fastest
+// ---------------------------------------------------------------------------
+static int pool_autotune(les_gpu* c, const les_params& probe)
+{
+    DuctState* d = c->d;
+    const int tks[4] = {0, 32, 64, 128};
+    double best = 1e30;
+    int btk = d->tk, bx = d->xcd_map;
+    for (int a = 0; a < 4; ++a)
+        for (int x = 0; x < 2; ++x) {
+            les_batch_config cfg;
+            memset(&cfg, 0, sizeof(cfg));
+            cfg.max_slots = 1;
+            cfg.pool_cells = (long long)probe.nx * probe.ny * probe.nz;
+            cfg.max_nx = probe.nx; cfg.max_ny = probe.ny; cfg.max_nz = probe.nz;
+            cfg.max_vortices = probe.num_vortices > 0 ? probe.num_vortices : 1;
+            cfg.precision = d->precision;
+            cfg.tile_k = tks[a];
+            cfg.xcd_map = x;
+            cfg.table_t0 = probe.table_t0; cfg.table_dt = probe.table_dt; cfg.table_n = probe.table_n;
+            cfg.point_cap = 16;
+            les_gpu* t = new (std::nothrow) les_gpu();
+            if (!t) return LES_ERR_HIP;
+            t->p = probe;
+            t->stream = c->stream;
+            int e = pool_create(t, cfg);
+            int slot = -1;
+            if (e == LES_OK) e = pool_admit(t, probe, 0, &slot);
+            if (e == LES_OK) e = slot_init(t, slot, probe);
+            if (e == LES_OK) e = pool_steps(t, 2);
+            float ms = 0.f;
+            if (e == LES_OK && hipEventRecord(c->ev[2], c->stream) == hipSuccess) {
+                e = pool_steps(t, 3);
+                (void)hipEventRecord(c->ev[3], c->stream);
+                (void)hipEventSynchronize(c->ev[3]);
+                (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+            }
+            if (e == LES_OK && ms > 0 && ms < best) {
+                best = ms;
+                btk = tks[a];
+                bx = x;
+            }
+            (void)hipStreamSynchronize(c->stream);
+            t->stream = nullptr;
+            les_gpu_destroy(t);
+            if (e != LES_OK) {
+                snprintf(c->err, sizeof(c->err), "autotune candidate tile_k=%d xcd_map=%d failed (%d)", tks[a], x, e);
+                return e;
+            }
+        }
+    d->tk = btk;
+    d->xcd_map = bx;
+    d->tuned_ms = best / 3.0;
+    d->graph_ok[0] = d->graph_ok[1] = false;
+    return LES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// downloads of one slot
+// ---------------------------------------------------------------------------
+static int slot_download_field(les_gpu* c, int s, const char* name, double* out)
+{
+    DuctState* d = c->d;
+    const SlotDesc& D = d->desc[s];
     struct { const char* nm; void* p; } tab[] = {
-        {"u_f", d->b.F[0]}, {"v_f", d->b.F[1]}, {"w_f", d->b.F[2]}, {"T_f", d->b.F[3]}, {"phi_f", d->b.F[4]},
-        {"nu_t", d->b.nut}, {"ST", d->b.st},
-        {"rhou", d->b.A[0]}, {"rhov", d->b.A[1]}, {"rhow", d->b.A[2]}, {"phi", d->b.A[3]}, {"T", d->b.A[4]}};
+        {"u_f", d->P.F[0]}, {"v_f", d->P.F[1]}, {"w_f", d->P.F[2]}, {"T_f", d->P.F[3]}, {"phi_f", d->P.F[4]},
+        {"nu_t", d->P.nut}, {"ST", d->P.st},
+        {"rhou", d->P.A[0]}, {"rhov", d->P.A[1]}, {"rhow", d->P.A[2]}, {"phi", d->P.A[3]}, {"T", d->P.A[4]}};
     for (auto& t : tab)
         if (strcmp(t.nm, name) == 0) {
-            int e = duct_xfer(c, t.p, out, false);
+            int e = slot_xfer(c, s, t.p, out, false);
             if (e != LES_OK) return e;
             DHIPC(hipStreamSynchronize(c->stream));
             return LES_OK;
         }
-    const size_t P = (size_t)c->p.ny * c->p.nz;
+    const size_t PL = (size_t)D.g.ny * D.g.nz;
     struct { const char* nm; const double* p; size_t cnt; } dt[] = {
-        {"inlet", d->b.inlet, 3 * P}, {"eddies", d->b.eddy, 7 * (size_t)c->p.num_vortices},
-        {"profiles", d->b.prof, 10 * (size_t)c->p.nx}, {"scalars", d->b.scal, 8}};
+        {"inlet", d->P.inlet + D.poff, 3 * PL}, {"eddies", d->P.eddy + 7 * D.eoff, 7 * (size_t)D.k.nvort},
+        {"profiles", d->P.prof + D.xoff, 10 * (size_t)D.g.nx}};
     for (auto& t : dt)
         if (strcmp(t.nm, name) == 0) {
             DHIPC(hipMemcpyAsync(out, t.p, t.cnt * 8, hipMemcpyDeviceToHost, c->stream));
             DHIPC(hipStreamSynchronize(c->stream));
             return LES_OK;
         }
-    if (strncmp(name, "stat", 4) == 0 && d->b.stat[0]) {
+    if (strcmp(name, "dyn") == 0) {          // dt, mean density, max |u|, time of the last step
+        SlotDyn y;
+        DHIPC(hipMemcpyAsync(&y, d->d_dyn + s, sizeof(y), hipMemcpyDeviceToHost, c->stream));
+        DHIPC(hipStreamSynchronize(c->stream));
+        out[0] = y.dt; out[1] = y.rho_avg; out[2] = y.maxvel; out[3] = y.time;
+        return LES_OK;
+    }
+    if (strncmp(name, "stat", 4) == 0 && d->P.stat[0]) {
         int a = atoi(name + 4);
         if (a < 0 || a > 9) return LES_ERR_PARAM;
-        DHIPC(hipMemcpyAsync(out, d->b.stat[a], n * 8, hipMemcpyDeviceToHost, c->stream));
+        DHIPC(hipMemcpyAsync(out, d->P.stat[a] + D.off, 8 * (size_t)D.g.n, hipMemcpyDeviceToHost, c->stream));
         DHIPC(hipStreamSynchronize(c->stream));
         return LES_OK;
     }
@@ -1400,46 +1897,367 @@ int duct_download_field(les_gpu* c, const char* name, double* out)
     return LES_ERR_PARAM;
 }
 
-// Testing hook: the RHS L(q) of the current state (stage-0 evaluation; its
-// combined output goes to the scratch stage buffers B, L itself to F-sized
-// scratch taken from B after the stage: B is dead between steps).
+static int slot_download(les_gpu* c, int s, double* const* f, int nf)
+{
+    for (int a = 0; a < nf && a < DNF; ++a) {
+        int e = slot_xfer(c, s, c->d->P.A[a], f[a], false);
+        if (e != LES_OK) return e;
+    }
+    DHIPC(hipStreamSynchronize(c->stream));
+    return LES_OK;
+}
+
+// fixed-point series since the last fetch: out[step][point][u, v, w, G]; the
+// ring keeps the newest point_cap steps (the sample of step t sits at t % cap)
+static int slot_point_fetch(les_gpu* c, int s, double* out, int max_steps, int* n_steps)
+{
+    DuctState* d = c->d;
+    const int total = d->host_step[s] - d->point_base[s];
+    const int avail = total < d->point_cap ? total : d->point_cap;
+    const int n = avail < max_steps ? avail : max_steps;
+    const int first = d->host_step[s] - avail;
+    const double* ring = d->P.points + d->desc[s].ptoff;
+    for (int a = 0; a < n; ++a) {
+        const int r = (first + a) % d->point_cap;
+        DHIPC(hipMemcpyAsync(out + (size_t)a * DUCT_NPTS * 4, ring + (size_t)r * DUCT_NPTS * 4,
+                             (size_t)DUCT_NPTS * 4 * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    DHIPC(hipStreamSynchronize(c->stream));
+    d->point_base[s] = d->host_step[s];
+    *n_steps = n;
+    return LES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// single-run duct context: a pool of one slot sized to its grid, 64 x 4 tiles
+// ---------------------------------------------------------------------------
+int duct_create(les_gpu* c)
+{
+    const les_params& p = c->p;
+    if (p.integrator != LES_INT_SSPRK3 || (p.advection != LES_ADV_WENO5 && p.advection != LES_ADV_MUSCL_MINMOD))
+        return LES_ERR_PARAM;
+    les_batch_config cfg;
+    les_batch_config_default(&cfg);
+    cfg.max_slots = 1;
+    cfg.pool_cells = (long long)p.nx * p.ny * p.nz;
+    cfg.max_nx = p.nx; cfg.max_ny = p.ny; cfg.max_nz = p.nz;
+    cfg.max_vortices = p.num_vortices > 0 ? p.num_vortices : 1;
+    cfg.precision = p.precision;
+    cfg.tile_k = 64;
+    cfg.xcd_map = 1;
+    cfg.point_cap = 4096;
+    cfg.stats_per_cell = (p.stats_start_step >= 0 && p.stats_per_cell) ? 1 : 0;
+    cfg.table_t0 = p.table_t0; cfg.table_dt = p.table_dt; cfg.table_n = p.table_n;
+    int e = pool_create(c, cfg);
+    if (e != LES_OK) return e;
+    int slot = -1;
+    e = pool_admit(c, p, 0, &slot);
+    if (e != LES_OK) {
+        snprintf(c->err, sizeof(c->err), "duct run does not fit its own pool (status %d)", e);
+        return e;
+    }
+    c->attr_h.assign((size_t)c->g.n, 3);
+    return LES_OK;
+}
+
+void duct_destroy(les_gpu* c)
+{
+    if (!c->d) return;
+    for (auto& g : c->d->gexec)
+        if (g) (void)hipGraphExecDestroy(g);
+    delete c->d;
+    c->d = nullptr;
+}
+
+void duct_params_changed(les_gpu* c)
+{
+    if (!c->d) return;
+    c->d->desc[0].k = make_duct_consts(c->p);
+    (void)pool_sync_tables(c);
+}
+
+int duct_init_state(les_gpu* c)
+{
+    int e = slot_init(c, 0, c->p);
+    if (e != LES_OK) return e;
+    c->step = 0;
+    c->stat_count = 0;
+    c->probe.clear();
+    return LES_OK;
+}
+
+int duct_upload_state(les_gpu* c, const double* const* f, int nf) { return slot_upload(c, 0, f, nf); }
+int duct_download_state(les_gpu* c, double* const* f, int nf) { return slot_download(c, 0, f, nf); }
+int duct_download_field(les_gpu* c, const char* name, double* out) { return slot_download_field(c, 0, name, out); }
+int duct_point_fetch(les_gpu* c, double* out, int max_steps, int* n_steps)
+{
+    return slot_point_fetch(c, 0, out, max_steps, n_steps);
+}
+
+int duct_step(les_gpu* c, int n_steps, les_step_report* rep)
+{
+    DuctState* d = c->d;
+    const int st0 = c->p.stats_start_step;
+    for (int a = 0; a < n_steps; ++a)
+        if (st0 >= 0 && d->host_step[0] + a >= st0) c->stat_count++;
+    DHIPC(hipEventRecord(c->ev[2], c->stream));
+    int e = pool_steps(c, n_steps);
+    if (e != LES_OK) return e;
+    DHIPC(hipEventRecord(c->ev[3], c->stream));
+    DHIPC(hipEventSynchronize(c->ev[3]));
+    float ms = 0.f;
+    DHIPC(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    c->step_ms = n_steps > 0 ? ms / n_steps : 0.0;
+    c->step += n_steps;
+    if (rep) {
+        e = slot_monitor(c, 0, rep);
+        if (e != LES_OK) return e;
+        rep->step = c->step - 1;
+        if (rep->nonfinite) {
+            snprintf(c->err, sizeof(c->err), "non-finite value in the state at step %d", rep->step);
+            return LES_ERR_NAN;
+        }
+    }
+    return LES_OK;
+}
+
+// Testing hook: the RHS L(q) of the current state (a stage-0 evaluation whose
+// combined output goes to the dead stage buffers B).
 int duct_rhs(les_gpu* c, double* const* out, int n_out)
 {
     if (n_out < DNF) return LES_ERR_PARAM;
     DuctState* d = c->d;
-    const size_t n = (size_t)c->g.n;
+    const SlotDesc& D = d->desc[0];
+    const size_t n = (size_t)D.g.n;
     std::vector<void*> Lbuf(DNF, nullptr);
     for (int f = 0; f < DNF; ++f) {
         DHIPC(hipMalloc(&Lbuf[f], n * d->esize));
         DHIPC(hipMemsetAsync(Lbuf[f], 0, n * d->esize, c->stream));   // boundary planes read as 0
     }
-    int e;
-    if (d->precision == LES_PREC_FP32)
-        e = duct_stage_t<float>(c, 0, (const float* const*)d->b.A, (float**)d->b.B, (float**)Lbuf.data());
-    else
-        e = duct_stage_t<double>(c, 0, (const double* const*)d->b.A, (double**)d->b.B, (double**)Lbuf.data());
-    for (int f = 0; f < DNF && e == LES_OK; ++f) {
-        e = duct_xfer(c, Lbuf[f], out[f], false);
-    }
+    int e = tk_dispatch(d->tk, [&](auto K) {
+        constexpr int TK = decltype(K)::value;
+        if (d->precision == LES_PREC_FP32) return stage_launch<float, TK>(c, 0, (float* const*)Lbuf.data());
+        return stage_launch<double, TK>(c, 0, (double* const*)Lbuf.data());
+    });
+    // slot 0 starts at cell 0 of the arenas, so slot-local buffers xfer as slot 0
+    for (int f = 0; f < DNF && e == LES_OK; ++f) e = slot_xfer(c, 0, (char*)Lbuf[f] - (size_t)D.off * d->esize, out[f], false);
     DHIPC(hipStreamSynchronize(c->stream));
     for (int f = 0; f < DNF; ++f) (void)hipFree(Lbuf[f]);
     return e;
 }
 
-// fixed-point time series since the last fetch: out[step][point][u, v, w, G]
-int duct_point_fetch(les_gpu* c, double* out, int max_steps, int* n_steps)
+// ---------------------------------------------------------------------------
+// batch ABI
+// ---------------------------------------------------------------------------
+struct les_batch {
+    les_gpu* c = nullptr;
+};
+
+static int batch_config_check(const les_batch_config* cfg)
 {
-    DuctState* d = c->d;
-    int avail = d->point_count < d->point_cap ? d->point_count : d->point_cap;
-    int n = avail < max_steps ? avail : max_steps;
-    int first = d->point_count - avail;    // oldest kept step
-    for (int s = 0; s < n; ++s) {
-        int slot = (first + s) % d->point_cap;
-        DHIPC(hipMemcpyAsync(out + (size_t)s * d->npts * 4, d->b.points + (size_t)slot * d->npts * 4,
-                             (size_t)d->npts * 4 * 8, hipMemcpyDeviceToHost, c->stream));
-    }
-    DHIPC(hipStreamSynchronize(c->stream));
-    d->point_count = 0;
-    *n_steps = n;
+    if (!cfg || cfg->max_slots < 1 || cfg->max_slots > 65536 || cfg->pool_cells < 1 || cfg->max_nx < 6 ||
+        cfg->max_ny < 1 || cfg->max_nz < 1 || cfg->max_vortices < 0 || !tk_valid(cfg->tile_k) ||
+        (cfg->precision != LES_PREC_FP64 && cfg->precision != LES_PREC_FP32) || cfg->point_cap < 1 ||
+        cfg->table_n < 2 || cfg->table_dt <= 0.0)
+        return LES_ERR_PARAM;
     return LES_OK;
+}
+
+extern "C" int les_batch_pool_bytes(const les_batch_config* cfg, size_t* bytes)
+{
+    if (batch_config_check(cfg) != LES_OK || !bytes) return LES_ERR_PARAM;
+    const size_t es = cfg->precision == LES_PREC_FP32 ? 4 : 8;
+    const size_t n = (size_t)cfg->pool_cells, S = (size_t)cfg->max_slots;
+    const size_t tiles = n / 256 + S * cfg->max_nx * (cfg->max_ny / 2 + cfg->max_nz / 32 + 2) + 64;
+    size_t b = 17 * n * es;                                   // A, B, F (5 each), nu_t, S_T
+    if (cfg->stats_per_cell) b += 80 * n;
+    if (cfg->precision == LES_PREC_FP32) b += 8 * n;           // FP64 staging of one run
+    b += tiles * (8 + 8 + 48 + 4);                            // rho, |u|, nu/alpha partials, tile list
+    const size_t nv = cfg->max_vortices > 0 ? cfg->max_vortices : 1;
+    b += S * 8 * (7 * nv + 3 * (size_t)cfg->max_ny * cfg->max_nz + 10 * (size_t)cfg->max_nx +
+                  4 * (size_t)DUCT_NPTS * cfg->point_cap + (size_t)cfg->max_ny * ((cfg->max_nz + 255) / 256) +
+                  10 * (size_t)cfg->max_nx * STAT_CHUNKS);
+    b += S * (sizeof(SlotDesc) + sizeof(SlotDyn)) + 16 * (size_t)cfg->table_n + 8 * 6 * 1024;
+    *bytes = b;
+    return LES_OK;
+}
+
+extern "C" int les_batch_create(const les_batch_config* cfg, const les_params* probe, les_batch** out)
+{
+    if (!out) return LES_ERR_PARAM;
+    *out = nullptr;
+    if (batch_config_check(cfg) != LES_OK) return LES_ERR_PARAM;
+    if (cfg->autotune && (!probe || les_params_validate(probe) != LES_OK || probe->mode != LES_MODE_DUCT))
+        return LES_ERR_PARAM;
+    les_batch* b = new (std::nothrow) les_batch();
+    if (!b) return LES_ERR_HIP;
+    les_gpu* c = new (std::nothrow) les_gpu();
+    if (!c) { delete b; return LES_ERR_HIP; }
+    b->c = c;
+    if (probe) c->p = *probe;
+    else les_params_default(&c->p, LES_PRESET_ENSEMBLE);
+    c->p.precision = cfg->precision;
+    les_mixture_synthetic(&c->mix);
+    int rc = LES_OK;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = LES_ERR_HIP;
+    for (int a = 0; a < 4 && rc == LES_OK; ++a)
+        if (hipEventCreate(&c->ev[a]) != hipSuccess) rc = LES_ERR_HIP;
+    if (rc == LES_OK) rc = pool_create(c, *cfg);
+    if (rc == LES_OK && cfg->autotune) rc = pool_autotune(c, *probe);
+    if (rc != LES_OK) {
+        fprintf(stderr, "les_batch_create: %s\n", c->err);
+        les_gpu_destroy(c);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return LES_OK;
+}
+
+extern "C" void les_batch_destroy(les_batch* b)
+{
+    if (!b) return;
+    les_gpu_destroy(b->c);
+    delete b;
+}
+
+extern "C" const char* les_batch_last_error(const les_batch* b) { return b && b->c ? b->c->err : "null context"; }
+
+extern "C" int les_batch_tuning(const les_batch* b, int* tile_k, int* grid_blocks, int* xcd_map, double* tuned_ms)
+{
+    if (!b) return LES_ERR_PARAM;
+    const DuctState* d = b->c->d;
+    if (tile_k) *tile_k = d->tk;
+    if (grid_blocks) *grid_blocks = d->grid;
+    if (xcd_map) *xcd_map = d->xcd_map;
+    if (tuned_ms) *tuned_ms = d->tuned_ms;
+    return LES_OK;
+}
+
+static int batch_slot_ok(les_batch* b, int slot)
+{
+    if (!b || slot < 0 || slot >= b->c->d->max_slots || !b->c->d->taken[slot]) {
+        if (b) snprintf(b->c->err, sizeof(b->c->err), "slot %d is not admitted", slot);
+        return 0;
+    }
+    return 1;
+}
+
+extern "C" int les_batch_admit(les_batch* b, const les_params* p, int n_steps, int init, int* slot)
+{
+    if (!b || !p || !slot || n_steps < 0) return LES_ERR_PARAM;
+    les_gpu* c = b->c;
+    *slot = -1;
+    if (les_params_validate(p) != LES_OK || p->mode != LES_MODE_DUCT) {
+        snprintf(c->err, sizeof(c->err), "batched runs are duct-mode runs with valid parameters");
+        return LES_ERR_PARAM;
+    }
+    const DuctState* d = c->d;
+    if (p->table_t0 != c->p.table_t0 || p->table_dt != c->p.table_dt || p->table_n != c->p.table_n ||
+        (p->stats_per_cell && !d->stats_cells)) {
+        snprintf(c->err, sizeof(c->err), "run needs a property table or per-cell statistics the pool lacks");
+        return LES_ERR_ADMISSION;
+    }
+    int e = pool_admit(c, *p, n_steps, slot);
+    if (e == LES_ERR_ADMISSION) {
+        snprintf(c->err, sizeof(c->err), "no free slot, cells or tiles for a %dx%dx%d run", p->nx, p->ny, p->nz);
+        return e;
+    }
+    if (e != LES_OK) return e;
+    if (init) {
+        e = slot_init(c, *slot, *p);
+        if (e != LES_OK) { (void)pool_release(c, *slot); *slot = -1; return e; }
+    }
+    return LES_OK;
+}
+
+extern "C" int les_batch_release(les_batch* b, int slot)
+{
+    if (!batch_slot_ok(b, slot)) return LES_ERR_PARAM;
+    return pool_release(b->c, slot);
+}
+
+extern "C" int les_batch_step(les_batch* b, int n_steps, double* ms_per_step)
+{
+    if (!b || n_steps < 0) return LES_ERR_PARAM;
+    les_gpu* c = b->c;
+    DHIPC(hipEventRecord(c->ev[2], c->stream));
+    int e = pool_steps(c, n_steps);
+    if (e != LES_OK) return e;
+    DHIPC(hipEventRecord(c->ev[3], c->stream));
+    DHIPC(hipEventSynchronize(c->ev[3]));
+    float ms = 0.f;
+    DHIPC(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    c->step_ms = n_steps > 0 ? ms / n_steps : 0.0;
+    if (ms_per_step) *ms_per_step = c->step_ms;
+    return LES_OK;
+}
+
+extern "C" int les_batch_slot_info(les_batch* b, int slot, les_slot_info* info)
+{
+    if (!b || !info || slot < 0 || slot >= b->c->d->max_slots) return LES_ERR_PARAM;
+    les_gpu* c = b->c;
+    DuctState* d = c->d;
+    memset(info, 0, sizeof(*info));
+    if (!d->taken[slot]) return LES_OK;
+    const SlotDesc& D = d->desc[slot];
+    SlotDyn y;
+    DHIPC(hipMemcpyAsync(&y, d->d_dyn + slot, sizeof(y), hipMemcpyDeviceToHost, c->stream));
+    DHIPC(hipStreamSynchronize(c->stream));
+    info->active = D.active;
+    info->done = y.done;
+    info->step = y.step;
+    info->n_steps = D.n_steps;
+    info->time = y.time;
+    info->dt = y.dt;
+    info->max_velocity = y.maxvel;
+    info->nx = D.g.nx; info->ny = D.g.ny; info->nz = D.g.nz;
+    info->cells = D.g.n;
+    return LES_OK;
+}
+
+extern "C" int les_batch_poll(les_batch* b, int* slots, int max_slots, int* n_done)
+{
+    if (!b || !n_done || (max_slots > 0 && !slots)) return LES_ERR_PARAM;
+    const DuctState* d = b->c->d;
+    int n = 0;
+    for (int s = 0; s < d->max_slots; ++s) {
+        const SlotDesc& D = d->desc[s];
+        if (d->taken[s] && D.active && D.n_steps > 0 && d->host_step[s] >= D.n_steps) {
+            if (n < max_slots) slots[n] = s;
+            ++n;
+        }
+    }
+    *n_done = n;
+    return LES_OK;
+}
+
+extern "C" int les_batch_upload(les_batch* b, int slot, const double* const* fields, int n_fields)
+{
+    if (!batch_slot_ok(b, slot) || !fields) return LES_ERR_PARAM;
+    return slot_upload(b->c, slot, fields, n_fields);
+}
+
+extern "C" int les_batch_download(les_batch* b, int slot, double* const* fields, int n_fields)
+{
+    if (!batch_slot_ok(b, slot) || !fields) return LES_ERR_PARAM;
+    return slot_download(b->c, slot, fields, n_fields);
+}
+
+extern "C" int les_batch_download_field(les_batch* b, int slot, const char* name, double* out)
+{
+    if (!batch_slot_ok(b, slot) || !name || !out) return LES_ERR_PARAM;
+    return slot_download_field(b->c, slot, name, out);
+}
+
+extern "C" int les_batch_report(les_batch* b, int slot, les_step_report* rep)
+{
+    if (!batch_slot_ok(b, slot) || !rep) return LES_ERR_PARAM;
+    return slot_monitor(b->c, slot, rep);
+}
+
+extern "C" int les_batch_point_fetch(les_batch* b, int slot, double* out, int max_steps, int* n_steps)
+{
+    if (!batch_slot_ok(b, slot) || !out || !n_steps || max_steps < 0) return LES_ERR_PARAM;
+    return slot_point_fetch(b->c, slot, out, max_steps, n_steps);
 }

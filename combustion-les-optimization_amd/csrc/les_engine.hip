@@ -711,6 +711,6 @@ extern "C" int les_gpu_point_fetch(les_gpu* c, double* out, int max_steps, int* 
 {
     if (!c || !out || !n_steps || max_steps < 0) return LES_ERR_PARAM;
     if (c->p.mode != LES_MODE_DUCT) return set_err(c, LES_ERR_STATE, "point series are recorded in duct mode");
-    if (n_points) *n_points = c->d->npts;
+    if (n_points) *n_points = DUCT_NPTS;
     return duct_point_fetch(c, out, max_steps, n_steps);
 }

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "les_ref_kernels.hpp"

@@ -356,12 +356,36 @@ int les_mixture_load(les_mixture* m, const char* chem_bin, const char* tran_bin)
     return LES_OK;
 }
 
-int les_abi_struct_sizes(size_t out[4])
+int les_abi_struct_sizes(size_t out[6])
 {
     if (!out) return LES_ERR_PARAM;
     out[0] = sizeof(les_params);
     out[1] = sizeof(les_mixture);
     out[2] = sizeof(les_step_report);
     out[3] = sizeof(les_probe_row);
+    out[4] = sizeof(les_batch_config);
+    out[5] = sizeof(les_slot_info);
+    return LES_OK;
+}
+
+int les_batch_config_default(les_batch_config* cfg)
+{
+    if (!cfg) return LES_ERR_PARAM;
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->max_slots = 64;
+    cfg->pool_cells = 64LL * 64 * 48 * 48;
+    cfg->max_nx = 128;
+    cfg->max_ny = 128;
+    cfg->max_nz = 128;
+    cfg->max_vortices = 2000;
+    cfg->precision = LES_PREC_FP32;
+    cfg->tile_k = 64;
+    cfg->xcd_map = 1;
+    cfg->autotune = 0;
+    cfg->point_cap = 256;
+    cfg->stats_per_cell = 0;
+    cfg->table_t0 = 300.0;
+    cfg->table_dt = 1.0;
+    cfg->table_n = 2001;
     return LES_OK;
 }

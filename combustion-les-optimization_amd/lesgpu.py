@@ -52,6 +52,25 @@ def lib():
         L.les_gpu_kernel_timing.argtypes = [vp, P(C.c_double), P(C.c_int), P(C.c_double)]
         L.les_gpu_set_timing.argtypes = [vp, C.c_int]
         L.les_gpu_stats_count.argtypes = [vp]
+        L.les_gpu_point_fetch.argtypes = [vp, P(C.c_double), C.c_int, P(C.c_int), P(C.c_int)]
+        L.les_batch_config_default.argtypes = [P(abi.BatchConfig)]
+        L.les_batch_pool_bytes.argtypes = [P(abi.BatchConfig), P(C.c_size_t)]
+        L.les_batch_create.argtypes = [P(abi.BatchConfig), P(abi.Params), P(vp)]
+        L.les_batch_destroy.argtypes = [vp]
+        L.les_batch_destroy.restype = None
+        L.les_batch_last_error.argtypes = [vp]
+        L.les_batch_last_error.restype = C.c_char_p
+        L.les_batch_tuning.argtypes = [vp, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_double)]
+        L.les_batch_admit.argtypes = [vp, P(abi.Params), C.c_int, C.c_int, P(C.c_int)]
+        L.les_batch_release.argtypes = [vp, C.c_int]
+        L.les_batch_step.argtypes = [vp, C.c_int, P(C.c_double)]
+        L.les_batch_slot_info.argtypes = [vp, C.c_int, P(abi.SlotInfo)]
+        L.les_batch_poll.argtypes = [vp, P(C.c_int), C.c_int, P(C.c_int)]
+        L.les_batch_upload.argtypes = [vp, C.c_int, P(P(C.c_double)), C.c_int]
+        L.les_batch_download.argtypes = [vp, C.c_int, P(P(C.c_double)), C.c_int]
+        L.les_batch_download_field.argtypes = [vp, C.c_int, C.c_char_p, P(C.c_double)]
+        L.les_batch_report.argtypes = [vp, C.c_int, P(abi.StepReport)]
+        L.les_batch_point_fetch.argtypes = [vp, C.c_int, P(C.c_double), C.c_int, P(C.c_int)]
         _lib = L
     return _lib
 
@@ -175,3 +194,132 @@ class LesGpu:
         ms, n, st = C.c_double(), C.c_int(), C.c_double()
         self._check(lib().les_gpu_kernel_timing(self.h, C.byref(ms), C.byref(n), C.byref(st)), "les_gpu_kernel_timing")
         return ms.value, n.value, st.value
+
+    def point_series(self, max_steps=4096):
+        """Fixed-point series recorded since the last fetch: array [step, point, (u, v, w, G)]."""
+        out = np.zeros(max_steps * abi.LES_NPOINT_MAX * 4)
+        ns, npnt = C.c_int(), C.c_int()
+        self._check(lib().les_gpu_point_fetch(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), max_steps,
+                                              C.byref(ns), C.byref(npnt)), "les_gpu_point_fetch")
+        return out[:ns.value * npnt.value * 4].reshape(ns.value, npnt.value, 4)
+
+
+def batch_config(**over):
+    cfg = abi.BatchConfig()
+    lib().les_batch_config_default(C.byref(cfg))
+    for k, v in over.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def pool_bytes(cfg):
+    b = C.c_size_t()
+    if lib().les_batch_pool_bytes(C.byref(cfg), C.byref(b)) != 0:
+        raise ValueError("invalid batch configuration")
+    return b.value
+
+
+class Batch:
+    """A pool of duct-mode runs advanced together (les_batch_* entry points).
+
+    Every ``step`` launch advances all resident, unfinished runs; runs are
+    admitted and released between steps (continuous batching).  A run's
+    results do not depend on what else is resident."""
+
+    def __init__(self, cfg=None, probe=None):
+        self.cfg = cfg if cfg is not None else batch_config()
+        h = C.c_void_p()
+        rc = lib().les_batch_create(C.byref(self.cfg), C.byref(probe) if probe is not None else None, C.byref(h))
+        if rc != 0:
+            raise LesError("les_batch_create failed (%d)" % rc)
+        self.h = h
+        self.shapes = {}
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().les_batch_last_error(self.h)
+            raise LesError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().les_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def tuning(self):
+        tk, gb, xm, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self._check(lib().les_batch_tuning(self.h, C.byref(tk), C.byref(gb), C.byref(xm), C.byref(ms)),
+                    "les_batch_tuning")
+        return {"tile_k": tk.value, "grid_blocks": gb.value, "xcd_map": xm.value, "tuned_ms": ms.value}
+
+    def admit(self, p, n_steps=0, init=True):
+        """Slot of the admitted run, or None when it does not fit (admission refused)."""
+        s = C.c_int(-1)
+        rc = lib().les_batch_admit(self.h, C.byref(p), n_steps, 1 if init else 0, C.byref(s))
+        if rc == abi.LES_ERR_ADMISSION:
+            return None
+        self._check(rc, "les_batch_admit")
+        self.shapes[s.value] = (p.nx, p.ny, p.nz)
+        return s.value
+
+    def release(self, slot):
+        self._check(lib().les_batch_release(self.h, slot), "les_batch_release")
+        self.shapes.pop(slot, None)
+
+    def step(self, n=1):
+        ms = C.c_double()
+        self._check(lib().les_batch_step(self.h, n, C.byref(ms)), "les_batch_step")
+        return ms.value
+
+    def info(self, slot):
+        i = abi.SlotInfo()
+        self._check(lib().les_batch_slot_info(self.h, slot, C.byref(i)), "les_batch_slot_info")
+        return i
+
+    def poll(self):
+        cap = self.cfg.max_slots
+        arr = (C.c_int * cap)()
+        n = C.c_int()
+        self._check(lib().les_batch_poll(self.h, arr, cap, C.byref(n)), "les_batch_poll")
+        return [arr[i] for i in range(min(n.value, cap))]
+
+    def upload(self, slot, fields):
+        arrs = [np.ascontiguousarray(f, dtype=np.float64).reshape(-1) for f in fields]
+        self._check(lib().les_batch_upload(self.h, slot, _ptrs(arrs), len(arrs)), "les_batch_upload")
+
+    def download(self, slot, n=5):
+        shape = self.shapes[slot]
+        cells = shape[0] * shape[1] * shape[2]
+        out = [np.empty(cells) for _ in range(n)]
+        self._check(lib().les_batch_download(self.h, slot, _ptrs(out), n), "les_batch_download")
+        return [o.reshape(shape) for o in out]
+
+    def field(self, slot, name, count):
+        out = np.empty(count)
+        self._check(lib().les_batch_download_field(self.h, slot, name.encode(),
+                                                   out.ctypes.data_as(C.POINTER(C.c_double))),
+                    "les_batch_download_field(%s)" % name)
+        return out
+
+    def report(self, slot):
+        rep = abi.StepReport()
+        self._check(lib().les_batch_report(self.h, slot, C.byref(rep)), "les_batch_report")
+        return rep
+
+    def point_series(self, slot, max_steps=256):
+        out = np.zeros(max_steps * abi.LES_NPOINT_MAX * 4)
+        ns = C.c_int()
+        self._check(lib().les_batch_point_fetch(self.h, slot, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                                max_steps, C.byref(ns)), "les_batch_point_fetch")
+        return out[:ns.value * abi.LES_NPOINT_MAX * 4].reshape(ns.value, abi.LES_NPOINT_MAX, 4)

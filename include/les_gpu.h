@@ -221,8 +221,82 @@ int les_gpu_set_timing(les_gpu* ctx, int on);
 /* Number of steps accumulated in the FP64 statistics (:3191 time_avg_steps). */
 int les_gpu_stats_count(const les_gpu* ctx);
 /* sizeof(les_params), sizeof(les_mixture), sizeof(les_step_report),
- * sizeof(les_probe_row): lets bindings check their struct mirrors. */
-int les_abi_struct_sizes(size_t out[4]);
+ * sizeof(les_probe_row), sizeof(les_batch_config), sizeof(les_slot_info):
+ * lets bindings check their struct mirrors. */
+int les_abi_struct_sizes(size_t out[6]);
+
+/* ------------------------------------------------------------------------
+ * Batched runs (duct mode).  The reference runs one simulation per process
+ * (main :4303-4700) and its optimiser drives it as a subprocess
+ * (bayesian_optimizer.py:189-230, one trial at a time).  A batch context holds
+ * a POOL of run slots sharing field arenas in device memory; every step
+ * launch advances all resident, unfinished runs at once, so an ensemble of
+ * trials, or a stream of requests admitted and retired while others run
+ * (continuous batching), costs one launch sequence per step.  Each run keeps
+ * its own grid, parameters, time step and step counter, and its results are
+ * bitwise the same as when it runs alone in a pool with the same launch
+ * configuration.
+ * ------------------------------------------------------------------------ */
+#define LES_TILE_FLAT        0   /* tiles of 256 consecutive cells of one x-plane */
+
+typedef struct les_batch_config {
+    int       max_slots;       /* runs resident at once */
+    long long pool_cells;      /* cells of the shared arenas (sum over resident runs) */
+    int       max_nx, max_ny, max_nz, max_vortices;   /* per-run limits */
+    int       precision;       /* LES_PREC_*: one storage type per pool */
+    int       tile_k;          /* tile width in k: LES_TILE_FLAT, 32, 64 or 128 */
+    int       xcd_map;         /* 1: the blocks of one XCD take one contiguous tile range */
+    int       autotune;        /* 1: time tile-width and XCD-mapping candidates at create
+                                  on the probe grid and keep the fastest */
+    int       point_cap;       /* steps kept in each run's point-series ring */
+    int       stats_per_cell;  /* 1: per-cell FP64 accumulators (10 per cell) */
+    double    table_t0, table_dt;   /* cp/mu tables, shared by the pool */
+    int       table_n;
+} les_batch_config;
+
+typedef struct les_slot_info {
+    int    active;             /* 1 while admitted */
+    int    done;               /* 1 once step == n_steps */
+    int    step, n_steps;
+    double time, dt, max_velocity;
+    int    nx, ny, nz;
+    long long cells;
+} les_slot_info;
+
+typedef struct les_batch les_batch;   /* opaque pool context */
+
+int les_batch_config_default(les_batch_config* cfg);
+/* Device bytes a pool of this configuration allocates (admission budgeting:
+ * size pool_cells from the free HBM before creating the pool). */
+int les_batch_pool_bytes(const les_batch_config* cfg, size_t* bytes);
+/* probe (may be NULL) is the run shape the autotuner times. */
+int  les_batch_create(const les_batch_config* cfg, const les_params* probe, les_batch** out);
+void les_batch_destroy(les_batch* b);
+const char* les_batch_last_error(const les_batch* b);
+/* Launch configuration in use (after autotuning): tile_k, blocks per stencil
+ * launch (one per tile of the resident runs), XCD mapping, and the tuned step
+ * time on the probe grid (ms, 0 if not tuned). */
+int les_batch_tuning(const les_batch* b, int* tile_k, int* grid_blocks, int* xcd_map, double* tuned_ms);
+
+/* Admit a duct-mode run of n_steps steps (0: unbounded) into a free slot.
+ * LES_ERR_ADMISSION when no slot is free, its cells or tiles do not fit in
+ * the free arena ranges, or it exceeds the pool's per-run limits or
+ * precision.  init = 1 sets the initial state (else upload one). */
+int les_batch_admit(les_batch* b, const les_params* p, int n_steps, int init, int* slot);
+int les_batch_release(les_batch* b, int slot);
+/* Advance every resident, unfinished run by n_steps steps; *ms_per_step
+ * (may be NULL) is the device time per step (HIP events). */
+int les_batch_step(les_batch* b, int n_steps, double* ms_per_step);
+int les_batch_slot_info(les_batch* b, int slot, les_slot_info* info);
+/* Slots whose runs have finished (done and still resident). */
+int les_batch_poll(les_batch* b, int* slots, int max_slots, int* n_done);
+/* State of one run: rho*u, rho*v, rho*w, phi, T as double. */
+int les_batch_upload(les_batch* b, int slot, const double* const* fields, int n_fields);
+int les_batch_download(les_batch* b, int slot, double* const* fields, int n_fields);
+/* "profiles" (10 per x-plane), "inlet", "eddies", "stat0".."stat9", "nu_t", "ST". */
+int les_batch_download_field(les_batch* b, int slot, const char* name, double* out);
+int les_batch_report(les_batch* b, int slot, les_step_report* rep);
+int les_batch_point_fetch(les_batch* b, int slot, double* out, int max_steps, int* n_steps);
 
 #ifdef __cplusplus
 }

@@ -31,7 +31,7 @@ def test_engine_library_exports_all(lesgpu):
 
 
 def test_struct_sizes_match(lesgpu, abi):
-    out = (C.c_size_t * 4)()
+    out = (C.c_size_t * 6)()
     L = lesgpu.lib()
     L.les_abi_struct_sizes.argtypes = [C.POINTER(C.c_size_t)]
     assert L.les_abi_struct_sizes(out) == 0
@@ -39,6 +39,8 @@ def test_struct_sizes_match(lesgpu, abi):
     assert out[1] == C.sizeof(abi.Mixture)
     assert out[2] == C.sizeof(abi.StepReport)
     assert out[3] == C.sizeof(abi.ProbeRow)
+    assert out[4] == C.sizeof(abi.BatchConfig)
+    assert out[5] == C.sizeof(abi.SlotInfo)
 
 
 def test_presets_validate(lesgpu, abi):

@@ -1,0 +1,146 @@
+"""Batched duct runs (les_batch_*): a pool of runs advanced by one launch
+sequence per step.
+
+* a run in a pool gives bitwise the same state as the same run in a
+  single-run context (same tile width), whatever else is resident — other
+  grids, parameters, precisions of step counts, runs admitted and retired
+  mid-way (continuous batching);
+* every tile shape and launch configuration the autotuner may pick stays
+  within the FP64 tolerance (1e-10 of each field's scale) of the oracle;
+* admission refuses runs that do not fit and reuses freed arena ranges.
+"""
+import numpy as np
+import pytest
+
+from common import rel_err
+
+pytestmark = pytest.mark.gpu
+DUCT = 1        # LES_PRESET_DUCT64
+TOL = 1e-10
+
+
+def run_params(lesgpu, precision=0, **over):
+    base = dict(precision=precision, reinit_interval=4, reinit_iters=2)
+    base.update(over)
+    return lesgpu.params(DUCT, **base)
+
+
+def single_run(lesgpu, p, steps):
+    g = lesgpu.LesGpu(p)
+    try:
+        g.init_state()
+        g.step(steps)
+        return g.download_state(5), g.field("profiles", 10 * p.nx)
+    finally:
+        g.close()
+
+
+def pool(lesgpu, precision=0, slots=4, cells=None, **over):
+    kw = dict(max_slots=slots, pool_cells=cells or 4 * 64 * 48 * 48, max_nx=96, max_ny=64, max_nz=64,
+              max_vortices=400, precision=precision, tile_k=64, point_cap=64)
+    kw.update(over)
+    cfg = lesgpu.batch_config(**kw)
+    return lesgpu.Batch(cfg)
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_member_in_mix_equals_alone(lesgpu, precision):
+    print("case: batch member vs single run, precision=%d" % precision, flush=True)
+    pa = run_params(lesgpu, precision, seed=7)
+    alone, prof_alone = single_run(lesgpu, pa, 9)
+    with pool(lesgpu, precision) as b:
+        others = [
+            run_params(lesgpu, precision, nx=48, ny=40, nz=24, seed=3, inlet_velocity=4.0),
+            run_params(lesgpu, precision, nx=96, ny=48, nz=48, seed=11, advection=2, reinit_interval=3),
+        ]
+        s1 = b.admit(others[0], n_steps=4)          # finishes first, then sits done
+        sa = b.admit(pa, n_steps=9)
+        b.step(3)
+        s2 = b.admit(others[1], n_steps=20)         # admitted mid-way
+        b.step(2)
+        assert s1 in b.poll()
+        b.release(s1)
+        s3 = b.admit(run_params(lesgpu, precision, nx=40, ny=32, nz=32, seed=5), n_steps=5)
+        b.step(4)
+        assert b.info(sa).step == 9 and b.info(sa).done == 1
+        st = b.download(sa)
+        for nm, x, y in zip(["rhou", "rhov", "rhow", "phi", "T"], st, alone):
+            assert np.array_equal(x, y), nm
+        assert np.array_equal(b.field(sa, "profiles", 10 * pa.nx), prof_alone)
+        assert b.info(s2).step == 6 and b.info(s3).step == 4
+        for s in (s2, s3):
+            rep = b.report(s)
+            assert not rep.nonfinite
+
+
+def test_runs_admitted_later_match_alone(lesgpu):
+    print("case: run admitted into a reused slot vs alone", flush=True)
+    pc = run_params(lesgpu, 0, nx=40, ny=32, nz=32, seed=21)
+    alone, _ = single_run(lesgpu, pc, 6)
+    with pool(lesgpu, 0, slots=2, cells=64 * 32 * 32 + 40 * 32 * 32) as b:
+        s0 = b.admit(run_params(lesgpu, 0, seed=1), n_steps=2)
+        b.step(2)
+        b.release(s0)
+        sc = b.admit(pc, n_steps=6)
+        b.step(8)                                   # two steps past its end: it must not move
+        assert b.poll() == [sc]
+        for x, y in zip(b.download(sc), alone):
+            assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("tile_k,xcd", [(0, 1), (32, 1), (128, 0), (64, 0)],
+                         ids=["flat_xcd", "tk32_xcd", "tk128", "tk64"])
+def test_launch_shapes_vs_oracle(lesgpu, oracle, tile_k, xcd):
+    print("case: batch tile_k=%d xcd=%d vs oracle" % (tile_k, xcd), flush=True)
+    over = dict(reinit_interval=3, reinit_iters=2)
+    p = lesgpu.params(DUCT, **over)
+    o = oracle.Oracle(oracle.params(DUCT, **over))
+    with pool(lesgpu, 0, slots=2, tile_k=tile_k, xcd_map=xcd) as b:
+        s = b.admit(p, n_steps=0)
+        # a second, differently shaped run shares every launch
+        b.admit(lesgpu.params(DUCT, nx=48, ny=24, nz=40, seed=9), n_steps=0)
+        o.init_state()
+        b.step(6)
+        o.step(6)
+        for nm, x, y in zip(["rhou", "rhov", "rhow", "phi", "T"], b.download(s), o.get_state(5)):
+            assert rel_err(x, y) <= TOL, (nm, rel_err(x, y))
+    o.close()
+
+
+def test_admission_limits(lesgpu):
+    print("case: admission refusal and reuse", flush=True)
+    with pool(lesgpu, 1, slots=3, cells=2 * 64 * 32 * 32) as b:
+        p = run_params(lesgpu, 1)
+        s0 = b.admit(p, 5)
+        s1 = b.admit(p, 5)
+        assert s0 is not None and s1 is not None
+        assert b.admit(p, 5) is None                             # arena full
+        assert b.admit(run_params(lesgpu, 1, nx=128), 5) is None  # beyond max_nx
+        assert b.admit(run_params(lesgpu, 0), 5) is None          # wrong precision
+        b.release(s0)
+        s2 = b.admit(run_params(lesgpu, 1, nx=32), 5)            # fits the freed range
+        assert s2 is not None
+        b.step(1)
+        assert b.info(s2).step == 1
+
+
+def test_batch_point_series_and_autotune(lesgpu):
+    print("case: batch point series + autotuner", flush=True)
+    probe = run_params(lesgpu, 1, nx=64, ny=48, nz=48)
+    cfg = lesgpu.batch_config(max_slots=2, pool_cells=2 * 64 * 48 * 48, max_nx=64, max_ny=48, max_nz=48,
+                              max_vortices=400, precision=1, autotune=1, point_cap=8)
+    with lesgpu.Batch(cfg, probe) as b:
+        t = b.tuning()
+        print("tuned:", t, flush=True)
+        assert t["tile_k"] in (0, 32, 64, 128) and t["grid_blocks"] > 0 and t["tuned_ms"] > 0
+        s = b.admit(probe, 12)
+        b.step(12)
+        ser = b.point_series(s, 64)
+        assert ser.shape == (8, 64, 4)                           # ring keeps the newest 8 steps
+        st = b.download(s)
+        rho = 1.2 * 300.0 / st[4]
+        u = (st[0] / rho).reshape(-1)
+        nx, ny, nz = 64, 48, 48
+        idx = [((a + 1) * nx // 5 * ny + (2 * c + 1) * ny // 8) * nz + (2 * e + 1) * nz // 8
+               for a in range(4) for c in range(4) for e in range(4)]
+        assert np.allclose(ser[-1, :, 0], u[idx], rtol=1e-6, atol=0)
