@@ -3,10 +3,19 @@
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
 
 A "step" is one full time step (all RK stages, SEM inflow, boundary
-conditions, probes) of one run on its synthetic initial state, with the state
-resident in HBM.  The metric is cell-updates per second per RK stage:
-nx*ny*nz * stages * K / (GPU time of the K timed steps).  One JSON line is
-printed (rank 0).
+conditions, reinitialisation when due, statistics, point series) of the
+synthetic initial state, with the state resident in HBM.
+
+Workloads (one JSON line each, rank 0):
+  duct128-fp64 (default)  128^3 duct, FP64 plain path: cell-updates/s per RK
+                          stage = nx*ny*nz * stages * K / (GPU time of K steps);
+                          also carries the FP32 form of the same run and the
+                          64-member ensemble (runs/hour, ratio to the plain
+                          FP64 path at the same grid, profile error).
+  duct128-fp32, duct128-muscl-fp64, ref128: single-run variants.
+  ensemble64-fp32         64 x 64x48x48 FP32 members in one pool: LES runs/hour.
+  served96-fp32           96 requests, Poisson arrivals (mean gap 2 s), uneven
+                          grids, admitted by HBM footprint: runs/hour, p50/p99.
 """
 import argparse
 import importlib
@@ -14,6 +23,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -41,7 +52,10 @@ WORKLOADS = {
     "duct128-muscl-fp64": dict(preset=2, nx=128, ny=128, nz=128, stages=3, bytes_per_cell_stage=106.7,
                                over=dict(precision=0, advection=2), dtype="f64",
                                desc="128^3 periodic duct, MUSCL-minmod, SSP-RK3 CFL 0.3, FP64"),
+    "ensemble64-fp32": dict(dtype="f32", desc="64 ensemble members, 64x48x48 each, FP32, one pool"),
+    "served96-fp32": dict(dtype="f32", desc="96 served requests, Poisson arrivals (mean gap 2 s)"),
 }
+RHS_KERNELS = "k_duct_rhs_scalar + k_duct_rhs_momentum (one RK stage's RHS and combine, two launches)"
 
 
 def parse():
@@ -51,33 +65,198 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="duct128-fp64", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the FP32 and ensemble legs of the default line")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--time-scale", type=float, default=1.0, help="served stream: arrival-time scale")
+    ap.add_argument("--requests", type=int, default=96, help="served stream: number of requests")
     return ap.parse_args()
 
 
-def cpu_baseline(wl, threads):
-    """Oracle (C + OpenMP restatement) timed on this host on a bounded sample:
-    one full time step of the same workload."""
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(workload):
+    """HBM bytes per launch of the RHS kernels as measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), where recorded."""
+    path = os.path.join(ROOT, "profiles", "rhs_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    return rec.get(workload)
+
+
+def cpu_baseline(preset, nx, ny, nz, over, stages, threads, steps=1):
+    """Oracle (C + OpenMP restatement, FP64) timed on this host on a bounded
+    sample: `steps` full time steps of the same grid."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     if threads > 0:
         os.environ["OMP_NUM_THREADS"] = str(threads)
     import pyoracle
-    p = pyoracle.params(wl["preset"])
-    p.nx, p.ny, p.nz = wl["nx"], wl["ny"], wl["nz"]
-    for k, v in wl["over"].items():
+    p = pyoracle.params(preset)
+    p.nx, p.ny, p.nz = nx, ny, nz
+    for k, v in over.items():
         if k != "precision":          # the oracle computes in FP64 only
             setattr(p, k, v)
     o = pyoracle.Oracle(p, pyoracle.mixture())
     o.init_state()
     t0 = time.perf_counter()
-    o.step(1)
-    dt = time.perf_counter() - t0
+    o.step(steps)
+    dt = (time.perf_counter() - t0) / steps
     o.close()
-    cells = wl["nx"] * wl["ny"] * wl["nz"]
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-    return {"value": cells * wl["stages"] / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": "1 full time step (%d RK stages) of the same %dx%dx%d workload, oracle C+OpenMP, %.2f s"
-                      % (wl["stages"], wl["nx"], wl["ny"], wl["nz"], dt)}
+    return dt, cores
+
+
+def single_run(lesgpu, p, steps, warmup, timing=True):
+    g = lesgpu.LesGpu(p, lesgpu.mixture())
+    try:
+        g.init_state()
+        if warmup:
+            g.step(warmup)
+        t0 = time.perf_counter()
+        rep = g.step(steps)
+        wall = time.perf_counter() - t0
+        _, _, step_ms = g.kernel_timing()
+        rhs_ms, launches = 0.0, 0
+        if timing:
+            g.set_timing(True)
+            g.step(2)
+            rhs_ms, launches, _ = g.kernel_timing()
+            g.set_timing(False)
+        return dict(step_ms=step_ms, wall=wall, rhs_ms=rhs_ms, launches=launches, dt=rep.dt)
+    finally:
+        g.close()
+
+
+def single_line(a, lesgpu, name):
+    wl = WORKLOADS[name]
+    p = lesgpu.params(wl["preset"], nx=wl["nx"], ny=wl["ny"], nz=wl["nz"], **wl["over"])
+    log("%s warmup %d, %d timed steps" % (name, a.warmup, a.steps))
+    r = single_run(lesgpu, p, a.steps, a.warmup)
+    cells = wl["nx"] * wl["ny"] * wl["nz"]
+    value = cells * wl["stages"] / (r["step_ms"] * 1e-3)
+    stage_ms = r["step_ms"] / wl["stages"]
+    B = wl["bytes_per_cell_stage"]
+    duct = wl["preset"] != 0
+    # dominant kernel(s): the RHS of one stage, HIP events around its launches
+    # on the engine stream; ALGORITHMIC bytes of one stage / their duration
+    k_ms = r["rhs_ms"] if r["rhs_ms"] > 0 else stage_ms
+    achieved = cells * B / (k_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r["step_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": wl["dtype"],
+        "data": "synthetic",
+        "config": {"workload": wl["desc"], "grid": [wl["nx"], wl["ny"], wl["nz"]],
+                   "stages_per_step": wl["stages"], "wall_s": r["wall"], "last_dt": r["dt"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles(name),
+                     "kernel": RHS_KERNELS if duct else "k_ref_rhs (one RK4 stage's RHS)",
+                     "kernel_ms_per_stage": k_ms, "launches_timed": r["launches"],
+                     "bytes_per_cell_stage": B, "cells": cells,
+                     "whole_stage_achieved": cells * B / (stage_ms * 1e-3) / 1e9},
+    }
+    return out, p
+
+
+def ensemble_leg(lesgpu, serving, steps, warmup, members=64, grid=(64, 48, 48), profile_steps=50):
+    """64 members in one FP32 pool vs the plain FP64 path one run at a time."""
+    ms = serving.ensemble_members(members, grid, seed=0)
+    cells = grid[0] * grid[1] * grid[2]
+    cfg = serving.pool_config(members, members * cells, grid, autotune=True)
+    log("ensemble: %d members %s, autotuning" % (members, grid))
+    with lesgpu.Batch(cfg, ms[0]) as b:
+        tune = b.tuning()
+        slots = [b.admit(p, 0, init=True) for p in ms]
+        assert all(s is not None for s in slots)
+        b.step(warmup)
+        t0 = time.perf_counter()
+        step_ms = b.step(steps)
+        wall = time.perf_counter() - t0
+    runs_per_hour = members * 3600.0 / (1000 * step_ms * 1e-3)
+    cu = members * cells * 3 / (step_ms * 1e-3)
+    # plain FP64 path, one run at a time, same grid
+    p64 = serving.ensemble_members(1, grid, seed=0, precision=0)[0]
+    log("ensemble: plain FP64 single run")
+    r64 = single_run(lesgpu, p64, steps, warmup, timing=False)
+    plain_rph = 3600.0 / (1000 * r64["step_ms"] * 1e-3)
+    err = profile_error(lesgpu, serving, grid, profile_steps)
+    return {"metric": "LES runs/hour (64 members x 1000 steps, 64x48x48, FP32, one pool)",
+            "value": runs_per_hour, "ms_per_pool_step": step_ms, "wall_s": wall,
+            "cell_updates_per_s_stage": cu, "tuning": tune,
+            "plain_fp64_runs_per_hour": plain_rph, "plain_fp64_ms_per_step": r64["step_ms"],
+            "ratio_vs_plain_fp64": runs_per_hour / plain_rph,
+            "reynolds_profile_rel_err_fp32_vs_fp64": err}
+
+
+def profile_error(lesgpu, serving, grid, steps):
+    """Max relative error (per profile column, against its own scale) of the
+    FP32 member's mean/Reynolds-stress plane profiles vs the FP64 path after
+    `steps` steps (the FP64 GPU path equals the FP64 oracle within 1e-10)."""
+    out = {}
+    profs = {}
+    for prec in (1, 0):
+        p = serving.ensemble_members(1, grid, seed=0, precision=prec)[0]
+        g = lesgpu.LesGpu(p, lesgpu.mixture())
+        try:
+            g.init_state()
+            g.step(steps)
+            profs[prec] = serving.reynolds_profiles(g.field("profiles", 10 * p.nx), p.nx, steps * 1.0)
+        finally:
+            g.close()
+    names = ["u", "v", "w", "c", "uu", "vv", "ww", "uv", "uw", "vw"]
+    a, b = profs[1], profs[0]
+    for k, nm in enumerate(names):
+        scale = max(np.abs(b[:, k]).max(), 1e-300)
+        out[nm] = float(np.abs(a[:, k] - b[:, k]).max() / scale)
+    out["steps"] = steps
+    return out
+
+
+def ensemble_line(a, lesgpu, serving):
+    e = ensemble_leg(lesgpu, serving, a.steps, a.warmup)
+    out = {"metric": e.pop("metric"), "value": e.pop("value"), "unit": "runs/hour", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": e["ms_per_pool_step"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic", "config": {"workload": WORKLOADS["ensemble64-fp32"]["desc"],
+                                           "members": 64, "grid": [64, 48, 48], "steps_per_run": 1000}}
+    out.update(e)
+    return out
+
+
+def served_line(a, lesgpu, serving):
+    reqs = serving.served_requests(a.requests, 2.0, seed=0)
+    budget = 8 << 30          # HBM budget of the pool (admission by footprint)
+    cfg = serving.pool_for_budget(budget, 64, (128, 64, 64))
+    probe = reqs[0][1]
+    log("served: %d requests, pool of %d cells (%.1f GiB budget)" % (len(reqs), cfg.pool_cells, budget / 2 ** 30))
+    with lesgpu.Batch(cfg, probe) as b:
+        srv = serving.Server(b, chunk=10)
+        wall = serving.run_stream(srv, reqs, a.time_scale, progress=log)
+        lat = np.array([srv.results[r]["latency_s"] for r in range(len(reqs))])
+        bad = sum(srv.results[r]["nonfinite"] for r in range(len(reqs)))
+        tune = b.tuning()
+    cell_steps = sum(p.nx * p.ny * p.nz * n for _, p, n in reqs)
+    return {"metric": "LES runs/hour + p50/p99 latency (96 served requests, FP32)",
+            "value": len(reqs) * 3600.0 / wall, "unit": "runs/hour", "n_gpus": 1,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": WORKLOADS["served96-fp32"]["desc"], "requests": len(reqs),
+                       "grids": serving.SERVED_GRIDS, "steps": "U[500,2000]", "time_scale": a.time_scale,
+                       "pool_cells": cfg.pool_cells, "hbm_budget_bytes": budget},
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()), "wall_s": wall, "gpu_busy_s": srv.gpu_ms * 1e-3,
+            "pool_steps": srv.steps_run, "cell_steps": cell_steps, "nonfinite_runs": int(bad), "tuning": tune}
 
 
 def main():
@@ -87,51 +266,29 @@ def main():
               % a.gpus, file=sys.stderr)
         sys.exit(2)
     lesgpu = importlib.import_module("combustion-les-optimization_amd.lesgpu")
+    serving = importlib.import_module("combustion-les-optimization_amd.serving")
+    if a.workload == "ensemble64-fp32":
+        print(json.dumps(ensemble_line(a, lesgpu, serving)))
+        return
+    if a.workload == "served96-fp32":
+        print(json.dumps(served_line(a, lesgpu, serving)))
+        return
+    out, p = single_line(a, lesgpu, a.workload)
     wl = WORKLOADS[a.workload]
-    p = lesgpu.params(wl["preset"], nx=wl["nx"], ny=wl["ny"], nz=wl["nz"], **wl["over"])
-    g = lesgpu.LesGpu(p, lesgpu.mixture())
-    g.init_state()
-    print("bench: %s warmup %d" % (a.workload, a.warmup), file=sys.stderr, flush=True)
-    if a.warmup:
-        g.step(a.warmup)
-    # timed region: K steps, synchronised on both sides inside les_gpu_step
-    # (HIP events on the engine stream bracket the K steps)
-    t0 = time.perf_counter()
-    rep = g.step(a.steps)
-    wall = time.perf_counter() - t0
-    _, _, step_ms = g.kernel_timing()
-    # dominant kernel: RHS, timed per launch with HIP events on the same stream
-    g.set_timing(True)
-    g.step(2)
-    rhs_ms, launches, _ = g.kernel_timing()
-    g.set_timing(False)
-    cells = wl["nx"] * wl["ny"] * wl["nz"]
-    gpu_s = step_ms * a.steps / 1000.0
-    value = cells * wl["stages"] * a.steps / gpu_s
-    stage_ms = step_ms / wl["stages"]
-    achieved = cells * wl["bytes_per_cell_stage"] / (stage_ms * 1e-3) / 1e9
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": step_ms,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": wl["dtype"],
-        "data": "synthetic",
-        "config": {"workload": wl["desc"], "grid": [wl["nx"], wl["ny"], wl["nz"]],
-                   "stages_per_step": wl["stages"], "wall_s": wall, "last_dt": rep.dt},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "whole RK stage", "rhs_kernel_ms": rhs_ms, "rhs_launches": launches},
-    }
-    g.close()
+    if a.workload == "duct128-fp64" and not a.no_extra:
+        f32, _ = single_line(a, lesgpu, "duct128-fp32")
+        out["fp32_form"] = {"value": f32["value"], "ms_per_step": f32["ms_per_step"], "roofline": f32["roofline"]}
+        out["ensemble"] = ensemble_leg(lesgpu, serving, a.steps, a.warmup)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, a.cpu_threads)
+        log("cpu baseline")
+        dt, cores = cpu_baseline(wl["preset"], wl["nx"], wl["ny"], wl["nz"], wl["over"], wl["stages"],
+                                 a.cpu_threads)
+        cells = wl["nx"] * wl["ny"] * wl["nz"]
+        out["cpu_baseline"] = {"value": cells * wl["stages"] / dt, "unit": "cell-updates/s", "cores": cores,
+                               "kind": "port",
+                               "sample": "1 full time step (%d RK stages) of the same %dx%dx%d workload, "
+                                         "oracle C+OpenMP FP64, %.2f s" % (wl["stages"], wl["nx"], wl["ny"],
+                                                                           wl["nz"], dt)}
     print(json.dumps(out))
 
 

@@ -113,6 +113,7 @@ class StepReport(C.Structure):
         ("min_Y_fuel", C.c_double),
         ("nonfinite", C.c_int),
         ("proj_residual", C.c_double),
+        ("proj_residual0", C.c_double),
     ]
 
 

@@ -22,8 +22,9 @@
 //   k_duct_sgs         nu_t, S_T (+ per-tile max nu / alpha for the next DT)
 //   k_duct_rhs_scalar  phi and T: RHS fused with the SSP-RK3 combine
 //   k_duct_rhs_momentum rho*u, rho*v, rho*w: RHS fused with the combine
-// Per step: DT, eddy update, inflow plane, its normalisation, inflow/outflow
-// planes, reinitialisation (slots due), statistics, point series, step end.
+// Per step: DT, eddy update, inflow plane, its normalisation, the 3 stages,
+// projection relaxation (runs with proj_iters > 0), inflow/outflow planes,
+// reinitialisation (runs due), statistics, point series, step end.
 #include "les_engine.hpp"
 
 // The duct scheme is this engine's own specification: its kernels may fuse
@@ -52,6 +53,7 @@ struct DuctConsts {
     int scheme;             // LES_ADV_WENO5 or LES_ADV_MUSCL_MINMOD
     int nvort;
     int reinit_interval, reinit_iters;
+    int proj_iters;         // projection relaxation sweeps per step (0: off)
     int stats_start, stats_per_cell;
     double L;               // eddy length scale
     double ly, lz;
@@ -77,6 +79,7 @@ struct SlotDesc {
 // Run state the kernels update (never written by the host while a step runs).
 struct SlotDyn {
     double dt, rho_avg, maxvel, time;
+    double proj_res0, proj_res;   // FP64 RMS residuals of the last projection
     int step;               // steps done
     int done;               // step >= n_steps
     int pad[2];
@@ -105,6 +108,7 @@ struct Pool {
     double* dpart;          // [3][tile][2]: max nu, alpha per stage
     double* bcpart;         // [slot][ny_max][nzb]: max |u| of the inflow plane
     double* spart;          // [slot][nx_max][STAT_CHUNKS][10]
+    double* ppart;          // per tile: FP64 sums of b^2 and r^2 of the projection
     int tile_cap;
     int ny_max, nz_max, nx_max, nzb;
     int point_cap;          // steps per point-series ring
@@ -1039,6 +1043,149 @@ __global__ void __launch_bounds__(256) k_duct_reinit(Pool P, int phase, int iter
 }
 
 // ---------------------------------------------------------------------------
+// projection relaxation of the mass flux m = rho*u on interior planes:
+// b = div m (central, periodic y/z); proj_iters Richardson sweeps
+// phi <- phi + (L phi - b)/d from phi = 0 on the compact Laplacian with ghost
+// rules phi(0) = phi(1) (inflow, Neumann) and phi(nx-1) = 0 (outflow); then
+// m <- m - grad phi.  b in B[0], phi ping-pongs B[1] -> B[2] -> B[1] ...
+// (B is dead after the last stage).  FP64 per-tile sums of b^2 and of the
+// final residual (b - L phi)^2, reduced per run in a fixed order.
+// ---------------------------------------------------------------------------
+template <typename real>
+__device__ __forceinline__ real proj_phi(const real* phi, const GridDims& g, int i, int j, int kk)
+{
+    if (i <= 0) i = 1;
+    if (i >= g.nx - 1) return (real)0.0;
+    return phi[i * (int)g.sx + wrapi(j, g.ny) * g.nz + wrapi(kk, g.nz)];
+}
+
+template <typename real>
+__device__ __forceinline__ real proj_lap(const real* phi, const GridDims& g, int i, int j, int kk)
+{
+    const real ix2 = (real)(1.0 / (g.dx * g.dx)), iy2 = (real)(1.0 / (g.dy * g.dy)), iz2 = (real)(1.0 / (g.dz * g.dz));
+    const real p0 = phi[i * (int)g.sx + j * g.nz + kk];
+    real lx = (proj_phi(phi, g, i + 1, j, kk) - (real)2.0 * p0 + proj_phi(phi, g, i - 1, j, kk)) * ix2;
+    real ly = (proj_phi(phi, g, i, j + 1, kk) - (real)2.0 * p0 + proj_phi(phi, g, i, j - 1, kk)) * iy2;
+    real lz = (proj_phi(phi, g, i, j, kk + 1) - (real)2.0 * p0 + proj_phi(phi, g, i, j, kk - 1)) * iz2;
+    return lx + ly + lz;
+}
+
+template <typename real, int TK>
+__global__ void __launch_bounds__(256) k_duct_proj_div(Pool P, int xcd_map)
+{
+    __shared__ double red[256];
+    const int t = tile_of_block(P, xcd_map);
+    TilePos tp;
+    if (t < 0 || !tile_decode<TK>(P, t, tp)) return;
+    const SlotView D = slot_view(P, tp.s);
+    if (D.k.proj_iters <= 0) return;
+    const GridDims g = D.g;
+    const int i = tp.i, j = tp.j, kk = tp.kk;
+    double sb = 0.0;
+    if (tp.in && i > 0 && i < g.nx - 1) {
+        const int q = i * (int)g.sx + j * g.nz + kk;
+        const real* m0 = (const real*)P.A[0] + D.off;
+        const real* m1 = (const real*)P.A[1] + D.off;
+        const real* m2 = (const real*)P.A[2] + D.off;
+        const real hx = (real)(0.5 / g.dx), hy = (real)(0.5 / g.dy), hz = (real)(0.5 / g.dz);
+        real v = (m0[q + (int)g.sx] - m0[q - (int)g.sx]) * hx +
+                 (m1[i * (int)g.sx + wrapi(j + 1, g.ny) * g.nz + kk] - m1[i * (int)g.sx + wrapi(j - 1, g.ny) * g.nz + kk]) * hy +
+                 (m2[i * (int)g.sx + j * g.nz + wrapi(kk + 1, g.nz)] - m2[i * (int)g.sx + j * g.nz + wrapi(kk - 1, g.nz)]) * hz;
+        ((real*)P.B[0] + D.off)[q] = v;
+        sb = (double)v * (double)v;
+    }
+    double tot = block_sum256(sb, red);
+    if (threadIdx.x == 0) {
+        P.ppart[2 * (long long)t] = tot;
+        P.ppart[2 * (long long)t + 1] = 0.0;
+    }
+}
+
+template <typename real, int TK>
+__global__ void __launch_bounds__(256) k_duct_proj_sweep(Pool P, int it, int xcd_map)
+{
+    const int t = tile_of_block(P, xcd_map);
+    TilePos tp;
+    if (t < 0 || !tile_decode<TK>(P, t, tp)) return;
+    const SlotView D = slot_view(P, tp.s);
+    const GridDims g = D.g;
+    const int i = tp.i;
+    if (it >= D.k.proj_iters || !tp.in || i == 0 || i == g.nx - 1) return;
+    const int q = i * (int)g.sx + tp.j * g.nz + tp.kk;
+    const real* po = (const real*)P.B[1 + (it & 1)] + D.off;
+    real* pn = (real*)P.B[1 + ((it + 1) & 1)] + D.off;
+    const real b = ((const real*)P.B[0] + D.off)[q];
+    const real inv_d = (real)(1.0 / (2.0 / (g.dx * g.dx) + 2.0 / (g.dy * g.dy) + 2.0 / (g.dz * g.dz)));
+    if (it == 0) pn[q] = ((real)0.0 - b) * inv_d;           // phi = 0 before the first sweep
+    else pn[q] = po[q] + (proj_lap<real>(po, g, i, tp.j, tp.kk) - b) * inv_d;
+}
+
+template <typename real, int TK>
+__global__ void __launch_bounds__(256) k_duct_proj_correct(Pool P, int xcd_map)
+{
+    __shared__ double red[256];
+    const int t = tile_of_block(P, xcd_map);
+    TilePos tp;
+    if (t < 0 || !tile_decode<TK>(P, t, tp)) return;
+    const SlotView D = slot_view(P, tp.s);
+    if (D.k.proj_iters <= 0) return;
+    const GridDims g = D.g;
+    const int i = tp.i, j = tp.j, kk = tp.kk;
+    double sr = 0.0, vm = 0.0;
+    if (tp.in && i > 0 && i < g.nx - 1) {
+        const int q = i * (int)g.sx + j * g.nz + kk;
+        const real* phi = (const real*)P.B[1 + (D.k.proj_iters & 1)] + D.off;
+        const real b = ((const real*)P.B[0] + D.off)[q];
+        const real r = b - proj_lap<real>(phi, g, i, j, kk);
+        sr = (double)r * (double)r;
+        const real hx = (real)(0.5 / g.dx), hy = (real)(0.5 / g.dy), hz = (real)(0.5 / g.dz);
+        real* m0 = (real*)P.A[0] + D.off;
+        real* m1 = (real*)P.A[1] + D.off;
+        real* m2 = (real*)P.A[2] + D.off;
+        const real a0 = m0[q] - (proj_phi(phi, g, i + 1, j, kk) - proj_phi(phi, g, i - 1, j, kk)) * hx;
+        const real a1 = m1[q] - (proj_phi(phi, g, i, j + 1, kk) - proj_phi(phi, g, i, j - 1, kk)) * hy;
+        const real a2 = m2[q] - (proj_phi(phi, g, i, j, kk + 1) - proj_phi(phi, g, i, j, kk - 1)) * hz;
+        m0[q] = a0; m1[q] = a1; m2[q] = a2;
+        const real rho = (real)D.k.rho_u_Tu / ((const real*)P.A[4] + D.off)[q];
+        const real uu = a0 / rho, vv = a1 / rho, ww = a2 / rho;
+        vm = (double)sqrt(uu * uu + vv * vv + ww * ww);
+    }
+    double tot = block_sum256(sr, red);
+    double dummy = 0.0;
+    block_max2_256(vm, dummy, red);
+    if (threadIdx.x == 0) {
+        P.ppart[2 * (long long)t + 1] = tot;
+        P.vpart[t] = vm;           // max |u| of the corrected state (next step's DT)
+    }
+}
+
+// per-run FP64 residual RMS before and after the sweeps
+__global__ void __launch_bounds__(1024) k_duct_proj_final(Pool P)
+{
+    __shared__ double r0[1024], r1[1024];
+    const int s = blockIdx.x;
+    const SlotDesc& D = P.desc[s];
+    if (!D.active || P.dyn[s].done || D.k.proj_iters <= 0) return;
+    double a = 0.0, b = 0.0;
+    for (int u = threadIdx.x; u < D.ntiles; u += 1024) {
+        a += P.ppart[2 * (long long)(D.tile0 + u)];
+        b += P.ppart[2 * (long long)(D.tile0 + u) + 1];
+    }
+    r0[threadIdx.x] = a;
+    r1[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) { r0[threadIdx.x] += r0[threadIdx.x + w]; r1[threadIdx.x] += r1[threadIdx.x + w]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double n_int = (double)(D.g.nx - 2) * D.g.ny * D.g.nz;
+        P.dyn[s].proj_res0 = sqrt(r0[0] / n_int);
+        P.dyn[s].proj_res = sqrt(r1[0] / n_int);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // statistics: per-cell FP64 accumulators (:3167-3192) + y-z plane profiles.
 // Grid (chunks, nx_max, slots); chunk sums are added in a fixed order.
 // ---------------------------------------------------------------------------
@@ -1306,6 +1453,7 @@ struct DuctState {
     double* stage64 = nullptr;
     double* mon_part = nullptr;
     int max_reinit_iters = 0;
+    int max_proj_iters = 0;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   // [0] plain step, [1] with reinitialisation
     bool graph_ok[2] = {false, false};
     int use_graph = 1;
@@ -1331,6 +1479,7 @@ static DuctConsts make_duct_consts(const les_params& p)
     k.scheme = p.advection;
     k.nvort = p.num_vortices;
     k.reinit_interval = p.reinit_interval; k.reinit_iters = p.reinit_iters;
+    k.proj_iters = p.proj_iters;
     k.stats_start = p.stats_start_step; k.stats_per_cell = p.stats_per_cell;
     k.L = p.turbulent_length_scale;
     k.ly = p.ly; k.lz = p.lz;
@@ -1420,13 +1569,14 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
     P.dpart = (double*)dalloc_bytes(c, 8 * 2 * 3 * (size_t)d->tile_cap);
     P.bcpart = (double*)dalloc_bytes(c, 8 * S * d->ny_max * d->nzb);
     P.spart = (double*)dalloc_bytes(c, 8 * 10 * S * d->nx_max * STAT_CHUNKS);
+    P.ppart = (double*)dalloc_bytes(c, 8 * 2 * (size_t)d->tile_cap);
     d->d_desc = (SlotDesc*)dalloc_bytes(c, sizeof(SlotDesc) * S);
     d->d_dyn = (SlotDyn*)dalloc_bytes(c, sizeof(SlotDyn) * S);
     d->d_tile_slot = (int*)dalloc_bytes(c, 4 * (size_t)d->tile_cap);
     d->d_ntile = (int*)dalloc_bytes(c, 64);
     d->mon_part = (double*)dalloc_bytes(c, 8 * 6 * 1024);
     if (!P.eddy || !P.inlet || !P.prof || !P.points || !P.rho_part || !P.vpart || !P.dpart || !P.bcpart ||
-        !P.spart || !d->d_desc || !d->d_dyn || !d->d_tile_slot || !d->d_ntile || !d->mon_part)
+        !P.spart || !P.ppart || !d->d_desc || !d->d_dyn || !d->d_tile_slot || !d->d_ntile || !d->mon_part)
         return LES_ERR_HIP;
     if (d->precision == LES_PREC_FP32 && !(d->stage64 = (double*)dalloc_bytes(c, 8 * (size_t)d->pool_cells)))
         return LES_ERR_HIP;
@@ -1461,11 +1611,14 @@ static int pool_sync_tables(les_gpu* c)
     DHIPC(hipMemcpyAsync(d->d_tile_slot, d->tile_slot.data(), 4 * (size_t)d->tile_cap, hipMemcpyHostToDevice, c->stream));
     DHIPC(hipMemcpyAsync(d->d_ntile, &d->ntile_hwm, 4, hipMemcpyHostToDevice, c->stream));
     DHIPC(hipStreamSynchronize(c->stream));
-    int maxit = 0;
-    for (auto& x : d->desc)
+    int maxit = 0, maxpj = 0;
+    for (auto& x : d->desc) {
         if (x.active && x.k.reinit_interval > 0 && x.k.reinit_iters > maxit) maxit = x.k.reinit_iters;
-    if (maxit != d->max_reinit_iters) { d->graph_ok[0] = d->graph_ok[1] = false; }
+        if (x.active && x.k.proj_iters > maxpj) maxpj = x.k.proj_iters;
+    }
+    if (maxit != d->max_reinit_iters || maxpj != d->max_proj_iters) d->graph_ok[0] = d->graph_ok[1] = false;
     d->max_reinit_iters = maxit;
+    d->max_proj_iters = maxpj;
     return LES_OK;
 }
 
@@ -1478,7 +1631,8 @@ static int pool_admit(les_gpu* c, const les_params& p, int n_steps, int* slot_ou
         (p.advection != LES_ADV_WENO5 && p.advection != LES_ADV_MUSCL_MINMOD))
         return LES_ERR_PARAM;
     if (p.precision != d->precision || p.nx > d->nx_max || p.ny > d->ny_max || p.nz > d->nz_max ||
-        p.num_vortices > d->nvort_max || p.reinit_iters < 0 || p.reinit_iters > 64)
+        p.num_vortices > d->nvort_max || p.reinit_iters < 0 || p.reinit_iters > 64 || p.proj_iters < 0 ||
+        p.proj_iters > 256)
         return LES_ERR_ADMISSION;
     int s = -1;
     for (int a = 0; a < d->max_slots; ++a) if (!d->taken[a]) { s = a; break; }
@@ -1583,6 +1737,15 @@ static int step_launches(les_gpu* c, bool reinit)
     for (int s = 0; s < 3; ++s) {
         int e = stage_launch<real, TK>(c, s, nullptr);
         if (e != LES_OK) return e;
+    }
+    if (d->max_proj_iters > 0) {
+        hipLaunchKernelGGL((k_duct_proj_div<real, TK>), dim3(d->grid), dim3(256), 0, c->stream, d->P, d->xcd_map);
+        for (int it = 0; it < d->max_proj_iters; ++it)
+            hipLaunchKernelGGL((k_duct_proj_sweep<real, TK>), dim3(d->grid), dim3(256), 0, c->stream, d->P, it,
+                               d->xcd_map);
+        hipLaunchKernelGGL((k_duct_proj_correct<real, TK>), dim3(d->grid), dim3(256), 0, c->stream, d->P, d->xcd_map);
+        hipLaunchKernelGGL(k_duct_proj_final, dim3(S), dim3(1024), 0, c->stream, d->P);
+        DLAUNCH_CHECK();
     }
     hipLaunchKernelGGL((k_duct_bc<real>), dim3(d->nzb, d->ny_max, S), dim3(256), 0, c->stream, d->P);
     DLAUNCH_CHECK();
@@ -1783,7 +1946,8 @@ static int slot_monitor_t(les_gpu* c, int s, les_step_report* rep)
     rep->step = y.step - 1;
     rep->dt = y.dt; rep->max_velocity = y.maxvel;
     rep->total_mass = tm; rep->min_T = mnT; rep->max_T = mxT; rep->min_G = mnG; rep->max_G = mxG;
-    rep->min_Y_fuel = 0.0; rep->nonfinite = bad > 0.0; rep->proj_residual = 0.0;
+    rep->min_Y_fuel = 0.0; rep->nonfinite = bad > 0.0;
+    rep->proj_residual = y.proj_res; rep->proj_residual0 = y.proj_res0;
     return LES_OK;
 }
 
@@ -1794,7 +1958,8 @@ static int slot_monitor(les_gpu* c, int s, les_step_report* rep)
 
 // ---------------------------------------------------------------------------
 // autotuner: times each (tile width, XCD mapping) candidate on a scratch pool
-// holding one run of the probe grid and keeps the fastest
+// filled with replicas of the probe run (as many as the pool's slots and cells
+// hold, so the timed launches have the pool's occupancy) and keeps the fastest
 // ---------------------------------------------------------------------------
 static int pool_autotune(les_gpu* c, const les_params& probe)
 {
@@ -1804,10 +1969,14 @@ static int pool_autotune(les_gpu* c, const les_params& probe)
     int btk = d->tk, bx = d->xcd_map;
     for (int a = 0; a < 4; ++a)
         for (int x = 0; x < 2; ++x) {
+            const long long pc = (long long)probe.nx * probe.ny * probe.nz;
+            long long reps = d->pool_cells / pc;
+            if (reps > d->max_slots) reps = d->max_slots;
+            if (reps < 1) reps = 1;
             les_batch_config cfg;
             memset(&cfg, 0, sizeof(cfg));
-            cfg.max_slots = 1;
-            cfg.pool_cells = (long long)probe.nx * probe.ny * probe.nz;
+            cfg.max_slots = (int)reps;
+            cfg.pool_cells = reps * pc;
             cfg.max_nx = probe.nx; cfg.max_ny = probe.ny; cfg.max_nz = probe.nz;
             cfg.max_vortices = probe.num_vortices > 0 ? probe.num_vortices : 1;
             cfg.precision = d->precision;
@@ -1820,9 +1989,11 @@ static int pool_autotune(les_gpu* c, const les_params& probe)
             t->p = probe;
             t->stream = c->stream;
             int e = pool_create(t, cfg);
-            int slot = -1;
-            if (e == LES_OK) e = pool_admit(t, probe, 0, &slot);
-            if (e == LES_OK) e = slot_init(t, slot, probe);
+            for (long long r = 0; r < reps && e == LES_OK; ++r) {
+                int slot = -1;
+                e = pool_admit(t, probe, 0, &slot);
+                if (e == LES_OK) e = slot_init(t, slot, probe);
+            }
             if (e == LES_OK) e = pool_steps(t, 2);
             float ms = 0.f;
             if (e == LES_OK && hipEventRecord(c->ev[2], c->stream) == hipSuccess) {
@@ -2072,7 +2243,7 @@ extern "C" int les_batch_pool_bytes(const les_batch_config* cfg, size_t* bytes)
     size_t b = 17 * n * es;                                   // A, B, F (5 each), nu_t, S_T
     if (cfg->stats_per_cell) b += 80 * n;
     if (cfg->precision == LES_PREC_FP32) b += 8 * n;           // FP64 staging of one run
-    b += tiles * (8 + 8 + 48 + 4);                            // rho, |u|, nu/alpha partials, tile list
+    b += tiles * (8 + 8 + 48 + 16 + 4);                       // rho, |u|, nu/alpha, residual partials, tile list
     const size_t nv = cfg->max_vortices > 0 ? cfg->max_vortices : 1;
     b += S * 8 * (7 * nv + 3 * (size_t)cfg->max_ny * cfg->max_nz + 10 * (size_t)cfg->max_nx +
                   4 * (size_t)DUCT_NPTS * cfg->point_cap + (size_t)cfg->max_ny * ((cfg->max_nz + 255) / 256) +

@@ -125,6 +125,7 @@ int les_params_default(les_params* p, int preset)
         p->precision = LES_PREC_FP32;
         p->turbulence_intensity = 0.15;
         p->num_vortices = 2000;
+        p->proj_iters = 8;           /* projection relaxation with FP64 residuals */
         p->num_probe_x = 16;
         for (int n = 0; n < 16; ++n) p->probe_x[n] = 0.010 + 0.015 * n;
         return LES_OK;
@@ -148,6 +149,8 @@ int les_params_validate(const les_params* p)
     if (p->mode == LES_MODE_DUCT) {
         if (p->table_n < 2 || !(p->table_dt > 0)) return LES_ERR_PARAM;
         if (!(p->rho_unburnt > 0) || !(p->prandtl > 0)) return LES_ERR_PARAM;
+        if (p->proj_iters < 0 || p->proj_iters > 256) return LES_ERR_PARAM;
+        if (p->reinit_iters < 0 || p->reinit_iters > 64 || p->reinit_interval < 0) return LES_ERR_PARAM;
     }
     if (p->mode == LES_MODE_REFERENCE && p->precision != LES_PREC_FP64) return LES_ERR_PARAM;
     return LES_OK;

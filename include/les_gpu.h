@@ -161,7 +161,9 @@ typedef struct les_step_report {
     double total_mass;           /* monitor reductions (:3885-3897) */
     double min_T, max_T, min_G, max_G, min_Y_fuel;
     int    nonfinite;            /* 1 if a non-finite value was found */
-    double proj_residual;        /* duct mode: FP64 projection residual (L2) */
+    double proj_residual;        /* duct mode: FP64 RMS residual of the projection's
+                                    Poisson relaxation after its sweeps */
+    double proj_residual0;       /* the same before the sweeps (RMS of div(rho u)) */
 } les_step_report;
 
 /* One flame-region probe row (the CSV line of :3453). */

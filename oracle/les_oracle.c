@@ -1330,6 +1330,7 @@ static int step_ref(ora_ctx* c, les_step_report* rep)
         rep->step = t; rep->dt = DT; rep->max_velocity = maxvel;
         monitor(c, rep);
         rep->proj_residual = 0.0;
+        rep->proj_residual0 = 0.0;
     }
     c->step++;
     return LES_OK;

@@ -57,6 +57,8 @@ double ora_duct_progress(double phi, double dx);
 double ora_duct_uniform(uint64_t seed, uint64_t a, uint64_t b, uint64_t slot);
 int  ora_duct_sem(ora_ctx* c, double dt);
 int  ora_duct_reinit(ora_ctx* c, int iters);
+/* projection relaxation of rho*u with `iters` sweeps; FP64 RMS residuals */
+int  ora_duct_project(ora_ctx* c, int iters, double* res0, double* res1);
 int  ora_duct_set_eddies(ora_ctx* c, const double* e, int n);
 int  ora_duct_set_dt_state(ora_ctx* c, double max_nu, double max_alpha);
 int  ora_duct_profile_count(const ora_ctx* c);

@@ -544,6 +544,84 @@ static void duct_boundary(ora_ctx* c)
         }
 }
 
+/* ---- pressure/projection relaxation of the mass flux m = rho*u (this
+ * engine's own specification: the reference has no pressure solve).
+ * b = div m (central differences, periodic y/z) on interior planes;
+ * `iters` Richardson sweeps phi <- phi + (L phi - b) / d from phi = 0 on the
+ * compact 7-point Laplacian L, d = 2/dx^2 + 2/dy^2 + 2/dz^2, with the ghost
+ * rules phi(plane 0) = phi(plane 1) (Neumann, inflow) and phi(plane nx-1) = 0
+ * (Dirichlet, outflow); then m <- m - grad phi (central, same ghosts) on
+ * interior planes.  FP64 RMS of the residual b - L phi over the interior
+ * before the sweeps (res0 = RMS of b) and after them (res1). ---- */
+static inline double proj_phi(const ora_ctx* c, const double* phi, int i, int j, int k)
+{
+    if (i <= 0) i = 1;
+    if (i >= c->nx - 1) return 0.0;
+    return phi[QX(c, i, j, k)];
+}
+
+static inline double proj_lap(const ora_ctx* c, const double* phi, int i, int j, int k)
+{
+    const double ix2 = 1.0 / (c->dx * c->dx), iy2 = 1.0 / (c->dy * c->dy), iz2 = 1.0 / (c->dz * c->dz);
+    const double p0 = phi[IX(c, i, j, k)];
+    double lx = (proj_phi(c, phi, i + 1, j, k) - 2.0 * p0 + proj_phi(c, phi, i - 1, j, k)) * ix2;
+    double ly = (proj_phi(c, phi, i, j + 1, k) - 2.0 * p0 + proj_phi(c, phi, i, j - 1, k)) * iy2;
+    double lz = (proj_phi(c, phi, i, j, k + 1) - 2.0 * p0 + proj_phi(c, phi, i, j, k - 1)) * iz2;
+    return lx + ly + lz;
+}
+
+int ora_duct_project(ora_ctx* c, int iters, double* res0, double* res1)
+{
+    ora_duct* d = c->duct;
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    double* b = d->L[0];
+    double* ph[2] = {d->L[1], d->L[2]};
+    const double hx = 0.5 / c->dx, hy = 0.5 / c->dy, hz = 0.5 / c->dz;
+    const double inv_d = 1.0 / (2.0 / (c->dx * c->dx) + 2.0 / (c->dy * c->dy) + 2.0 / (c->dz * c->dz));
+    const double n_int = (double)(nx - 2) * ny * nz;
+    double s0 = 0.0;
+    memset(ph[0], 0, c->n * 8);
+    memset(ph[1], 0, c->n * 8);
+    memset(b, 0, c->n * 8);
+    #pragma omp parallel for schedule(static) reduction(+ : s0)
+    for (int i = 1; i < nx - 1; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                double v = (d->q[0][QX(c, i + 1, j, k)] - d->q[0][QX(c, i - 1, j, k)]) * hx +
+                           (d->q[1][QX(c, i, j + 1, k)] - d->q[1][QX(c, i, j - 1, k)]) * hy +
+                           (d->q[2][QX(c, i, j, k + 1)] - d->q[2][QX(c, i, j, k - 1)]) * hz;
+                b[IX(c, i, j, k)] = v;
+                s0 += v * v;
+            }
+    for (int it = 0; it < iters; ++it) {
+        const double* po = ph[it & 1];
+        double* pn = ph[(it + 1) & 1];
+        #pragma omp parallel for schedule(static)
+        for (int i = 1; i < nx - 1; ++i)
+            for (int j = 0; j < ny; ++j)
+                for (int k = 0; k < nz; ++k) {
+                    size_t q = IX(c, i, j, k);
+                    pn[q] = po[q] + (proj_lap(c, po, i, j, k) - b[q]) * inv_d;
+                }
+    }
+    const double* phi = ph[iters & 1];
+    double s1 = 0.0;
+    #pragma omp parallel for schedule(static) reduction(+ : s1)
+    for (int i = 1; i < nx - 1; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                size_t q = IX(c, i, j, k);
+                double r = b[q] - proj_lap(c, phi, i, j, k);
+                s1 += r * r;
+                d->q[0][q] -= (proj_phi(c, phi, i + 1, j, k) - proj_phi(c, phi, i - 1, j, k)) * hx;
+                d->q[1][q] -= (proj_phi(c, phi, i, j + 1, k) - proj_phi(c, phi, i, j - 1, k)) * hy;
+                d->q[2][q] -= (proj_phi(c, phi, i, j, k + 1) - proj_phi(c, phi, i, j, k - 1)) * hz;
+            }
+    if (res0) *res0 = sqrt(s0 / n_int);
+    if (res1) *res1 = sqrt(s1 / n_int);
+    return LES_OK;
+}
+
 /* ---- level-set reinitialisation: Sussman-Smereka-Osher with first-order
  * Godunov upwinding, pseudo time step 0.3*dx, interior planes only ---- */
 int ora_duct_reinit(ora_ctx* c, int iters)
@@ -651,6 +729,8 @@ int ora_duct_step(ora_ctx* c, les_step_report* rep)
     duct_stage_update(c, 1, DT, d->qs, d->qs);
     duct_rhs_eval(c, d->qs, d->L);
     duct_stage_update(c, 2, DT, d->qs, d->q);
+    double pr0 = 0.0, pr1 = 0.0;
+    if (c->p.proj_iters > 0) ora_duct_project(c, c->p.proj_iters, &pr0, &pr1);
     duct_boundary(c);
     if (c->p.reinit_interval > 0 && (c->step + 1) % c->p.reinit_interval == 0)
         ora_duct_reinit(c, c->p.reinit_iters);
@@ -669,7 +749,7 @@ int ora_duct_step(ora_ctx* c, les_step_report* rep)
             for (int f = 0; f < DNF; ++f) if (!isfinite(d->q[f][q])) bad = 1;
         }
         rep->total_mass = tm; rep->min_T = mnT; rep->max_T = mxT; rep->min_G = mnG; rep->max_G = mxG;
-        rep->min_Y_fuel = 0.0; rep->nonfinite = bad; rep->proj_residual = 0.0;
+        rep->min_Y_fuel = 0.0; rep->nonfinite = bad; rep->proj_residual = pr1; rep->proj_residual0 = pr0;
     }
     c->step++;
     return LES_OK;

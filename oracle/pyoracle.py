@@ -73,6 +73,7 @@ def lib():
         L.ora_duct_progress.argtypes = [C.c_double, C.c_double]
         L.ora_duct_progress.restype = C.c_double
         L.ora_duct_reinit.argtypes = [C.c_void_p, C.c_int]
+        L.ora_duct_project.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ora_duct_sem.argtypes = [C.c_void_p, C.c_double]
         L.ora_duct_set_eddies.argtypes = [C.c_void_p, P(C.c_double), C.c_int]
         L.ora_duct_set_dt_state.argtypes = [C.c_void_p, C.c_double, C.c_double]

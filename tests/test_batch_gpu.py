@@ -107,6 +107,21 @@ def test_launch_shapes_vs_oracle(lesgpu, oracle, tile_k, xcd):
     o.close()
 
 
+def test_projection_runs_in_mix_equal_alone(lesgpu):
+    print("case: runs with different projection sweep counts in one pool vs alone", flush=True)
+    pa = run_params(lesgpu, 0, proj_iters=3, seed=2)
+    pb = run_params(lesgpu, 0, nx=48, ny=40, nz=24, proj_iters=6, seed=4)
+    pc = run_params(lesgpu, 0, nx=40, ny=32, nz=32, seed=6)           # no projection
+    alone = [single_run(lesgpu, p, 5)[0] for p in (pa, pb, pc)]
+    with pool(lesgpu, 0, slots=3) as b:
+        slots = [b.admit(p, n_steps=5) for p in (pa, pb, pc)]
+        b.step(5)
+        for s, want in zip(slots, alone):
+            for x, y in zip(b.download(s), want):
+                assert np.array_equal(x, y)
+        assert b.report(slots[0]).proj_residual > 0 and b.report(slots[2]).proj_residual == 0
+
+
 def test_admission_limits(lesgpu):
     print("case: admission refusal and reuse", flush=True)
     with pool(lesgpu, 1, slots=3, cells=2 * 64 * 32 * 32) as b:

@@ -152,3 +152,23 @@ def test_point_series(lesgpu):
         assert np.allclose(ser[-1, :, 0], u[idx], rtol=0, atol=0)
     finally:
         g.close()
+
+
+def test_duct_projection_fp64(lesgpu, oracle):
+    print("case: duct64 FP64 with projection relaxation (5 sweeps), 6 steps", flush=True)
+    g, o = make_pair(lesgpu, oracle, 1, proj_iters=5, reinit_interval=3, reinit_iters=2)
+    try:
+        g.init_state()
+        o.init_state()
+        for s in range(6):
+            rg = g.step(1)
+            ro = o.step(1)
+            assert abs(rg.dt - ro.dt) <= 1e-12 * ro.dt, s
+            assert ro.proj_residual < ro.proj_residual0
+            assert abs(rg.proj_residual0 - ro.proj_residual0) <= 1e-10 * ro.proj_residual0, s
+            assert abs(rg.proj_residual - ro.proj_residual) <= 1e-9 * ro.proj_residual, s
+        for nm, a, b in zip(DUCT, g.download_state(5), o.get_state(5)):
+            assert rel_err(a, b) <= TOL, (nm, rel_err(a, b))
+    finally:
+        g.close()
+        o.close()
