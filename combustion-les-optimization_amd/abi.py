@@ -141,6 +141,7 @@ class BatchConfig(C.Structure):
         ("xcd_map", C.c_int),
         ("autotune", C.c_int),
         ("point_cap", C.c_int),
+        ("probe_cap", C.c_int),
         ("stats_per_cell", C.c_int),
         ("table_t0", C.c_double), ("table_dt", C.c_double),
         ("table_n", C.c_int),

@@ -45,7 +45,7 @@ struct RefConsts {
     int    transition_step;
 };
 
-__device__ __forceinline__ long long lidx(const GridDims& g, int i, int j, int k)
+__host__ __device__ __forceinline__ long long lidx(const GridDims& g, int i, int j, int k)
 {
     return ((long long)i * g.ny + j) * g.nz + k;
 }

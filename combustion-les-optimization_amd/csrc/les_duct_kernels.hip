@@ -73,6 +73,10 @@ struct SlotDesc {
     int active;             // 1 while admitted
     int n_steps;            // steps requested
     int point_idx[DUCT_NPTS];
+    int npts;               // points in use (<= DUCT_NPTS)
+    int nprobe;             // flame-region probe stations
+    int probe_i[LES_NPROBE_MAX];
+    long long proff;        // first value of the run's probe ring
     DuctConsts k;
 };
 
@@ -112,7 +116,11 @@ struct Pool {
     int tile_cap;
     int ny_max, nz_max, nx_max, nzb;
     int point_cap;          // steps per point-series ring
+    double* probe;          // probe ring: [slot][probe_cap][station][2 + 6*ny_max]
+    int probe_cap;          // steps per probe ring (0: no probes)
 };
+
+#define PROBE_STRIDE(P) (2 + 6 * (P).ny_max)
 // ---------------------------------------------------------------------------
 // helpers shared by host-side checks and kernels
 // ---------------------------------------------------------------------------
@@ -1264,7 +1272,7 @@ __global__ void k_duct_points(Pool P)
     const SlotDesc& D = P.desc[s];
     if (!D.active || P.dyn[s].done) return;
     const int t = threadIdx.x;
-    if (t >= DUCT_NPTS) return;
+    if (t >= D.npts) return;
     const int q = D.point_idx[t];
     real rho = (real)D.k.rho_u_Tu / ((const real*)P.A[4] + D.off)[q];
     const int slot = P.dyn[s].step % P.point_cap;
@@ -1275,7 +1283,79 @@ __global__ void k_duct_points(Pool P)
     o[3] = (double)progress<real>(((const real*)P.A[3] + D.off)[q], (real)D.g.dx);
 }
 
+// flame-region probe lines (record_flame_region_data :3403-3471 structure):
+// one block per (station, run).  The first G = 0.5 crossing along y on the
+// line (i_probe, ., nz/2) is the smallest j whose segment [j, j+1] crosses
+// (block minimum in LDS), located by linear interpolation; then the line's u,
+// v, w, G, nu_t and |S| (|S| recomputed from the last stage's filtered
+// velocities, nu_t of the last stage) go into the run's ring, one thread per j.
+template <typename real>
+__global__ void __launch_bounds__(256) k_duct_probe(Pool P)
+{
+    __shared__ int jmin;
+    const int s = blockIdx.y, n = blockIdx.x;
+    const SlotDesc& D = P.desc[s];
+    if (P.probe_cap <= 0 || !D.active || P.dyn[s].done || n >= D.nprobe) return;
+    const GridDims g = D.g;
+    double* o = P.probe + D.proff + ((long long)(P.dyn[s].step % P.probe_cap) * LES_NPROBE_MAX + n) * PROBE_STRIDE(P);
+    const int ip = D.probe_i[n];
+    if (ip <= 0 || ip >= g.nx - 1) {
+        if (threadIdx.x == 0) o[0] = 0.0;
+        return;
+    }
+    const int kp = g.nz / 2;
+    const real* A3 = (const real*)P.A[3] + D.off;
+    const real dxr = (real)g.dx;
+    if (threadIdx.x == 0) jmin = 0x7fffffff;
+    __syncthreads();
+    for (int j = threadIdx.x; j < g.ny - 1; j += blockDim.x) {
+        double g1 = (double)progress<real>(A3[lidx(g, ip, j, kp)], dxr);
+        double g2 = (double)progress<real>(A3[lidx(g, ip, j + 1, kp)], dxr);
+        if ((g1 - 0.5) * (g2 - 0.5) <= 0.0) atomicMin(&jmin, j);
+    }
+    __syncthreads();
+    const int jc = jmin;
+    double yf = -1.0;
+    if (jc != 0x7fffffff) {
+        double g1 = (double)progress<real>(A3[lidx(g, ip, jc, kp)], dxr);
+        double g2 = (double)progress<real>(A3[lidx(g, ip, jc + 1, kp)], dxr);
+        double y1 = jc * g.dy, y2 = (jc + 1) * g.dy;
+        if (fabs(g2 - g1) < 1e-9) yf = y1;
+        else yf = y1 + (0.5 - g1) / (g2 - g1) * (y2 - y1);
+    }
+    if (!(yf > 0.0)) {
+        if (threadIdx.x == 0) o[0] = 0.0;
+        return;
+    }
+    if (threadIdx.x == 0) { o[0] = 1.0; o[1] = yf; }
+    const real* uf = (const real*)P.F[0] + D.off;
+    const real* vf = (const real*)P.F[1] + D.off;
+    const real* wf = (const real*)P.F[2] + D.off;
+    const real hx = (real)(0.5 / g.dx), hy = (real)(0.5 / g.dy), hz = (real)(0.5 / g.dz);
+    for (int j = threadIdx.x; j < g.ny; j += blockDim.x) {
+        const int q = (int)lidx(g, ip, j, kp);
+        const real rho = (real)D.k.rho_u_Tu / ((const real*)P.A[4] + D.off)[q];
+        double* r = o + 2 + 6 * j;
+        r[0] = (double)(((const real*)P.A[0] + D.off)[q] / rho);
+        r[1] = (double)(((const real*)P.A[1] + D.off)[q] / rho);
+        r[2] = (double)(((const real*)P.A[2] + D.off)[q] / rho);
+        r[3] = (double)progress<real>(A3[q], dxr);
+        r[4] = (double)((const real*)P.nut + D.off)[q];
+        const int xp = q + (int)g.sx, xm = q - (int)g.sx;
+        const int yp = (int)lidx(g, ip, wrapi(j + 1, g.ny), kp), ym = (int)lidx(g, ip, wrapi(j - 1, g.ny), kp);
+        const int zp = (int)lidx(g, ip, j, wrapi(kp + 1, g.nz)), zm = (int)lidx(g, ip, j, wrapi(kp - 1, g.nz));
+        real du_dx = (uf[xp] - uf[xm]) * hx, dv_dy = (vf[yp] - vf[ym]) * hy, dw_dz = (wf[zp] - wf[zm]) * hz;
+        real du_dy = (uf[yp] - uf[ym]) * hy, du_dz = (uf[zp] - uf[zm]) * hz;
+        real dv_dx = (vf[xp] - vf[xm]) * hx, dv_dz = (vf[zp] - vf[zm]) * hz;
+        real dw_dx = (wf[xp] - wf[xm]) * hx, dw_dy = (wf[yp] - wf[ym]) * hy;
+        real S12 = (real)0.5 * (du_dy + dv_dx), S13 = (real)0.5 * (du_dz + dw_dx), S23 = (real)0.5 * (dv_dz + dw_dy);
+        real ss = (real)2.0 * (S12 * S12 + S13 * S13 + S23 * S23) + du_dx * du_dx + dv_dy * dv_dy + dw_dz * dw_dz;
+        r[5] = (double)sqrt((real)2.0 * ss);
+    }
+}
+
 __global__ void k_duct_step_end(Pool P, int nslots)
+
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
@@ -1431,6 +1511,7 @@ struct DuctState {
     int tile_cap = 0;
     int nx_max = 0, ny_max = 0, nz_max = 0, nvort_max = 0, nzb = 0;
     int point_cap = 1024;
+    int probe_cap = 0;
     int stats_cells = 0;
     int cus = 256;
     // launch configuration (autotuned)
@@ -1448,6 +1529,9 @@ struct DuctState {
     std::vector<int> taken;        // slot admitted (its state may not be set yet)
     std::vector<int> host_step;
     std::vector<int> point_base;   // host step index of the oldest unfetched point sample
+    std::vector<int> probe_base;   // same for the probe ring
+    std::vector<std::vector<les_probe_row>> probe_rows;   // drained, not yet fetched (batch)
+    std::vector<std::vector<double>> probe_x;             // station x per run
     RangeAlloc cells, tiles;
     int ntile_hwm = 0;
     double* stage64 = nullptr;
@@ -1521,6 +1605,7 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
     d->nx_max = cfg.max_nx; d->ny_max = cfg.max_ny; d->nz_max = cfg.max_nz;
     d->nvort_max = cfg.max_vortices > 0 ? cfg.max_vortices : 1;
     d->point_cap = cfg.point_cap > 0 ? cfg.point_cap : 1024;
+    d->probe_cap = cfg.probe_cap > 0 ? cfg.probe_cap : 0;
     d->stats_cells = cfg.stats_per_cell;
     d->nzb = (d->nz_max + 255) / 256;
     d->tk = cfg.tile_k;
@@ -1584,12 +1669,19 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
     P.tile_cap = d->tile_cap;
     P.ny_max = d->ny_max; P.nz_max = d->nz_max; P.nx_max = d->nx_max; P.nzb = d->nzb;
     P.point_cap = d->point_cap;
+    P.probe_cap = d->probe_cap;
+    if (d->probe_cap > 0 &&
+        !(P.probe = (double*)dalloc_bytes(c, 8 * S * d->probe_cap * LES_NPROBE_MAX * (size_t)PROBE_STRIDE(P))))
+        return LES_ERR_HIP;
     d->desc.assign(S, SlotDesc());
     for (auto& x : d->desc) memset(&x, 0, sizeof(x));
     d->tile_slot.assign(d->tile_cap, -1);
     d->taken.assign(S, 0);
     d->host_step.assign(S, 0);
     d->point_base.assign(S, 0);
+    d->probe_base.assign(S, 0);
+    d->probe_rows.assign(S, std::vector<les_probe_row>());
+    d->probe_x.assign(S, std::vector<double>(LES_NPROBE_MAX, 0.0));
     d->cells.reset(d->pool_cells);
     d->tiles.reset(d->tile_cap);
     std::vector<int> neg(d->tile_cap, -1);
@@ -1663,10 +1755,19 @@ static int pool_admit(les_gpu* c, const les_params& p, int n_steps, int* slot_ou
                 int ii = (a + 1) * p.nx / 5, jj = (2 * b + 1) * p.ny / 8, kk = (2 * e + 1) * p.nz / 8;
                 D.point_idx[t++] = (int)(((long long)ii * p.ny + jj) * p.nz + kk);
             }
+    D.npts = DUCT_NPTS;
     D.k = make_duct_consts(p);
+    D.nprobe = p.num_probe_x;
+    for (int n = 0; n < p.num_probe_x; ++n) {
+        D.probe_i[n] = (int)(p.probe_x[n] / g.dx);
+        d->probe_x[s][n] = p.probe_x[n];
+    }
+    d->probe_rows[s].clear();
+    D.proff = (long long)s * d->probe_cap * LES_NPROBE_MAX * (2 + 6LL * d->ny_max);
     for (long long a = 0; a < ntiles; ++a) d->tile_slot[t0 + a] = s;
     d->host_step[s] = 0;
     d->point_base[s] = 0;
+    d->probe_base[s] = 0;
     d->taken[s] = 1;
     SlotDyn y;
     memset(&y, 0, sizeof(y));
@@ -1762,6 +1863,8 @@ static int step_launches(les_gpu* c, bool reinit)
     hipLaunchKernelGGL((k_duct_stats<real>), dim3(STAT_CHUNKS, d->nx_max, S), dim3(256), 0, c->stream, d->P);
     hipLaunchKernelGGL(k_duct_stats_final, dim3((d->nx_max * 10 + 255) / 256, S), dim3(256), 0, c->stream, d->P);
     hipLaunchKernelGGL((k_duct_points<real>), dim3(S), dim3(64), 0, c->stream, d->P);
+    if (d->probe_cap > 0)
+        hipLaunchKernelGGL((k_duct_probe<real>), dim3(LES_NPROBE_MAX, S), dim3(256), 0, c->stream, d->P);
     hipLaunchKernelGGL(k_duct_step_end, dim3((S + 255) / 256), dim3(256), 0, c->stream, d->P, S);
     DLAUNCH_CHECK();
     return LES_OK;
@@ -1908,6 +2011,7 @@ static int slot_init(les_gpu* c, int s, const les_params& p)
     DHIPC(hipMemcpyAsync(d->d_dyn + s, &y, sizeof(y), hipMemcpyHostToDevice, c->stream));
     d->host_step[s] = 0;
     d->point_base[s] = 0;
+    d->probe_base[s] = 0;
     const double* f[5] = {&h[0], &h[n], &h[2 * n], &h[3 * n], &h[4 * n]};
     int e = slot_upload(c, s, f, 5);
     if (e != LES_OK) return e;
@@ -2090,12 +2194,49 @@ static int slot_point_fetch(les_gpu* c, int s, double* out, int max_steps, int* 
     const double* ring = d->P.points + d->desc[s].ptoff;
     for (int a = 0; a < n; ++a) {
         const int r = (first + a) % d->point_cap;
-        DHIPC(hipMemcpyAsync(out + (size_t)a * DUCT_NPTS * 4, ring + (size_t)r * DUCT_NPTS * 4,
-                             (size_t)DUCT_NPTS * 4 * 8, hipMemcpyDeviceToHost, c->stream));
+        const int np = d->desc[s].npts;
+        DHIPC(hipMemcpyAsync(out + (size_t)a * np * 4, ring + (size_t)r * DUCT_NPTS * 4, (size_t)np * 4 * 8,
+                             hipMemcpyDeviceToHost, c->stream));
     }
     DHIPC(hipStreamSynchronize(c->stream));
     d->point_base[s] = d->host_step[s];
     *n_steps = n;
+    return LES_OK;
+}
+
+// probe rows recorded since the last drain (the newest probe_cap steps), in
+// the row format of the reference's probe CSV (SAMPLING_RANGE 1.0 m, :280)
+static int slot_probe_drain(les_gpu* c, int s, const double* probe_x, std::vector<les_probe_row>& out)
+{
+    DuctState* d = c->d;
+    const SlotDesc& D = d->desc[s];
+    if (d->probe_cap <= 0 || D.nprobe <= 0) { d->probe_base[s] = d->host_step[s]; return LES_OK; }
+    const int total = d->host_step[s] - d->probe_base[s];
+    const int avail = total < d->probe_cap ? total : d->probe_cap;
+    const int first = d->host_step[s] - avail;
+    const size_t stride = 2 + 6 * (size_t)d->ny_max;
+    std::vector<double> h((size_t)LES_NPROBE_MAX * stride);
+    for (int a = 0; a < avail; ++a) {
+        const int step = first + a;
+        const double* src = d->P.probe + D.proff + (size_t)(step % d->probe_cap) * LES_NPROBE_MAX * stride;
+        DHIPC(hipMemcpyAsync(h.data(), src, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+        DHIPC(hipStreamSynchronize(c->stream));
+        for (int n = 0; n < D.nprobe; ++n) {
+            const double* e = &h[(size_t)n * stride];
+            if (e[0] != 1.0) continue;
+            for (int j = 0; j < D.g.ny; ++j) {
+                const double* r = e + 2 + 6 * j;
+                double yp = j * D.g.dy, yr = yp - e[1];
+                if (fabs(yr) > 1.0) continue;
+                les_probe_row pr;
+                pr.timestep = step; pr.station = n; pr.x_location = probe_x[n];
+                pr.y_flame = e[1]; pr.y_physical = yp; pr.y_relative = yr;
+                pr.u = r[0]; pr.v = r[1]; pr.w = r[2]; pr.G = r[3]; pr.nu_t = r[4]; pr.S_mag = r[5];
+                out.push_back(pr);
+            }
+        }
+    }
+    d->probe_base[s] = d->host_step[s];
     return LES_OK;
 }
 
@@ -2117,6 +2258,7 @@ int duct_create(les_gpu* c)
     cfg.tile_k = 64;
     cfg.xcd_map = 1;
     cfg.point_cap = 4096;
+    cfg.probe_cap = p.num_probe_x > 0 ? 64 : 0;
     cfg.stats_per_cell = (p.stats_start_step >= 0 && p.stats_per_cell) ? 1 : 0;
     cfg.table_t0 = p.table_t0; cfg.table_dt = p.table_dt; cfg.table_n = p.table_n;
     int e = pool_create(c, cfg);
@@ -2165,21 +2307,47 @@ int duct_point_fetch(les_gpu* c, double* out, int max_steps, int* n_steps)
     return slot_point_fetch(c, 0, out, max_steps, n_steps);
 }
 
+int duct_npoints(const les_gpu* c) { return c->d->desc[0].npts; }
+
+int duct_set_points(les_gpu* c, const int* cells, int n)
+{
+    SlotDesc& D = c->d->desc[0];
+    for (int a = 0; a < n; ++a) D.point_idx[a] = cells[a];
+    D.npts = n;
+    c->d->point_base[0] = c->d->host_step[0];
+    return pool_sync_tables(c);
+}
+
 int duct_step(les_gpu* c, int n_steps, les_step_report* rep)
 {
     DuctState* d = c->d;
     const int st0 = c->p.stats_start_step;
     for (int a = 0; a < n_steps; ++a)
         if (st0 >= 0 && d->host_step[0] + a >= st0) c->stat_count++;
-    DHIPC(hipEventRecord(c->ev[2], c->stream));
-    int e = pool_steps(c, n_steps);
-    if (e != LES_OK) return e;
-    DHIPC(hipEventRecord(c->ev[3], c->stream));
-    DHIPC(hipEventSynchronize(c->ev[3]));
-    float ms = 0.f;
-    DHIPC(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-    c->step_ms = n_steps > 0 ? ms / n_steps : 0.0;
+    // probe rows are drained into the context's row list at least every
+    // probe_cap steps (the ring holds that many); the step time excludes the
+    // drains
+    const int chunk = d->probe_cap > 0 ? d->probe_cap : (n_steps > 0 ? n_steps : 1);
+    double ms_sum = 0.0;
+    for (int done = 0; done < n_steps;) {
+        const int n = n_steps - done < chunk ? n_steps - done : chunk;
+        DHIPC(hipEventRecord(c->ev[2], c->stream));
+        int e = pool_steps(c, n);
+        if (e != LES_OK) return e;
+        DHIPC(hipEventRecord(c->ev[3], c->stream));
+        DHIPC(hipEventSynchronize(c->ev[3]));
+        float ms = 0.f;
+        DHIPC(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+        ms_sum += ms;
+        done += n;
+        if (d->probe_cap > 0) {
+            e = slot_probe_drain(c, 0, c->p.probe_x, c->probe);
+            if (e != LES_OK) return e;
+        }
+    }
+    c->step_ms = n_steps > 0 ? ms_sum / n_steps : 0.0;
     c->step += n_steps;
+    int e = LES_OK;
     if (rep) {
         e = slot_monitor(c, 0, rep);
         if (e != LES_OK) return e;
@@ -2427,8 +2595,149 @@ extern "C" int les_batch_report(les_batch* b, int slot, les_step_report* rep)
     return slot_monitor(b->c, slot, rep);
 }
 
+extern "C" int les_batch_probe_fetch(les_batch* b, int slot, les_probe_row* rows, int max_rows, int* n_rows)
+{
+    if (!batch_slot_ok(b, slot) || !rows || !n_rows || max_rows < 0) return LES_ERR_PARAM;
+    les_gpu* c = b->c;
+    DuctState* d = c->d;
+    std::vector<les_probe_row>& v = d->probe_rows[slot];
+    int e = slot_probe_drain(c, slot, d->probe_x[slot].data(), v);
+    if (e != LES_OK) return e;
+    const int n = (int)v.size() < max_rows ? (int)v.size() : max_rows;
+    if (n > 0) memcpy(rows, v.data(), (size_t)n * sizeof(les_probe_row));
+    v.erase(v.begin(), v.begin() + n);
+    *n_rows = n;
+    return LES_OK;
+}
+
 extern "C" int les_batch_point_fetch(les_batch* b, int slot, double* out, int max_steps, int* n_steps)
 {
     if (!batch_slot_ok(b, slot) || !out || !n_steps || max_steps < 0) return LES_ERR_PARAM;
     return slot_point_fetch(b->c, slot, out, max_steps, n_steps);
+}
+
+// ---------------------------------------------------------------------------
+// resume support of a single-run duct context (slot 0)
+// ---------------------------------------------------------------------------
+int duct_upload_field(les_gpu* c, const char* name, const double* data)
+{
+    DuctState* d = c->d;
+    const SlotDesc& D = d->desc[0];
+    if (strcmp(name, "profiles") == 0) {
+        DHIPC(hipMemcpyAsync(d->P.prof + D.xoff, data, 8 * 10 * (size_t)D.g.nx, hipMemcpyHostToDevice, c->stream));
+    } else if (strcmp(name, "eddies") == 0) {
+        DHIPC(hipMemcpyAsync(d->P.eddy + 7 * D.eoff, data, 8 * 7 * (size_t)D.k.nvort, hipMemcpyHostToDevice, c->stream));
+    } else if (strncmp(name, "stat", 4) == 0 && d->P.stat[0]) {
+        int a = atoi(name + 4);
+        if (a < 0 || a > 9) return LES_ERR_PARAM;
+        DHIPC(hipMemcpyAsync(d->P.stat[a] + D.off, data, 8 * (size_t)D.g.n, hipMemcpyHostToDevice, c->stream));
+    } else {
+        snprintf(c->err, sizeof(c->err), "duct field %s cannot be uploaded", name);
+        return LES_ERR_PARAM;
+    }
+    DHIPC(hipStreamSynchronize(c->stream));
+    return LES_OK;
+}
+
+struct AuxDuct {
+    char magic[8];
+    int mode, step, stat_count, nvort, nx, pad;
+    double time, max_nu, max_alpha;
+};
+
+static const char DUCT_AUX_MAGIC[8] = {'L', 'E', 'S', 'A', 'U', 'X', '1', 0};
+
+int duct_export_aux(les_gpu* c, void* buf, size_t* bytes)
+{
+    DuctState* d = c->d;
+    const SlotDesc& D = d->desc[0];
+    const size_t ne = 7 * (size_t)D.k.nvort, np = 10 * (size_t)D.g.nx;
+    const size_t need = sizeof(AuxDuct) + (ne + np) * 8;
+    if (!buf || *bytes < need) { *bytes = need; return buf ? LES_ERR_PARAM : LES_OK; }
+    AuxDuct a;
+    memset(&a, 0, sizeof(a));
+    memcpy(a.magic, DUCT_AUX_MAGIC, 8);
+    a.mode = LES_MODE_DUCT; a.step = c->step; a.stat_count = c->stat_count; a.nvort = D.k.nvort; a.nx = D.g.nx;
+    SlotDyn y;
+    std::vector<double> dp(6 * (size_t)D.ntiles);
+    DHIPC(hipMemcpyAsync(&y, d->d_dyn, sizeof(y), hipMemcpyDeviceToHost, c->stream));
+    for (int stg = 0; stg < 3; ++stg)
+        DHIPC(hipMemcpyAsync(dp.data() + 2 * (size_t)stg * D.ntiles, d->P.dpart + 2 * ((size_t)stg * d->tile_cap + D.tile0),
+                             16 * (size_t)D.ntiles, hipMemcpyDeviceToHost, c->stream));
+    char* o = (char*)buf + sizeof(a);
+    DHIPC(hipMemcpyAsync(o, d->P.eddy + 7 * D.eoff, ne * 8, hipMemcpyDeviceToHost, c->stream));
+    DHIPC(hipMemcpyAsync(o + ne * 8, d->P.prof + D.xoff, np * 8, hipMemcpyDeviceToHost, c->stream));
+    DHIPC(hipStreamSynchronize(c->stream));
+    // the diffusive DT history: maxima of nu and alpha over the last step's stages
+    double mn = 0.0, ma = 0.0;
+    for (size_t t = 0; t < 3 * (size_t)D.ntiles; ++t) { mn = fmax(mn, dp[2 * t]); ma = fmax(ma, dp[2 * t + 1]); }
+    a.time = y.time; a.max_nu = mn; a.max_alpha = ma;
+    memcpy(buf, &a, sizeof(a));
+    *bytes = need;
+    return LES_OK;
+}
+
+// after les_gpu_upload_state (which recomputes the velocity maxima of the new
+// state and clears the diffusive history): restores step, run time, eddies,
+// profiles and the nu/alpha maxima the next DT needs
+int duct_import_aux(les_gpu* c, const void* buf, size_t bytes)
+{
+    DuctState* d = c->d;
+    const SlotDesc& D = d->desc[0];
+    AuxDuct a;
+    if (bytes < sizeof(a)) { snprintf(c->err, sizeof(c->err), "aux blob too short"); return LES_ERR_PARAM; }
+    memcpy(&a, buf, sizeof(a));
+    const size_t ne = 7 * (size_t)D.k.nvort, np = 10 * (size_t)D.g.nx;
+    if (memcmp(a.magic, DUCT_AUX_MAGIC, 8) != 0 || a.mode != LES_MODE_DUCT || a.nvort != D.k.nvort || a.nx != D.g.nx ||
+        bytes != sizeof(a) + (ne + np) * 8 || a.step < 0) {
+        snprintf(c->err, sizeof(c->err), "aux blob does not match this context");
+        return LES_ERR_PARAM;
+    }
+    const char* in = (const char*)buf + sizeof(a);
+    DHIPC(hipMemcpyAsync(d->P.eddy + 7 * D.eoff, in, ne * 8, hipMemcpyHostToDevice, c->stream));
+    DHIPC(hipMemcpyAsync(d->P.prof + D.xoff, in + ne * 8, np * 8, hipMemcpyHostToDevice, c->stream));
+    SlotDyn y;
+    DHIPC(hipMemcpyAsync(&y, d->d_dyn, sizeof(y), hipMemcpyDeviceToHost, c->stream));
+    DHIPC(hipStreamSynchronize(c->stream));
+    y.step = a.step;
+    y.time = a.time;
+    y.done = 0;
+    const double h[2] = {a.max_nu, a.max_alpha};
+    DHIPC(hipMemcpyAsync(d->d_dyn, &y, sizeof(y), hipMemcpyHostToDevice, c->stream));
+    DHIPC(hipMemcpyAsync(d->P.dpart + 2 * (size_t)D.tile0, h, 16, hipMemcpyHostToDevice, c->stream));
+    DHIPC(hipStreamSynchronize(c->stream));
+    d->host_step[0] = a.step;
+    d->point_base[0] = a.step;
+    d->probe_base[0] = a.step;
+    c->step = a.step;
+    c->stat_count = a.stat_count;
+    c->probe.clear();
+    return LES_OK;
+}
+
+int duct_set_step(les_gpu* c, int step)
+{
+    DuctState* d = c->d;
+    SlotDyn y;
+    DHIPC(hipMemcpyAsync(&y, d->d_dyn, sizeof(y), hipMemcpyDeviceToHost, c->stream));
+    DHIPC(hipStreamSynchronize(c->stream));
+    y.step = step;
+    y.done = 0;
+    DHIPC(hipMemcpyAsync(d->d_dyn, &y, sizeof(y), hipMemcpyHostToDevice, c->stream));
+    DHIPC(hipStreamSynchronize(c->stream));
+    d->host_step[0] = d->point_base[0] = d->probe_base[0] = step;
+    c->step = step;
+    return LES_OK;
+}
+
+int duct_init_sem(les_gpu* c)
+{
+    DuctState* d = c->d;
+    const SlotDesc& D = d->desc[0];
+    if (D.k.nvort > 0) {
+        hipLaunchKernelGGL(k_duct_eddy_init, dim3((D.k.nvort + 255) / 256), dim3(256), 0, c->stream, d->P, 0);
+        DLAUNCH_CHECK();
+    }
+    DHIPC(hipStreamSynchronize(c->stream));
+    return LES_OK;
 }

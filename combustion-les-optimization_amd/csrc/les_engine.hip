@@ -239,6 +239,26 @@ static int create_ref(les_gpu* c)
     c->allocs.push_back(at);
     r.attr = (int*)at;
     for (int a = 0; a < 10; ++a) if (!(r.stat[a] = dalloc(c, n))) return set_err(c, LES_ERR_HIP, "device allocation failed");
+    {
+        void* pi = nullptr;
+        HIPC(hipMalloc(&pi, LES_NPOINT_MAX * sizeof(int)));
+        c->allocs.push_back(pi);
+        r.pt_idx = (int*)pi;
+        r.pt_ring = dalloc(c, (size_t)c->pt_cap_r * LES_NPOINT_MAX * 4);
+        if (!r.pt_ring) return set_err(c, LES_ERR_HIP, "device allocation failed");
+        // setup_measurement_points (:3780-3822): the centre axis and the shear
+        // layer (j = R_inlet/DY) at x = 4 R_inlet and 8 R_inlet, k = NZ/2
+        int cells[4], np = 0;
+        const int js = (int)(c->p.inlet_radius / c->g.dy);
+        const int il[2] = {(int)(4.0 * c->p.inlet_radius / c->g.dx), (int)(8.0 * c->p.inlet_radius / c->g.dx)};
+        for (int a = 0; a < 2; ++a) {
+            if (il[a] >= c->p.nx) continue;
+            cells[np++] = (int)lidx(c->g, il[a], 0, c->p.nz / 2);
+            cells[np++] = (int)lidx(c->g, il[a], js, c->p.nz / 2);
+        }
+        HIPC(hipMemcpy(r.pt_idx, cells, np * sizeof(int), hipMemcpyHostToDevice));
+        c->npts_r = np;
+    }
     c->vort.assign(7 * (size_t)(c->p.num_vortices > 0 ? c->p.num_vortices : 1), 0.0);
     c->noise_h.assign(3 * plane, 0.0);
     attribute_map(c, c->attr_h);
@@ -374,6 +394,15 @@ static int ref_rhs(les_gpu* c, bool temp_state, int stage, int t_loop)
     return LES_OK;
 }
 
+__global__ void k_ref_points(const int* idx, int npts, const double* u, const double* v, const double* w,
+                             const double* G, double* out)
+{
+    const int p = threadIdx.x;
+    if (p >= npts) return;
+    const int q = idx[p];
+    out[4 * p] = u[q]; out[4 * p + 1] = v[q]; out[4 * p + 2] = w[q]; out[4 * p + 3] = G[q];
+}
+
 static int ref_step(les_gpu* c, les_step_report* rep, bool last)
 {
     const GridDims& g = c->g;
@@ -428,6 +457,12 @@ static int ref_step(les_gpu* c, les_step_report* rep, bool last)
     hipLaunchKernelGGL(k_ref_bc_eos, dim3(nbc), dim3(256), 0, c->stream, g, rc, sp);
     LAUNCH_CHECK();
 
+    // point series (:4557-4569) into the device ring
+    if (c->npts_r > 0) {
+        hipLaunchKernelGGL(k_ref_points, dim3(1), dim3(64), 0, c->stream, r.pt_idx, c->npts_r, r.u, r.v, r.w, r.G,
+                           r.pt_ring + (size_t)(t % c->pt_cap_r) * LES_NPOINT_MAX * 4);
+        LAUNCH_CHECK();
+    }
     // probes (:4572-4574) into the host row buffer
     if (c->p.num_probe_x > 0) {
         ProbeArgs pa;
@@ -710,7 +745,172 @@ extern "C" int les_gpu_stats_count(const les_gpu* c) { return c ? c->stat_count 
 extern "C" int les_gpu_point_fetch(les_gpu* c, double* out, int max_steps, int* n_steps, int* n_points)
 {
     if (!c || !out || !n_steps || max_steps < 0) return LES_ERR_PARAM;
-    if (c->p.mode != LES_MODE_DUCT) return set_err(c, LES_ERR_STATE, "point series are recorded in duct mode");
-    if (n_points) *n_points = DUCT_NPTS;
-    return duct_point_fetch(c, out, max_steps, n_steps);
+    if (c->p.mode == LES_MODE_DUCT) {
+        if (n_points) *n_points = duct_npoints(c);
+        return duct_point_fetch(c, out, max_steps, n_steps);
+    }
+    // reference scheme: the newest pt_cap_r steps since the last fetch
+    const int total = c->step - c->pt_base_r;
+    const int avail = total < c->pt_cap_r ? total : c->pt_cap_r;
+    const int n = avail < max_steps ? avail : max_steps;
+    const int first = c->step - avail;
+    const int np = c->npts_r;
+    for (int a = 0; a < n; ++a) {
+        const double* src = c->r.pt_ring + (size_t)((first + a) % c->pt_cap_r) * LES_NPOINT_MAX * 4;
+        HIPC(hipMemcpyAsync(out + (size_t)a * np * 4, src, (size_t)np * 4 * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    }
+    HIPC(hipStreamSynchronize(c->stream));
+    c->pt_base_r = c->step;
+    *n_steps = n;
+    if (n_points) *n_points = np;
+    return LES_OK;
+}
+
+extern "C" int les_gpu_set_points(les_gpu* c, const int* ijk, int n)
+{
+    if (!c || n < 0 || n > LES_NPOINT_MAX || (n > 0 && !ijk)) return LES_ERR_PARAM;
+    int cells[LES_NPOINT_MAX];
+    for (int a = 0; a < n; ++a) {
+        const int i = ijk[3 * a], j = ijk[3 * a + 1], k = ijk[3 * a + 2];
+        if (i < 0 || i >= c->p.nx || j < 0 || j >= c->p.ny || k < 0 || k >= c->p.nz)
+            return set_err(c, LES_ERR_PARAM, "point %d (%d, %d, %d) is outside the grid", a, i, j, k);
+        cells[a] = (int)lidx(c->g, i, j, k);
+    }
+    if (c->p.mode == LES_MODE_DUCT) return duct_set_points(c, cells, n);
+    if (n > 0) HIPC(hipMemcpy(c->r.pt_idx, cells, n * sizeof(int), hipMemcpyHostToDevice));
+    c->npts_r = n;
+    c->pt_base_r = c->step;
+    return LES_OK;
+}
+
+extern "C" int les_gpu_get_points(les_gpu* c, int* ijk, int max_n, int* n)
+{
+    if (!c || !n || (max_n > 0 && !ijk)) return LES_ERR_PARAM;
+    std::vector<int> cells(LES_NPOINT_MAX);
+    int np;
+    if (c->p.mode == LES_MODE_DUCT) {
+        np = duct_npoints(c);
+        memcpy(cells.data(), c->d->desc[0].point_idx, np * sizeof(int));
+    } else {
+        np = c->npts_r;
+        HIPC(hipMemcpy(cells.data(), c->r.pt_idx, np * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    for (int a = 0; a < np && a < max_n; ++a) {
+        const int q = cells[a];
+        ijk[3 * a] = q / (c->p.ny * c->p.nz);
+        ijk[3 * a + 1] = (q / c->p.nz) % c->p.ny;
+        ijk[3 * a + 2] = q % c->p.nz;
+    }
+    *n = np;
+    return LES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// resume support: fields outside the checkpoint layout, and the engine state
+// that is not a field (versioned byte blob)
+// ---------------------------------------------------------------------------
+extern "C" int les_gpu_upload_field(les_gpu* c, const char* name, const double* data)
+{
+    if (!c || !name || !data) return LES_ERR_PARAM;
+    if (c->p.mode == LES_MODE_DUCT) return duct_upload_field(c, name, data);
+    const size_t b = (size_t)c->g.n * sizeof(double);
+    double* dst = nullptr;
+    if (strcmp(name, "nu") == 0) dst = c->r.nu;
+    else if (strcmp(name, "alpha") == 0) dst = c->r.alpha;
+    else if (strncmp(name, "stat", 4) == 0) {
+        int a = atoi(name + 4);
+        if (a < 0 || a > 9) return LES_ERR_PARAM;
+        dst = c->r.stat[a];
+    }
+    if (!dst) return set_err(c, LES_ERR_PARAM, "field %s cannot be uploaded", name);
+    HIPC(hipMemcpyAsync(dst, data, b, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return LES_OK;
+}
+
+extern "C" int les_gpu_set_stats_count(les_gpu* c, int count)
+{
+    if (!c || count < 0) return LES_ERR_PARAM;
+    c->stat_count = count;
+    return LES_OK;
+}
+
+static const char AUX_MAGIC[8] = {'L', 'E', 'S', 'A', 'U', 'X', '1', 0};
+
+struct AuxRef {
+    char magic[8];
+    int mode, step, stat_count, nvort;
+    int rand_type, rand_deg, rand_sep;
+    long long fptr, rptr, state, end;     // offsets into the state bytes
+    char rstate[128];
+};
+
+extern "C" int les_gpu_export_aux(les_gpu* c, void* buf, size_t* bytes)
+{
+    if (!c || !bytes) return LES_ERR_PARAM;
+    if (c->p.mode == LES_MODE_DUCT) return duct_export_aux(c, buf, bytes);
+    const size_t nv = 7 * (size_t)c->p.num_vortices;
+    const size_t need = sizeof(AuxRef) + nv * sizeof(double);
+    if (!buf || *bytes < need) { *bytes = need; return buf ? LES_ERR_PARAM : LES_OK; }
+    AuxRef a;
+    memset(&a, 0, sizeof(a));
+    memcpy(a.magic, AUX_MAGIC, 8);
+    a.mode = LES_MODE_REFERENCE; a.step = c->step; a.stat_count = c->stat_count; a.nvort = c->p.num_vortices;
+    a.rand_type = c->rd.rand_type; a.rand_deg = c->rd.rand_deg; a.rand_sep = c->rd.rand_sep;
+    const char* base = c->rstate;
+    a.fptr = (const char*)c->rd.fptr - base; a.rptr = (const char*)c->rd.rptr - base;
+    a.state = (const char*)c->rd.state - base; a.end = (const char*)c->rd.end_ptr - base;
+    memcpy(a.rstate, c->rstate, sizeof(a.rstate));
+    memcpy(buf, &a, sizeof(a));
+    memcpy((char*)buf + sizeof(a), c->vort.data(), nv * sizeof(double));
+    *bytes = need;
+    return LES_OK;
+}
+
+extern "C" int les_gpu_import_aux(les_gpu* c, const void* buf, size_t bytes)
+{
+    if (!c || !buf) return LES_ERR_PARAM;
+    if (c->p.mode == LES_MODE_DUCT) return duct_import_aux(c, buf, bytes);
+    AuxRef a;
+    if (bytes < sizeof(a)) return set_err(c, LES_ERR_PARAM, "aux blob too short");
+    memcpy(&a, buf, sizeof(a));
+    const size_t nv = 7 * (size_t)c->p.num_vortices;
+    if (memcmp(a.magic, AUX_MAGIC, 8) != 0 || a.mode != LES_MODE_REFERENCE || a.nvort != c->p.num_vortices ||
+        bytes != sizeof(a) + nv * sizeof(double))
+        return set_err(c, LES_ERR_PARAM, "aux blob does not match this context");
+    const long long lim = (long long)sizeof(c->rstate);
+    if (a.fptr < 0 || a.fptr >= lim || a.rptr < 0 || a.rptr >= lim || a.state < 0 || a.state >= lim || a.end < 0 ||
+        a.end > lim)
+        return set_err(c, LES_ERR_PARAM, "aux blob holds an invalid random-state layout");
+    memcpy(c->rstate, a.rstate, sizeof(c->rstate));
+    c->rd.rand_type = a.rand_type; c->rd.rand_deg = a.rand_deg; c->rd.rand_sep = a.rand_sep;
+    c->rd.fptr = (int32_t*)(c->rstate + a.fptr); c->rd.rptr = (int32_t*)(c->rstate + a.rptr);
+    c->rd.state = (int32_t*)(c->rstate + a.state); c->rd.end_ptr = (int32_t*)(c->rstate + a.end);
+    memcpy(c->vort.data(), (const char*)buf + sizeof(a), nv * sizeof(double));
+    HIPC(hipMemcpy(c->r.vort, c->vort.data(), nv * sizeof(double), hipMemcpyHostToDevice));
+    c->step = a.step;
+    c->stat_count = a.stat_count;
+    c->pt_base_r = a.step;
+    c->probe.clear();
+    return LES_OK;
+}
+
+// restart from a state without engine aux state (VTK restart, :4269-4280):
+// step index and fresh SEM eddies (initialize_vortices :861-877)
+extern "C" int les_gpu_set_step(les_gpu* c, int step)
+{
+    if (!c || step < 0) return LES_ERR_PARAM;
+    if (c->p.mode == LES_MODE_DUCT) return duct_set_step(c, step);
+    c->step = step;
+    c->pt_base_r = step;
+    return LES_OK;
+}
+
+extern "C" int les_gpu_init_sem(les_gpu* c)
+{
+    if (!c) return LES_ERR_PARAM;
+    if (c->p.mode == LES_MODE_DUCT) return duct_init_sem(c);
+    init_vortices(c);
+    return LES_OK;
 }

@@ -32,6 +32,8 @@ struct R {
     double* probe_rows;
     int* probe_flags;
     double* stat[10];
+    int* pt_idx;             // point-series cells
+    double* pt_ring;         // [cap][point][u, v, w, G]
 };
 
 struct les_gpu {
@@ -61,6 +63,10 @@ struct les_gpu {
     double step_ms = 0.0;
     hipGraphExec_t graph_exec = nullptr;
     double* mon_partial = nullptr;
+    // point series of the reference scheme (point sampling :4557-4569)
+    int npts_r = 0;
+    int pt_cap_r = 4096;
+    int pt_base_r = 0;
     R r;
     DuctState* d = nullptr;
 };
@@ -79,3 +85,10 @@ int  duct_step(les_gpu* c, int n_steps, les_step_report* rep);
 int  duct_rhs(les_gpu* c, double* const* out, int n_out);
 void duct_params_changed(les_gpu* c);
 int  duct_point_fetch(les_gpu* c, double* out, int max_steps, int* n_steps);
+int  duct_set_points(les_gpu* c, const int* cells, int n);
+int  duct_upload_field(les_gpu* c, const char* name, const double* data);
+int  duct_export_aux(les_gpu* c, void* buf, size_t* bytes);
+int  duct_import_aux(les_gpu* c, const void* buf, size_t bytes);
+int  duct_set_step(les_gpu* c, int step);
+int  duct_init_sem(les_gpu* c);
+int  duct_npoints(const les_gpu* c);

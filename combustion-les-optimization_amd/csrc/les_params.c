@@ -386,6 +386,7 @@ int les_batch_config_default(les_batch_config* cfg)
     cfg->xcd_map = 1;
     cfg->autotune = 0;
     cfg->point_cap = 256;
+    cfg->probe_cap = 0;
     cfg->stats_per_cell = 0;
     cfg->table_t0 = 300.0;
     cfg->table_dt = 1.0;

@@ -207,10 +207,32 @@ int les_gpu_rhs(les_gpu* ctx, int t_loop, double* const* out, int n_out);
 
 /* Probe rows recorded since the last fetch (up to max_rows); *n_rows = count. */
 int les_gpu_probe_fetch(les_gpu* ctx, les_probe_row* rows, int max_rows, int* n_rows);
-/* Fixed-point time series (duct mode: 64 points, 4 x-stations x 4 y x 4 z)
- * recorded since the last fetch: out[step][point][u, v, w, G], oldest first
- * (point sampling :4558-4569; flush_point_data :3824-3850 writes them out). */
+/* Fixed-point time series recorded since the last fetch (the newest 4096
+ * steps): out[step][point][u, v, w, G], oldest first (point sampling
+ * :4557-4569; flush_point_data :3824-3850 writes them out). */
 int les_gpu_point_fetch(les_gpu* ctx, double* out, int max_steps, int* n_steps, int* n_points);
+/* Choose the point-series cells (n <= 64 triples i, j, k); mode R starts
+ * with the reference's setup_measurement_points (:3780-3822) list, duct mode
+ * with 4 x 4 x 4 evenly spaced points.  Replaces setup_measurement_points. */
+int les_gpu_set_points(les_gpu* ctx, const int* ijk, int n);
+int les_gpu_get_points(les_gpu* ctx, int* ijk, int max_n, int* n);
+/* Resume support (load_quick_checkpoint :3701-3737, load_statistics_backup
+ * :3759-3777, with the state those files leave out).  upload_field writes
+ * "stat0".."stat9" (FP64 accumulators), mode R "nu"/"alpha" (the diffusive
+ * history the next DT reads), duct "profiles"/"eddies".  export_aux writes
+ * an opaque versioned blob of the engine state that is not a field (step,
+ * SEM state: mode R's glibc rand() stream and eddies, duct eddies, run time,
+ * profiles and DT history); *bytes is the capacity in, the size out (buf NULL
+ * asks for the size).  import_aux restores it after les_gpu_upload_state;
+ * with the fields, the run then continues exactly as if never interrupted. */
+int les_gpu_upload_field(les_gpu* ctx, const char* name, const double* data);
+int les_gpu_set_stats_count(les_gpu* ctx, int count);
+int les_gpu_export_aux(les_gpu* ctx, void* buf, size_t* bytes);
+int les_gpu_import_aux(les_gpu* ctx, const void* buf, size_t bytes);
+/* For a restart without aux state (VTK): the step index, and fresh SEM
+ * eddies (initialize_vortices :861-877, as the reference's restart does). */
+int les_gpu_set_step(les_gpu* ctx, int step);
+int les_gpu_init_sem(les_gpu* ctx);
 /* Change the optimiser-tunable constants between steps. */
 int les_gpu_set_params(les_gpu* ctx, const les_params* p);
 /* Host-visible SEM state (mode R keeps the eddies and glibc rand() stream on the host). */
@@ -251,6 +273,7 @@ typedef struct les_batch_config {
     int       autotune;        /* 1: time tile-width and XCD-mapping candidates at create
                                   on the probe grid and keep the fastest */
     int       point_cap;       /* steps kept in each run's point-series ring */
+    int       probe_cap;       /* steps kept in each run's flame-region probe ring (0: off) */
     int       stats_per_cell;  /* 1: per-cell FP64 accumulators (10 per cell) */
     double    table_t0, table_dt;   /* cp/mu tables, shared by the pool */
     int       table_n;
@@ -299,6 +322,9 @@ int les_batch_download(les_batch* b, int slot, double* const* fields, int n_fiel
 int les_batch_download_field(les_batch* b, int slot, const char* name, double* out);
 int les_batch_report(les_batch* b, int slot, les_step_report* rep);
 int les_batch_point_fetch(les_batch* b, int slot, double* out, int max_steps, int* n_steps);
+/* Flame-region probe rows of one run recorded since the last fetch (the newest
+ * probe_cap steps; record_flame_region_data :3403-3471 rows). */
+int les_batch_probe_fetch(les_batch* b, int slot, les_probe_row* rows, int max_rows, int* n_rows);
 
 #ifdef __cplusplus
 }

@@ -714,6 +714,45 @@ static double duct_dt(ora_ctx* c, double* maxvel_out)
     return fmin(dtc, fmin(dtn, dta));
 }
 
+/* ---- flame-region probe rows (record_flame_region_data :3403-3471
+ * structure) on the duct state: G = c(phi), u = rho*u / rho, nu_t and |S| of
+ * the last stage ---- */
+static void duct_probes(ora_ctx* c, int t_loop)
+{
+    ora_duct* d = c->duct;
+    const int kp = c->nz / 2;
+    for (int n = 0; n < c->p.num_probe_x; ++n) {
+        double xp = c->p.probe_x[n];
+        int ip = (int)(xp / c->dx);
+        if (ip <= 0 || ip >= c->nx - 1) continue;
+        double yf = -1.0;
+        for (int j = 0; j < c->ny - 1; ++j) {
+            double g1 = ora_duct_progress(d->q[3][IX(c, ip, j, kp)], c->dx);
+            double g2 = ora_duct_progress(d->q[3][IX(c, ip, j + 1, kp)], c->dx);
+            if ((g1 - 0.5) * (g2 - 0.5) <= 0.0) {
+                double y1 = j * c->dy, y2 = (j + 1) * c->dy;
+                if (fabs(g2 - g1) < 1e-9) yf = y1;
+                else yf = y1 + (0.5 - g1) / (g2 - g1) * (y2 - y1);
+                break;
+            }
+        }
+        if (!(yf > 0.0)) continue;
+        for (int j = 0; j < c->ny; ++j) {
+            double yp = j * c->dy, yr = yp - yf;
+            if (fabs(yr) > 1.0) continue;
+            if (c->n_probe >= c->probe_cap) return;
+            size_t q = IX(c, ip, j, kp);
+            double rho = duct_rho(c, d->q[4][q]);
+            les_probe_row* r = &c->probe[c->n_probe++];
+            r->timestep = t_loop; r->station = n;
+            r->x_location = xp; r->y_flame = yf; r->y_physical = yp; r->y_relative = yr;
+            r->u = d->q[0][q] / rho; r->v = d->q[1][q] / rho; r->w = d->q[2][q] / rho;
+            r->G = ora_duct_progress(d->q[3][q], c->dx);
+            r->nu_t = d->nu_t[q]; r->S_mag = d->S_mag[q];
+        }
+    }
+}
+
 int ora_duct_step(ora_ctx* c, les_step_report* rep)
 {
     ora_duct* d = c->duct;
@@ -735,6 +774,7 @@ int ora_duct_step(ora_ctx* c, les_step_report* rep)
     if (c->p.reinit_interval > 0 && (c->step + 1) % c->p.reinit_interval == 0)
         ora_duct_reinit(c, c->p.reinit_iters);
     if (c->p.stats_start_step >= 0 && c->step >= c->p.stats_start_step) duct_stats(c);
+    duct_probes(c, c->step);
     if (rep) {
         rep->step = c->step; rep->dt = DT; rep->max_velocity = maxvel;
         double tm = 0, mnT = 1e9, mxT = -1e9, mnG = 1e9, mxG = -1e9;
