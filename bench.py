@@ -54,6 +54,7 @@ WORKLOADS = {
                                desc="128^3 periodic duct, MUSCL-minmod, SSP-RK3 CFL 0.3, FP64"),
     "ensemble64-fp32": dict(dtype="f32", desc="64 ensemble members, 64x48x48 each, FP32, one pool"),
     "served96-fp32": dict(dtype="f32", desc="96 served requests, Poisson arrivals (mean gap 2 s)"),
+    "copy": dict(dtype="f64", desc="plain device copy of 1 GiB (HBM calibration point)"),
 }
 RHS_KERNELS = "k_duct_rhs_scalar + k_duct_rhs_momentum (one RK stage's RHS and combine, two launches)"
 
@@ -272,6 +273,17 @@ def main():
         return
     if a.workload == "served96-fp32":
         print(json.dumps(served_line(a, lesgpu, serving)))
+        return
+    if a.workload == "copy":
+        import ctypes as C
+        L = lesgpu.lib()
+        L.les_copy_bandwidth.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        ms, gbps = C.c_double(), C.c_double()
+        assert L.les_copy_bandwidth(1 << 30, max(a.steps, 1), C.byref(ms), C.byref(gbps)) == 0
+        print(json.dumps({"metric": "device copy bandwidth (read + write)", "value": gbps.value, "unit": "GB/s",
+                          "bytes_copied": 1 << 30, "ms_per_copy": ms.value, "reps": max(a.steps, 1),
+                          "peak": PEAK_HBM_GBS, "frac": gbps.value / PEAK_HBM_GBS, "dtype": "f64",
+                          "data": "synthetic"}))
         return
     out, p = single_line(a, lesgpu, a.workload)
     wl = WORKLOADS[a.workload]

@@ -914,3 +914,52 @@ extern "C" int les_gpu_init_sem(les_gpu* c)
     init_vortices(c);
     return LES_OK;
 }
+
+// ---------------------------------------------------------------------------
+// HBM reference point: a plain streaming copy of `bytes` (FP64 elements, one
+// read and one write each), timed with HIP events over `reps` launches.  Its
+// rocprof FETCH_SIZE / WRITE_SIZE readings calibrate the counters the stencil
+// kernels are measured with, and its GB/s is the achievable bandwidth.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_copy_stream(long long n, const double2* __restrict__ a,
+                                                     double2* __restrict__ b)
+{
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+        b[q] = a[q];
+}
+
+extern "C" int les_copy_bandwidth(size_t bytes, int reps, double* ms_per_copy, double* gbps)
+{
+    if (bytes < 4096 || reps < 1 || !ms_per_copy || !gbps) return LES_ERR_PARAM;
+    const long long n = (long long)(bytes / 16);
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = LES_OK;
+    if (hipMalloc(&a, (size_t)n * 16) != hipSuccess || hipMalloc(&b, (size_t)n * 16) != hipSuccess ||
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+        rc = LES_ERR_HIP;
+    if (rc == LES_OK && hipMemsetAsync(a, 0, (size_t)n * 16, s) != hipSuccess) rc = LES_ERR_HIP;
+    const int grid = 4096;
+    if (rc == LES_OK) {
+        hipLaunchKernelGGL(k_copy_stream, dim3(grid), dim3(256), 0, s, n, (const double2*)a, (double2*)b);  // warm-up
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_copy_stream, dim3(grid), dim3(256), 0, s, n, (const double2*)a, (double2*)b);
+        (void)hipEventRecord(e1, s);
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) rc = LES_ERR_HIP;
+    }
+    if (rc == LES_OK) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        *ms_per_copy = ms / reps;
+        *gbps = 2.0 * (double)n * 16 / (*ms_per_copy * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}

@@ -244,6 +244,9 @@ int les_gpu_kernel_timing(const les_gpu* ctx, double* rhs_ms, int* launches, dou
 int les_gpu_set_timing(les_gpu* ctx, int on);
 /* Number of steps accumulated in the FP64 statistics (:3191 time_avg_steps). */
 int les_gpu_stats_count(const les_gpu* ctx);
+/* Achievable HBM bandwidth: a plain device copy of `bytes` (read + write),
+ * timed with HIP events (the calibration point of the roofline figures). */
+int les_copy_bandwidth(size_t bytes, int reps, double* ms_per_copy, double* gbps);
 /* sizeof(les_params), sizeof(les_mixture), sizeof(les_step_report),
  * sizeof(les_probe_row), sizeof(les_batch_config), sizeof(les_slot_info):
  * lets bindings check their struct mirrors. */
