@@ -55,6 +55,7 @@ struct DuctConsts {
     int reinit_interval, reinit_iters;
     int proj_iters;         // projection relaxation sweeps per step (0: off)
     int stats_start, stats_per_cell;
+    double dx4;             // dx^4 (host pow, as the oracle's)
     double L;               // eddy length scale
     double ly, lz;
     unsigned long long seed;
@@ -141,16 +142,42 @@ __host__ __device__ __forceinline__ double duniform(unsigned long long seed, uns
 
 __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
+// Arithmetic of the two storage types.  double: IEEE operations and the
+// device libm (the FP64 path is compared with the oracle).  float (the FP32
+// form): the hardware's native reciprocal, exp2 and log2 (v_rcp_f32,
+// v_exp_f32, v_log_f32; about 1 ulp), which the FP32 tolerances cover.
+template <typename real>
+struct M {
+    static __device__ __forceinline__ real div(real a, real b) { return a / b; }
+    static __device__ __forceinline__ real exp(real x) { return ::exp(x); }
+    static __device__ __forceinline__ real log(real x) { return ::log(x); }
+    static __device__ __forceinline__ real pow(real x, real y) { return ::pow(x, y); }
+    static __device__ __forceinline__ real tanh(real x) { return ::tanh(x); }
+};
+
+template <>
+struct M<float> {
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504089f); }
+    static __device__ __forceinline__ float log(float x) { return __builtin_amdgcn_logf(x) * 0.69314718056f; }
+    static __device__ __forceinline__ float pow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+    static __device__ __forceinline__ float tanh(float x)
+    {
+        const float t = __builtin_amdgcn_exp2f(-2.88539008178f * fabsf(x));     // exp(-2|x|)
+        return copysignf((1.0f - t) * __builtin_amdgcn_rcpf(1.0f + t), x);
+    }
+};
+
 template <typename real>
 __device__ __forceinline__ real progress(real phi, real dx)
 {
-    return (real)0.5 * ((real)1.0 + tanh(phi / ((real)2.0 * dx)));
+    return (real)0.5 * ((real)1.0 + M<real>::tanh(M<real>::div(phi, (real)2.0 * dx)));
 }
 
 template <typename real>
 __device__ __forceinline__ real table_lookup(const double* tab, const DuctConsts& k, real T)
 {
-    real s = (T - (real)k.table_t0) / (real)k.table_dt;
+    real s = M<real>::div(T - (real)k.table_t0, (real)k.table_dt);
     if (s < (real)0.0) s = (real)0.0;
     if (s > (real)(k.table_n - 1)) s = (real)(k.table_n - 1);
     int i0 = (int)s;
@@ -176,7 +203,7 @@ __device__ __forceinline__ real weno5_face(real v0, real v1, real v2, real v3, r
     real p0 = (real)2.0 * v0 - (real)7.0 * v1 + (real)11.0 * v2;
     real p1 = -v1 + (real)5.0 * v2 + (real)2.0 * v3;
     real p2 = (real)2.0 * v2 + (real)5.0 * v3 - v4;
-    return (w0 * p0 + w1 * p1 + w2 * p2) / ((real)6.0 * (w0 + w1 + w2));
+    return M<real>::div(w0 * p0 + w1 * p1 + w2 * p2, (real)6.0 * (w0 + w1 + w2));
 }
 
 template <typename real>
@@ -441,12 +468,12 @@ __global__ void __launch_bounds__(256) k_duct_sgs(Pool P, int stage, int xcd_map
                 nut[q] = (real)0.0; stv[q] = SL;
             } else {
                 const real DX = (real)g.dx;
-                real T = Tf[q], rf = (real)k.rho_u_Tu / T;
+                real T = Tf[q], rf = M<real>::div((real)k.rho_u_Tu, T);
                 real mu = table_lookup<real>(P.mu_tab, k, T);
                 real cp = table_lookup<real>(P.cp_tab, k, T);
-                real nu = mu / rf;
-                real lam = mu * cp / (real)k.prandtl;
-                real al = lam / (rf * cp);
+                real nu = M<real>::div(mu, rf);
+                real lam = M<real>::div(mu * cp, (real)k.prandtl);
+                real al = M<real>::div(lam, rf * cp);
                 mn = (double)nu; ma = (double)al;
                 const int xp = q + (int)g.sx, xm = q - (int)g.sx;
                 const int yp = (int)lidx(g, i, wrapi(j + 1, g.ny), kk), ym = (int)lidx(g, i, wrapi(j - 1, g.ny), kk);
@@ -471,19 +498,19 @@ __global__ void __launch_bounds__(256) k_duct_sgs(Pool P, int stage, int xcd_map
                 nut[q] = (cs * DX) * (cs * DX) * smag;
                 // SSR-SGS flame speed
                 real num = ss - (div * div) * ((real)1.0 / (real)3.0);
-                real X = (pow(DX, (real)4.0) / (nu * nu)) * num;
+                real X = M<real>::div((real)k.dx4, nu * nu) * num;
                 real d_eta;
                 if (X < (real)LES_SMALL) d_eta = (real)1.0;
                 else {
-                    real px = pow(X, (real)0.75);
+                    real px = M<real>::pow(X, (real)0.75);
                     real alg = (real)0.2187 * px;
-                    d_eta = (real)0.2340 * px * (alg > (real)1.0 ? log(alg) : (real)0.0);
+                    d_eta = (real)0.2340 * px * (alg > (real)1.0 ? M<real>::log(alg) : (real)0.0);
                 }
-                real eta = DX / (d_eta + (real)LES_SMALL);
-                real dF = (real)k.nu_unburnt / (SL + (real)LES_SMALL);
-                real asf = (real)8.0 * exp((real)10.0 * dF / ((real)8.0 * eta + (real)LES_SMALL));
-                real wr = pow(asf * eta / DX, (real)2.0 - (real)k.d3);
-                real te = ((real)k.tau * SL > (real)LES_SMALL) ? DX * fmax((real)0.0, div) / ((real)k.tau * SL) : (real)0.0;
+                real eta = M<real>::div(DX, d_eta + (real)LES_SMALL);
+                real dF = M<real>::div((real)k.nu_unburnt, SL + (real)LES_SMALL);
+                real asf = (real)8.0 * M<real>::exp(M<real>::div((real)10.0 * dF, (real)8.0 * eta + (real)LES_SMALL));
+                real wr = M<real>::pow(M<real>::div(asf * eta, DX), (real)2.0 - (real)k.d3);
+                real te = ((real)k.tau * SL > (real)LES_SMALL) ? M<real>::div(DX * fmax((real)0.0, div), (real)k.tau * SL) : (real)0.0;
                 stv[q] = fmax(SL, SL * fmax((real)0.0, wr + te));
             }
         }
@@ -596,16 +623,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
         real art = (real)0;
         if (g0 > (real)0.1 && g0 < (real)0.99) {
             real df = g0 - ctr;
-            real win = exp(-(df * df) / ((real)2.0 * wdt * wdt));
+            real win = M<real>::exp(M<real>::div(-(df * df), (real)2.0 * wdt * wdt));
             art = (real)5.0 * (nu0 + (real)1.0e-5) * win * lap[0];
         }
         const real* stv = (const real*)P.st + D.off;
         real L3 = -cv[0] + stv[q] * gmag + sg[0] + art;
-        const real rf0 = rut / T0;
+        const real rf0 = M<real>::div(rut, T0);
         real mu = table_lookup<real>(P.mu_tab, k, T0);
         real cp = table_lookup<real>(P.cp_tab, k, T0);
-        real lam = mu * cp / (real)k.prandtl;
-        real al = lam / (rf0 * cp);
+        real lam = M<real>::div(mu * cp, (real)k.prandtl);
+        real al = M<real>::div(lam, rf0 * cp);
         const real Tu = (real)k.Tu, Tb = (real)k.Tb;
         real Ttg = Tu + g0 * (Tb - Tu);
         real L4 = -cv[1] + al * lap[1] + sg[1] + (real)k.relaxation * (Ttg - T0);
@@ -664,7 +691,7 @@ __global__ void __launch_bounds__(256) k_duct_rhs_momentum(Pool P, RhsArgs<real>
                 const int scheme = k.scheme;
                 const real* vel[3] = {uf, vf, wf};
                 const real T0 = Tf[q];
-                const real rf0 = rut / T0;
+                const real rf0 = M<real>::div(rut, T0);
                 real conv[3] = {(real)0, (real)0, (real)0};
 #pragma unroll 1
                 for (int d = 0; d < 3; ++d) {
@@ -673,8 +700,8 @@ __global__ void __launch_bounds__(256) k_duct_rhs_momentum(Pool P, RhsArgs<real>
                     const real* U = vel[d];
                     real up = (real)0.5 * (U[qp] + U[q]);
                     real um = (real)0.5 * (U[q] + U[qm]);
-                    real rp = (real)0.5 * (rut / Tf[qp] + rf0);
-                    real rm = (real)0.5 * (rf0 + rut / Tf[qm]);
+                    real rp = (real)0.5 * (M<real>::div(rut, Tf[qp]) + rf0);
+                    real rm = (real)0.5 * (rf0 + M<real>::div(rut, Tf[qm]));
                     real mp = rp * up, mm = rm * um;
                     const bool band = (d == 0 && xband);
 #pragma unroll
@@ -723,7 +750,7 @@ __global__ void __launch_bounds__(256) k_duct_rhs_momentum(Pool P, RhsArgs<real>
                     real gy = (progress<real>(pf[yp], DX) - progress<real>(pf[ym], DX)) * (real)(0.5 / g.dy);
                     real gz = (progress<real>(pf[zp], DX) - progress<real>(pf[zm], DX)) * (real)(0.5 / g.dz);
                     real df = g0 - ctr;
-                    real win = exp(-(df * df) / ((real)2.0 * wdt * wdt));
+                    real win = M<real>::exp(M<real>::div(-(df * df), (real)2.0 * wdt * wdt));
                     real common = (real)k.accel_strength * rf0 * win;
                     fx = common * gx * (real)k.weight_x;
                     fy = common * gy;
@@ -750,8 +777,8 @@ __global__ void __launch_bounds__(256) k_duct_rhs_momentum(Pool P, RhsArgs<real>
                     qo[f][q] = out[f];
                 }
                 if (a.track_vel) {
-                    real rho = rut / ((const real*)QO[4] + D.off)[q];
-                    real uu = out[0] / rho, vv = out[1] / rho, ww = out[2] / rho;
+                    real rho = M<real>::div(rut, ((const real*)QO[4] + D.off)[q]);
+                    real uu = M<real>::div(out[0], rho), vv = M<real>::div(out[1], rho), ww = M<real>::div(out[2], rho);
                     vmax = (double)sqrt(uu * uu + vv * vv + ww * ww);
                 }
             }
@@ -820,10 +847,10 @@ __global__ void __launch_bounds__(256) k_duct_velmax(Pool P, int only_slot, int 
         double vm = 0.0;
         if (tp.in && tp.i <= D.g.nx - 2) {
             const int q = (int)lidx(D.g, tp.i, tp.j, tp.kk);
-            real rho = (real)D.k.rho_u_Tu / ((const real*)P.A[4] + D.off)[q];
-            real u = ((const real*)P.A[0] + D.off)[q] / rho;
-            real v = ((const real*)P.A[1] + D.off)[q] / rho;
-            real w = ((const real*)P.A[2] + D.off)[q] / rho;
+            real rho = M<real>::div((real)D.k.rho_u_Tu, ((const real*)P.A[4] + D.off)[q]);
+            real u = M<real>::div(((const real*)P.A[0] + D.off)[q], rho);
+            real v = M<real>::div(((const real*)P.A[1] + D.off)[q], rho);
+            real w = M<real>::div(((const real*)P.A[2] + D.off)[q], rho);
             vm = (double)sqrt(u * u + v * v + w * w);
         }
         double dummy = 0.0;
@@ -983,8 +1010,8 @@ __global__ void __launch_bounds__(256) k_duct_bc(Pool P)
         q2[p] = (real)(rho_u * w);
         q3[p] = q3[p + g.sx] - (real)g.dx;
         q4[p] = (real)k.Tu;
-        real rr = (real)k.rho_u_Tu / q4[p];
-        real uu = q0[p] / rr, vv = q1[p] / rr, ww = q2[p] / rr;
+        real rr = M<real>::div((real)k.rho_u_Tu, q4[p]);
+        real uu = M<real>::div(q0[p], rr), vv = M<real>::div(q1[p], rr), ww = M<real>::div(q2[p], rr);
         vm = (double)sqrt(uu * uu + vv * vv + ww * ww);
         const long long qn = (long long)(g.nx - 1) * g.sx + p, qm = qn - g.sx;
         q0[qn] = q0[qm]; q1[qn] = q1[qm]; q2[qn] = q2[qm]; q3[qn] = q3[qm]; q4[qn] = q4[qm];
@@ -1154,8 +1181,8 @@ __global__ void __launch_bounds__(256) k_duct_proj_correct(Pool P, int xcd_map)
         const real a1 = m1[q] - (proj_phi(phi, g, i, j + 1, kk) - proj_phi(phi, g, i, j - 1, kk)) * hy;
         const real a2 = m2[q] - (proj_phi(phi, g, i, j, kk + 1) - proj_phi(phi, g, i, j, kk - 1)) * hz;
         m0[q] = a0; m1[q] = a1; m2[q] = a2;
-        const real rho = (real)D.k.rho_u_Tu / ((const real*)P.A[4] + D.off)[q];
-        const real uu = a0 / rho, vv = a1 / rho, ww = a2 / rho;
+        const real rho = M<real>::div((real)D.k.rho_u_Tu, ((const real*)P.A[4] + D.off)[q]);
+        const real uu = M<real>::div(a0, rho), vv = M<real>::div(a1, rho), ww = M<real>::div(a2, rho);
         vm = (double)sqrt(uu * uu + vv * vv + ww * ww);
     }
     double tot = block_sum256(sr, red);
@@ -1566,6 +1593,7 @@ static DuctConsts make_duct_consts(const les_params& p)
     k.proj_iters = p.proj_iters;
     k.stats_start = p.stats_start_step; k.stats_per_cell = p.stats_per_cell;
     k.L = p.turbulent_length_scale;
+    k.dx4 = pow(p.lx / (double)p.nx, 4.0);
     k.ly = p.ly; k.lz = p.lz;
     k.seed = (unsigned long long)p.seed;
     return k;
