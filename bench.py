@@ -55,6 +55,9 @@ WORKLOADS = {
     "ensemble64-fp32": dict(dtype="f32", desc="64 ensemble members, 64x48x48 each, FP32, one pool"),
     "served96-fp32": dict(dtype="f32", desc="96 served requests, Poisson arrivals (mean gap 2 s)"),
     "copy": dict(dtype="f64", desc="plain device copy of 1 GiB (HBM calibration point)"),
+    "large256-fp32": dict(preset=4, nx=256, ny=192, nz=192, stages=3, bytes_per_cell_stage=53.3, over={},
+                          dtype="f32", desc="256x192x192 duct, FP32 bulk, FP64 projection residuals and "
+                                            "statistics, I=15 %, 2000 SEM eddies, 16 x-stations"),
 }
 RHS_KERNELS = "k_duct_rhs_scalar + k_duct_rhs_momentum (one RK stage's RHS and combine, two launches)"
 
@@ -204,7 +207,6 @@ def profile_error(lesgpu, serving, grid, steps):
     """Max relative error (per profile column, against its own scale) of the
     FP32 member's mean/Reynolds-stress plane profiles vs the FP64 path after
     `steps` steps (the FP64 GPU path equals the FP64 oracle within 1e-10)."""
-    out = {}
     profs = {}
     for prec in (1, 0):
         p = serving.ensemble_members(1, grid, seed=0, precision=prec)[0]
@@ -212,14 +214,33 @@ def profile_error(lesgpu, serving, grid, steps):
         try:
             g.init_state()
             g.step(steps)
-            profs[prec] = serving.reynolds_profiles(g.field("profiles", 10 * p.nx), p.nx, steps * 1.0)
+            profs[prec] = serving.reynolds_profiles(g.field("profiles", 10 * p.nx), p.nx, p.ny, p.nz, steps)
         finally:
             g.close()
-    names = ["u", "v", "w", "c", "uu", "vv", "ww", "uv", "uw", "vw"]
-    a, b = profs[1], profs[0]
-    for k, nm in enumerate(names):
-        scale = max(np.abs(b[:, k]).max(), 1e-300)
-        out[nm] = float(np.abs(a[:, k] - b[:, k]).max() / scale)
+    p = serving.ensemble_members(1, grid, seed=0, precision=0)[0]
+    out = serving.profile_errors(profs[1], profs[0], p.inlet_velocity, p.turbulence_intensity)
+    out["steps"] = steps
+    return out
+
+
+def station_profile_error(lesgpu, serving, p, steps):
+    """FP32 form vs the FP64 path (which equals the FP64 oracle within 1e-10):
+    mean and Reynolds-stress plane profiles at the run's x-stations after
+    `steps` steps, each column relative to its own largest value."""
+    profs = {}
+    for prec in (1, 0):
+        q = lesgpu.params(4, precision=prec)
+        g = lesgpu.LesGpu(q, lesgpu.mixture())
+        try:
+            g.init_state()
+            g.step(steps)
+            profs[prec] = serving.reynolds_profiles(g.field("profiles", 10 * q.nx), q.nx, q.ny, q.nz, steps)
+        finally:
+            g.close()
+    dx = p.lx / p.nx
+    idx = [int(p.probe_x[n] / dx) for n in range(p.num_probe_x) if 0 < int(p.probe_x[n] / dx) < p.nx]
+    out = serving.profile_errors(profs[1][idx], profs[0][idx], p.inlet_velocity, p.turbulence_intensity)
+    out["stations"] = len(idx)
     out["steps"] = steps
     return out
 
@@ -287,6 +308,9 @@ def main():
         return
     out, p = single_line(a, lesgpu, a.workload)
     wl = WORKLOADS[a.workload]
+    if a.workload == "large256-fp32":
+        out["metric"] = "cell-updates/s per RK stage, 256x192x192"
+        out["profiles_fp32_vs_fp64"] = station_profile_error(lesgpu, serving, p, a.steps)
     if a.workload == "duct128-fp64" and not a.no_extra:
         f32, _ = single_line(a, lesgpu, "duct128-fp32")
         out["fp32_form"] = {"value": f32["value"], "ms_per_step": f32["ms_per_step"], "roofline": f32["roofline"]}

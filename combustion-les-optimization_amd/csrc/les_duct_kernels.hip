@@ -117,6 +117,8 @@ struct Pool {
     int tile_cap;
     int ny_max, nz_max, nx_max, nzb;
     int point_cap;          // steps per point-series ring
+    double* sem_part;       // SEM partial planes: [slot][sem_split][3][ny_max*nz_max]
+    int sem_split;          // eddy chunks of SEM_CHUNK per run
     double* probe;          // probe ring: [slot][probe_cap][station][2 + 6*ny_max]
     int probe_cap;          // steps per probe ring (0: no probes)
 };
@@ -539,7 +541,8 @@ struct RhsArgs {
 
 // scalar RHS (phi: G equation :2706-2818; T :2912-2955) + SSP-RK3 combine
 template <typename real, int TK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_duct_rhs_scalar(Pool P, RhsArgs<real> a)
+// FP64: at least 4 waves per SIMD (128 VGPRs, a 12-byte spill), measured faster than 3
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_duct_rhs_scalar(Pool P, RhsArgs<real> a)
 {
     const int t = tile_of_block(P, a.xcd_map);
     TilePos tp;
@@ -656,7 +659,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 // momentum RHS (:2552-2691 structure) + SSP-RK3 combine; in the last stage also
 // the per-tile max |u| of the new state (its T was written by the scalar kernel)
 template <typename real, int TK>
-__global__ void __launch_bounds__(256) k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
+// FP64: at least 5 waves per SIMD (96 VGPRs, a 36-byte spill) measured faster
+// than 4 waves without the spill; FP32 needs fewer registers than the cap
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
 {
     __shared__ double vred[8];
     const int t = tile_of_block(P, a.xcd_map);
@@ -903,28 +908,33 @@ __global__ void k_duct_eddy_update(Pool P)
     }
 }
 
-// One thread per inflow cell; the slot's eddy table is staged through LDS.
-__global__ void __launch_bounds__(256) k_duct_sem_plane(Pool P)
+// One thread per inflow cell and eddy chunk: block (k-chunk of 64, j, run x
+// chunk) sums the Gaussian contributions of one chunk of SEM_CHUNK eddies
+// (staged through LDS) into an FP64 partial plane; k_duct_sem_norm adds the
+// chunks in order.
+#define SEM_CHUNK 256
+__global__ void __launch_bounds__(64) k_duct_sem_plane(Pool P)
 {
-    __shared__ double se[256 * 7];
-    const int s = blockIdx.z;
+    __shared__ double se[SEM_CHUNK * 7];
+    const int s = blockIdx.z / P.sem_split, sp = blockIdx.z % P.sem_split;
     const SlotDesc& D = P.desc[s];
     if (!D.active || P.dyn[s].done) return;
     const GridDims& g = D.g;
     const DuctConsts& k = D.k;
     const int j = blockIdx.y;
     if (j >= g.ny) return;
-    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
-    const long long PL = (long long)g.ny * g.nz;
-    const double y = j * g.dy, z = kk * g.dz, inv_l2 = 1.0 / (k.L * k.L);
-    const double* eddy = P.eddy + 7 * D.eoff;
+    const int kk = blockIdx.x * 64 + threadIdx.x;
+    const long long PLM = (long long)P.ny_max * P.nz_max;
+    double* out = P.sem_part + ((long long)s * P.sem_split + sp) * 3 * PLM;
+    const int e0 = sp * SEM_CHUNK;
+    const int cnt = min(SEM_CHUNK, k.nvort - e0);
     double su = 0.0, sv = 0.0, sw = 0.0;
-    for (int base = 0; base < k.nvort; base += 256) {
-        int cnt = min(256, k.nvort - base);
-        __syncthreads();
-        for (int u = threadIdx.x; u < cnt * 7; u += blockDim.x) se[u] = eddy[7 * (long long)base + u];
+    if (cnt > 0) {
+        const double* eddy = P.eddy + 7 * (D.eoff + e0);
+        for (int u = threadIdx.x; u < cnt * 7; u += blockDim.x) se[u] = eddy[u];
         __syncthreads();
         if (kk < g.nz) {
+            const double y = j * g.dy, z = kk * g.dz, inv_l2 = 1.0 / (k.L * k.L);
             for (int e = 0; e < cnt; ++e) {
                 const double* E = se + 7 * e;
                 double ddx = 0.0 - E[0];
@@ -938,13 +948,13 @@ __global__ void __launch_bounds__(256) k_duct_sem_plane(Pool P)
         }
     }
     if (kk < g.nz) {
-        double* in = P.inlet + D.poff;
         long long p = (long long)j * g.nz + kk;
-        in[p] = su; in[PL + p] = sv; in[2 * PL + p] = sw;
+        out[p] = su; out[PLM + p] = sv; out[2 * PLM + p] = sw;
     }
 }
 
-// mean removal and rms rescale, one 1024-thread block per (component, slot)
+// chunk sums -> plane (fixed order), mean removal and rms rescale; one
+// 1024-thread block per (component, run)
 __global__ void __launch_bounds__(1024) k_duct_sem_norm(Pool P)
 {
     __shared__ double red[1024];
@@ -952,10 +962,17 @@ __global__ void __launch_bounds__(1024) k_duct_sem_norm(Pool P)
     const SlotDesc& D = P.desc[s];
     if (!D.active || P.dyn[s].done) return;
     const long long PL = (long long)D.g.ny * D.g.nz;
+    const long long PLM = (long long)P.ny_max * P.nz_max;
     const double target = D.k.U * D.k.intensity;
     double* f = P.inlet + D.poff + blockIdx.x * PL;
+    const double* part = P.sem_part + (long long)s * P.sem_split * 3 * PLM + blockIdx.x * PLM;
     double sm = 0.0;
-    for (long long p = threadIdx.x; p < PL; p += 1024) sm += f[p];
+    for (long long p = threadIdx.x; p < PL; p += 1024) {
+        double v = 0.0;
+        for (int sp = 0; sp < P.sem_split; ++sp) v += part[(long long)sp * 3 * PLM + p];
+        f[p] = v;
+        sm += v;
+    }
     red[threadIdx.x] = sm;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1) {
@@ -1698,6 +1715,14 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
     P.ny_max = d->ny_max; P.nz_max = d->nz_max; P.nx_max = d->nx_max; P.nzb = d->nzb;
     P.point_cap = d->point_cap;
     P.probe_cap = d->probe_cap;
+    P.sem_split = (d->nvort_max + SEM_CHUNK - 1) / SEM_CHUNK;
+    if (P.sem_split < 1) P.sem_split = 1;
+    if ((long long)S * P.sem_split > 65535) {
+        snprintf(c->err, sizeof(c->err), "max_slots x eddy chunks exceeds the launch grid (65535)");
+        return LES_ERR_PARAM;
+    }
+    if (!(P.sem_part = (double*)dalloc_bytes(c, 8 * S * P.sem_split * 3 * (size_t)d->ny_max * d->nz_max)))
+        return LES_ERR_HIP;
     if (d->probe_cap > 0 &&
         !(P.probe = (double*)dalloc_bytes(c, 8 * S * d->probe_cap * LES_NPROBE_MAX * (size_t)PROBE_STRIDE(P))))
         return LES_ERR_HIP;
@@ -1860,7 +1885,8 @@ static int step_launches(les_gpu* c, bool reinit)
     const int S = d->max_slots;
     hipLaunchKernelGGL(k_duct_dt, dim3(S), dim3(1024), 0, c->stream, d->P);
     hipLaunchKernelGGL(k_duct_eddy_update, dim3((d->nvort_max + 255) / 256, S), dim3(256), 0, c->stream, d->P);
-    hipLaunchKernelGGL(k_duct_sem_plane, dim3(d->nzb, d->ny_max, S), dim3(256), 0, c->stream, d->P);
+    hipLaunchKernelGGL(k_duct_sem_plane, dim3((d->nz_max + 63) / 64, d->ny_max, S * d->P.sem_split), dim3(64), 0,
+                       c->stream, d->P);
     hipLaunchKernelGGL(k_duct_sem_norm, dim3(3, S), dim3(1024), 0, c->stream, d->P);
     DLAUNCH_CHECK();
     for (int s = 0; s < 3; ++s) {
@@ -2422,7 +2448,7 @@ struct les_batch {
 
 static int batch_config_check(const les_batch_config* cfg)
 {
-    if (!cfg || cfg->max_slots < 1 || cfg->max_slots > 65536 || cfg->pool_cells < 1 || cfg->max_nx < 6 ||
+    if (!cfg || cfg->max_slots < 1 || cfg->max_slots > 8192 || cfg->pool_cells < 1 || cfg->max_nx < 6 ||
         cfg->max_ny < 1 || cfg->max_nz < 1 || cfg->max_vortices < 0 || !tk_valid(cfg->tile_k) ||
         (cfg->precision != LES_PREC_FP64 && cfg->precision != LES_PREC_FP32) || cfg->point_cap < 1 ||
         cfg->table_n < 2 || cfg->table_dt <= 0.0)
@@ -2445,6 +2471,7 @@ extern "C" int les_batch_pool_bytes(const les_batch_config* cfg, size_t* bytes)
                   4 * (size_t)DUCT_NPTS * cfg->point_cap + (size_t)cfg->max_ny * ((cfg->max_nz + 255) / 256) +
                   10 * (size_t)cfg->max_nx * STAT_CHUNKS);
     b += S * (sizeof(SlotDesc) + sizeof(SlotDyn)) + 16 * (size_t)cfg->table_n + 8 * 6 * 1024;
+    b += S * 8 * 3 * (size_t)cfg->max_ny * cfg->max_nz * ((nv + SEM_CHUNK - 1) / SEM_CHUNK);   // SEM partials
     *bytes = b;
     return LES_OK;
 }

@@ -56,13 +56,13 @@ def served_requests(n=96, mean_gap_s=2.0, seed=0, precision=abi.LES_PREC_FP32):
     return out
 
 
-def reynolds_profiles(prof, nx, count):
-    """Plane profiles (10 FP64 sums per x-plane over count steps x ny*nz cells)
-    -> means and resolved Reynolds stresses per x-plane, the
-    calculate_resolved_reynolds_stresses (les_combustion_solver.c:3195-3219)
-    formulas <u_i u_j> - <u_i><u_j> applied to plane-time sums:
+def reynolds_profiles(prof, nx, ny, nz, steps):
+    """Plane profiles (10 FP64 sums per x-plane, each over `steps` steps x the
+    plane's ny*nz cells) -> means and resolved Reynolds stresses per x-plane,
+    the calculate_resolved_reynolds_stresses (les_combustion_solver.c:3195-3219)
+    formulas <u_i u_j> - <u_i><u_j> applied to plane-time averages:
     columns u, v, w, c, uu, vv, ww, uv, uw, vw."""
-    s = np.asarray(prof, dtype=np.float64).reshape(nx, 10) / float(count)
+    s = np.asarray(prof, dtype=np.float64).reshape(nx, 10) / (float(steps) * ny * nz)
     m = s[:, :4]
     r = np.empty((nx, 6))
     r[:, 0] = s[:, 4] - m[:, 0] * m[:, 0]
@@ -72,6 +72,28 @@ def reynolds_profiles(prof, nx, count):
     r[:, 4] = s[:, 8] - m[:, 0] * m[:, 2]
     r[:, 5] = s[:, 9] - m[:, 1] * m[:, 2]
     return np.concatenate([m, r], axis=1)
+
+
+PROFILE_COLUMNS = ["u", "v", "w", "c", "uu", "vv", "ww", "uv", "uw", "vw"]
+
+
+def profile_errors(a, b, U, intensity):
+    """Errors of profiles a against reference profiles b (rows: x-planes,
+    columns PROFILE_COLUMNS).  "rel": max |a - b| / max |b| per column.
+    "scaled": the same against the column's physical scale where the column is
+    near zero: U for the means, 1 for c, max(peak, 1 % of (U I)^2) for the
+    stresses (where the inflow turbulence has not arrived, <v>, <u'v'> ... are
+    far below the rounding of the means they are computed from)."""
+    out = {"rel": {}, "scaled": {}}
+    floor = 1e-2 * (U * intensity) ** 2
+    for k, nm in enumerate(PROFILE_COLUMNS):
+        d = float(np.abs(a[:, k] - b[:, k]).max())
+        peak = float(np.abs(b[:, k]).max())
+        out["rel"][nm] = d / max(peak, 1e-300)
+        scale = U if k < 3 else (1.0 if k == 3 else max(peak, floor))
+        out["scaled"][nm] = d / scale
+    out["max_scaled"] = max(out["scaled"].values())
+    return out
 
 
 def pool_config(max_slots, pool_cells, max_grid, precision=abi.LES_PREC_FP32, autotune=True, point_cap=64):

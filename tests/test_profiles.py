@@ -78,3 +78,26 @@ def test_csv_round_trip(tmp_path):
     want = brute(list(cols[:, 8]), list(cols[:, 5]), list(cols[:, 6]), 20)
     ok = ~np.isnan(want)
     assert counts.sum() == 800 and np.allclose(vals[ok], want[ok], rtol=1e-12, atol=1e-14)
+
+
+def test_plane_profiles_to_reynolds_stresses():
+    """Plane-time sums (the engine's 10 accumulators per x-plane) -> means and
+    <u_i u_j> - <u_i><u_j> (calculate_resolved_reynolds_stresses :3195-3219)
+    equal the sample statistics of the same samples."""
+    serving = importlib.import_module("combustion-les-optimization_amd.serving")
+    rng = np.random.default_rng(7)
+    nx, ny, nz, steps = 5, 4, 6, 9
+    u = 5 + rng.standard_normal((steps, nx, ny, nz))
+    v = 0.4 * (u - 5) + rng.standard_normal(u.shape)
+    w = rng.standard_normal(u.shape)
+    c = rng.uniform(0, 1, u.shape)
+    sums = np.zeros((nx, 10))
+    for a, f in enumerate([u, v, w, c, u * u, v * v, w * w, u * v, u * w, v * w]):
+        sums[:, a] = f.sum(axis=(0, 2, 3))
+    pr = serving.reynolds_profiles(sums.reshape(-1), nx, ny, nz, steps)
+    for i in range(nx):
+        U, V, W = u[:, i].ravel(), v[:, i].ravel(), w[:, i].ravel()
+        want = [U.mean(), V.mean(), W.mean(), c[:, i].mean(), U.var(), V.var(), W.var(),
+                np.mean(U * V) - U.mean() * V.mean(), np.mean(U * W) - U.mean() * W.mean(),
+                np.mean(V * W) - V.mean() * W.mean()]
+        assert np.allclose(pr[i], want, rtol=1e-10, atol=1e-12)
