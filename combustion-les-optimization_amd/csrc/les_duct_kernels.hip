@@ -59,7 +59,39 @@ struct DuctConsts {
     double L;               // eddy length scale
     double ly, lz;
     unsigned long long seed;
+    // reciprocals of run constants, computed on the host in FP64 (the value a
+    // device FP64 division gives) and rounded to FP32, so the kernels issue no
+    // uniform divisions or conversions; indices DUCT_RC_*
+    double rc[14];
+    float rcf[14];
 };
+
+enum {
+    DUCT_RC_ID = 0,     // 1/dx, 1/dy, 1/dz
+    DUCT_RC_HID = 3,    // 0.5/dx, 0.5/dy, 0.5/dz
+    DUCT_RC_ID2 = 6,    // 1/dx^2, 1/dy^2, 1/dz^2
+    DUCT_RC_IRUT = 9,   // 1/(rho_u T_u)
+    DUCT_RC_IDEN = 10,  // 1/(6 + filter centre weight)
+    DUCT_RC_ISC = 11,   // 0.5/Sc_t
+    DUCT_RC_IPR = 12,   // 0.5/Pr_t
+    DUCT_RC_INVD = 13   // 1/(2/dx^2 + 2/dy^2 + 2/dz^2) (Jacobi diagonal)
+};
+
+template <typename real>
+__device__ __forceinline__ real rcp_const(const DuctConsts& k, int i);
+template <>
+__device__ __forceinline__ float rcp_const<float>(const DuctConsts& k, int i) { return k.rcf[i]; }
+template <>
+__device__ __forceinline__ double rcp_const<double>(const DuctConsts& k, int i) { return k.rc[i]; }
+
+// per-direction entry base + d for a d known only at run time: a select, not
+// an indexed load (a dynamic index would move the constants to scratch)
+template <typename real>
+__device__ __forceinline__ real rcp_dir(const DuctConsts& k, int base, int d)
+{
+    const real a = rcp_const<real>(k, base), b = rcp_const<real>(k, base + 1), c = rcp_const<real>(k, base + 2);
+    return d == 0 ? a : (d == 1 ? b : c);
+}
 
 // Run descriptor: written by the host when a run is admitted or retired.
 struct SlotDesc {
@@ -222,19 +254,21 @@ __device__ __forceinline__ real face_left(int scheme, real v0, real v1, real v2,
     return v2 + (real)0.5 * minmod<real>(v2 - v1, v3 - v2);
 }
 
-// upwind face values of the cell from the 7-point line v[0..6] (offset -3..3)
+// upwind face values of the cell from the 7-point line v[0..6] (offset -3..3);
+// the upwind-ordered stencil is selected first and reconstructed once, so a
+// wave whose lanes disagree on the sign evaluates one reconstruction, not two
 template <typename real>
 __device__ __forceinline__ real face_plus(int scheme, const real* v, bool positive)
 {
-    return positive ? face_left<real>(scheme, v[1], v[2], v[3], v[4], v[5])
-                    : face_left<real>(scheme, v[6], v[5], v[4], v[3], v[2]);
+    return face_left<real>(scheme, positive ? v[1] : v[6], positive ? v[2] : v[5], positive ? v[3] : v[4],
+                           positive ? v[4] : v[3], positive ? v[5] : v[2]);
 }
 
 template <typename real>
 __device__ __forceinline__ real face_minus(int scheme, const real* v, bool positive)
 {
-    return positive ? face_left<real>(scheme, v[0], v[1], v[2], v[3], v[4])
-                    : face_left<real>(scheme, v[5], v[4], v[3], v[2], v[1]);
+    return face_left<real>(scheme, positive ? v[0] : v[5], positive ? v[1] : v[4], positive ? v[2] : v[3],
+                           positive ? v[3] : v[2], positive ? v[4] : v[1]);
 }
 
 
@@ -380,7 +414,7 @@ __global__ void __launch_bounds__(256) k_duct_filter(Pool P, int qin_is_B, int x
         if (tp.in) {
             const int i = tp.i, j = tp.j, kk = tp.kk;
             const int q = (int)lidx(g, i, j, kk);
-            const real rut = (real)k.rho_u_Tu, irut = (real)(1.0 / k.rho_u_Tu);
+            const real rut = (real)k.rho_u_Tu, irut = rcp_const<real>(k, DUCT_RC_IRUT);
             real fu, fv, fw, fT, fp;
             if (i == 0 || i == g.nx - 1) {
                 real T = q4[q], ri = T * irut;          // 1/rho
@@ -402,7 +436,7 @@ __global__ void __launch_bounds__(256) k_duct_filter(Pool P, int qin_is_B, int x
                     pu[u] = q0[r[u]] * ri; pv[u] = q1[r[u]] * ri; pw[u] = q2[r[u]] * ri;
                     pT[u] = T; pp[u] = q3[r[u]];
                 }
-                const real wc = (real)k.wc, iden = (real)(1.0 / (6.0 + k.wc));
+                const real wc = (real)k.wc, iden = rcp_const<real>(k, DUCT_RC_IDEN);
                 fu = (pu[0] + pu[1] + pu[2] + wc * pu[3] + pu[4] + pu[5] + pu[6]) * iden;
                 fv = (pv[0] + pv[1] + pv[2] + wc * pv[3] + pv[4] + pv[5] + pv[6]) * iden;
                 fw = (pw[0] + pw[1] + pw[2] + wc * pw[3] + pw[4] + pw[5] + pw[6]) * iden;
@@ -480,7 +514,7 @@ __global__ void __launch_bounds__(256) k_duct_sgs(Pool P, int stage, int xcd_map
                 const int xp = q + (int)g.sx, xm = q - (int)g.sx;
                 const int yp = (int)lidx(g, i, wrapi(j + 1, g.ny), kk), ym = (int)lidx(g, i, wrapi(j - 1, g.ny), kk);
                 const int zp = (int)lidx(g, i, j, wrapi(kk + 1, g.nz)), zm = (int)lidx(g, i, j, wrapi(kk - 1, g.nz));
-                const real hx = (real)(0.5 / g.dx), hy = (real)(0.5 / g.dy), hz = (real)(0.5 / g.dz);
+                const real hx = rcp_const<real>(D.k, DUCT_RC_HID), hy = rcp_const<real>(D.k, DUCT_RC_HID + 1), hz = rcp_const<real>(D.k, DUCT_RC_HID + 2);
                 real du_dx = (uf[xp] - uf[xm]) * hx;
                 real dv_dy = (vf[yp] - vf[ym]) * hy;
                 real dw_dz = (wf[zp] - wf[zm]) * hz;
@@ -587,12 +621,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const real* Fs = S2[s];
-            const real isc2 = (real)(0.5 / ((s == 0) ? k.sc_t : k.pr_t));
+            const real isc2 = rcp_const<real>(k, (s == 0) ? DUCT_RC_ISC : DUCT_RC_IPR);
             real acc = (real)0, sacc = (real)0, lacc = (real)0;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const double dd64 = d == 0 ? g.dx : (d == 1 ? g.dy : g.dz);
-                const real id = (real)(1.0 / dd64), id2 = (real)(1.0 / (dd64 * dd64)), hid = (real)(0.5 / dd64);
+                const real id = rcp_const<real>(k, DUCT_RC_ID + d), id2 = rcp_const<real>(k, DUCT_RC_ID2 + d);
+                const real hid = rcp_const<real>(k, DUCT_RC_HID + d);
                 const int qp = d == 0 ? xp : (d == 1 ? yp : zp);
                 const int qm = d == 0 ? xm : (d == 1 ? ym : zm);
                 real ua = vel[d][q];
@@ -700,7 +734,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
                 real conv[3] = {(real)0, (real)0, (real)0};
 #pragma unroll 1
                 for (int d = 0; d < 3; ++d) {
-                    const real id = (real)(1.0 / (d == 0 ? g.dx : (d == 1 ? g.dy : g.dz)));
+                    const real id = rcp_dir<real>(k, DUCT_RC_ID, d);
                     const int qp = line_idx(g, bi, j, kk, d, 1), qm = line_idx(g, bi, j, kk, d, -1);
                     const real* U = vel[d];
                     real up = (real)0.5 * (U[qp] + U[q]);
@@ -734,7 +768,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
                 const int xp = q + (int)g.sx, xm = q - (int)g.sx;
                 const int yp = line_idx(g, bi, j, kk, 1, 1), ym = line_idx(g, bi, j, kk, 1, -1);
                 const int zp = line_idx(g, bi, j, kk, 2, 1), zm = line_idx(g, bi, j, kk, 2, -1);
-                const real ix2 = (real)(1.0 / (g.dx * g.dx)), iy2 = (real)(1.0 / (g.dy * g.dy)), iz2 = (real)(1.0 / (g.dz * g.dz));
+                const real ix2 = rcp_const<real>(k, DUCT_RC_ID2), iy2 = rcp_const<real>(k, DUCT_RC_ID2 + 1), iz2 = rcp_const<real>(k, DUCT_RC_ID2 + 2);
                 real diff[3];
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
@@ -751,9 +785,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
                 real fx = (real)0, fy = (real)0, fz = (real)0;
                 const real wdt = (real)k.accel_width, ctr = (real)k.accel_center, Fm = (real)k.max_force;
                 if (g0 > (real)0.01 && g0 < (real)0.99) {
-                    real gx = (progress<real>(pf[xp], DX) - progress<real>(pf[xm], DX)) * (real)(0.5 / g.dx);
-                    real gy = (progress<real>(pf[yp], DX) - progress<real>(pf[ym], DX)) * (real)(0.5 / g.dy);
-                    real gz = (progress<real>(pf[zp], DX) - progress<real>(pf[zm], DX)) * (real)(0.5 / g.dz);
+                    real gx = (progress<real>(pf[xp], DX) - progress<real>(pf[xm], DX)) * rcp_const<real>(k, DUCT_RC_HID);
+                    real gy = (progress<real>(pf[yp], DX) - progress<real>(pf[ym], DX)) * rcp_const<real>(k, DUCT_RC_HID + 1);
+                    real gz = (progress<real>(pf[zp], DX) - progress<real>(pf[zm], DX)) * rcp_const<real>(k, DUCT_RC_HID + 2);
                     real df = g0 - ctr;
                     real win = M<real>::exp(M<real>::div(-(df * df), (real)2.0 * wdt * wdt));
                     real common = (real)k.accel_strength * rf0 * win;
@@ -1112,9 +1146,9 @@ __device__ __forceinline__ real proj_phi(const real* phi, const GridDims& g, int
 }
 
 template <typename real>
-__device__ __forceinline__ real proj_lap(const real* phi, const GridDims& g, int i, int j, int kk)
+__device__ __forceinline__ real proj_lap(const real* phi, const GridDims& g, const DuctConsts& k, int i, int j, int kk)
 {
-    const real ix2 = (real)(1.0 / (g.dx * g.dx)), iy2 = (real)(1.0 / (g.dy * g.dy)), iz2 = (real)(1.0 / (g.dz * g.dz));
+    const real ix2 = rcp_const<real>(k, DUCT_RC_ID2), iy2 = rcp_const<real>(k, DUCT_RC_ID2 + 1), iz2 = rcp_const<real>(k, DUCT_RC_ID2 + 2);
     const real p0 = phi[i * (int)g.sx + j * g.nz + kk];
     real lx = (proj_phi(phi, g, i + 1, j, kk) - (real)2.0 * p0 + proj_phi(phi, g, i - 1, j, kk)) * ix2;
     real ly = (proj_phi(phi, g, i, j + 1, kk) - (real)2.0 * p0 + proj_phi(phi, g, i, j - 1, kk)) * iy2;
@@ -1139,7 +1173,7 @@ __global__ void __launch_bounds__(256) k_duct_proj_div(Pool P, int xcd_map)
         const real* m0 = (const real*)P.A[0] + D.off;
         const real* m1 = (const real*)P.A[1] + D.off;
         const real* m2 = (const real*)P.A[2] + D.off;
-        const real hx = (real)(0.5 / g.dx), hy = (real)(0.5 / g.dy), hz = (real)(0.5 / g.dz);
+        const real hx = rcp_const<real>(D.k, DUCT_RC_HID), hy = rcp_const<real>(D.k, DUCT_RC_HID + 1), hz = rcp_const<real>(D.k, DUCT_RC_HID + 2);
         real v = (m0[q + (int)g.sx] - m0[q - (int)g.sx]) * hx +
                  (m1[i * (int)g.sx + wrapi(j + 1, g.ny) * g.nz + kk] - m1[i * (int)g.sx + wrapi(j - 1, g.ny) * g.nz + kk]) * hy +
                  (m2[i * (int)g.sx + j * g.nz + wrapi(kk + 1, g.nz)] - m2[i * (int)g.sx + j * g.nz + wrapi(kk - 1, g.nz)]) * hz;
@@ -1167,9 +1201,9 @@ __global__ void __launch_bounds__(256) k_duct_proj_sweep(Pool P, int it, int xcd
     const real* po = (const real*)P.B[1 + (it & 1)] + D.off;
     real* pn = (real*)P.B[1 + ((it + 1) & 1)] + D.off;
     const real b = ((const real*)P.B[0] + D.off)[q];
-    const real inv_d = (real)(1.0 / (2.0 / (g.dx * g.dx) + 2.0 / (g.dy * g.dy) + 2.0 / (g.dz * g.dz)));
+    const real inv_d = rcp_const<real>(D.k, DUCT_RC_INVD);
     if (it == 0) pn[q] = ((real)0.0 - b) * inv_d;           // phi = 0 before the first sweep
-    else pn[q] = po[q] + (proj_lap<real>(po, g, i, tp.j, tp.kk) - b) * inv_d;
+    else pn[q] = po[q] + (proj_lap<real>(po, g, D.k, i, tp.j, tp.kk) - b) * inv_d;
 }
 
 template <typename real, int TK>
@@ -1188,9 +1222,9 @@ __global__ void __launch_bounds__(256) k_duct_proj_correct(Pool P, int xcd_map)
         const int q = i * (int)g.sx + j * g.nz + kk;
         const real* phi = (const real*)P.B[1 + (D.k.proj_iters & 1)] + D.off;
         const real b = ((const real*)P.B[0] + D.off)[q];
-        const real r = b - proj_lap<real>(phi, g, i, j, kk);
+        const real r = b - proj_lap<real>(phi, g, D.k, i, j, kk);
         sr = (double)r * (double)r;
-        const real hx = (real)(0.5 / g.dx), hy = (real)(0.5 / g.dy), hz = (real)(0.5 / g.dz);
+        const real hx = rcp_const<real>(D.k, DUCT_RC_HID), hy = rcp_const<real>(D.k, DUCT_RC_HID + 1), hz = rcp_const<real>(D.k, DUCT_RC_HID + 2);
         real* m0 = (real*)P.A[0] + D.off;
         real* m1 = (real*)P.A[1] + D.off;
         real* m2 = (real*)P.A[2] + D.off;
@@ -1375,7 +1409,7 @@ __global__ void __launch_bounds__(256) k_duct_probe(Pool P)
     const real* uf = (const real*)P.F[0] + D.off;
     const real* vf = (const real*)P.F[1] + D.off;
     const real* wf = (const real*)P.F[2] + D.off;
-    const real hx = (real)(0.5 / g.dx), hy = (real)(0.5 / g.dy), hz = (real)(0.5 / g.dz);
+    const real hx = rcp_const<real>(D.k, DUCT_RC_HID), hy = rcp_const<real>(D.k, DUCT_RC_HID + 1), hz = rcp_const<real>(D.k, DUCT_RC_HID + 2);
     for (int j = threadIdx.x; j < g.ny; j += blockDim.x) {
         const int q = (int)lidx(g, ip, j, kp);
         const real rho = (real)D.k.rho_u_Tu / ((const real*)P.A[4] + D.off)[q];
@@ -1587,6 +1621,17 @@ struct DuctState {
     int use_graph = 1;
 };
 
+static void grid_dims(const les_params& p, GridDims& g)
+{
+    g.nx = p.nx; g.ny = p.ny; g.nz = p.nz;
+    g.n = (long long)p.nx * p.ny * p.nz;
+    g.sx = (long long)p.ny * p.nz;
+    g.sy = p.nz;
+    g.dx = p.lx / (double)p.nx;
+    g.dy = p.ly / (double)p.ny;
+    g.dz = p.lz / (double)p.nz;
+}
+
 static DuctConsts make_duct_consts(const les_params& p)
 {
     DuctConsts k;
@@ -1613,6 +1658,20 @@ static DuctConsts make_duct_consts(const les_params& p)
     k.dx4 = pow(p.lx / (double)p.nx, 4.0);
     k.ly = p.ly; k.lz = p.lz;
     k.seed = (unsigned long long)p.seed;
+    GridDims g;
+    grid_dims(p, g);
+    const double dd[3] = {g.dx, g.dy, g.dz};
+    for (int d = 0; d < 3; ++d) {
+        k.rc[DUCT_RC_ID + d] = 1.0 / dd[d];
+        k.rc[DUCT_RC_HID + d] = 0.5 / dd[d];
+        k.rc[DUCT_RC_ID2 + d] = 1.0 / (dd[d] * dd[d]);
+    }
+    k.rc[DUCT_RC_IRUT] = 1.0 / k.rho_u_Tu;
+    k.rc[DUCT_RC_IDEN] = 1.0 / (6.0 + k.wc);
+    k.rc[DUCT_RC_ISC] = 0.5 / k.sc_t;
+    k.rc[DUCT_RC_IPR] = 0.5 / k.pr_t;
+    k.rc[DUCT_RC_INVD] = 1.0 / (2.0 / (g.dx * g.dx) + 2.0 / (g.dy * g.dy) + 2.0 / (g.dz * g.dz));
+    for (int r = 0; r < 14; ++r) k.rcf[r] = (float)k.rc[r];
     return k;
 }
 
@@ -1623,17 +1682,6 @@ static void* dalloc_bytes(les_gpu* c, size_t bytes)
     c->allocs.push_back(p);
     (void)hipMemset(p, 0, bytes + 256);
     return p;
-}
-
-static void grid_dims(const les_params& p, GridDims& g)
-{
-    g.nx = p.nx; g.ny = p.ny; g.nz = p.nz;
-    g.n = (long long)p.nx * p.ny * p.nz;
-    g.sx = (long long)p.ny * p.nz;
-    g.sy = p.nz;
-    g.dx = p.lx / (double)p.nx;
-    g.dy = p.ly / (double)p.ny;
-    g.dz = p.lz / (double)p.nz;
 }
 
 // Allocates the pool of one context.  cfg limits come from the batch config
