@@ -942,22 +942,23 @@ __global__ void k_duct_eddy_update(Pool P)
     }
 }
 
-// One thread per inflow cell and eddy chunk: block (k-chunk of 64, j, run x
-// chunk) sums the Gaussian contributions of one chunk of SEM_CHUNK eddies
+// One thread per inflow cell and eddy chunk: block (k-chunk of 64 x eddy
+// chunk, j, run) sums the Gaussian contributions of one chunk of SEM_CHUNK eddies
 // (staged through LDS) into an FP64 partial plane; k_duct_sem_norm adds the
 // chunks in order.
-#define SEM_CHUNK 256
+#define SEM_CHUNK 64
 __global__ void __launch_bounds__(64) k_duct_sem_plane(Pool P)
 {
     __shared__ double se[SEM_CHUNK * 7];
-    const int s = blockIdx.z / P.sem_split, sp = blockIdx.z % P.sem_split;
+    const int nkc = (P.nz_max + 63) / 64;
+    const int s = blockIdx.z, sp = (int)blockIdx.x / nkc;
     const SlotDesc& D = P.desc[s];
     if (!D.active || P.dyn[s].done) return;
     const GridDims& g = D.g;
     const DuctConsts& k = D.k;
     const int j = blockIdx.y;
     if (j >= g.ny) return;
-    const int kk = blockIdx.x * 64 + threadIdx.x;
+    const int kk = ((int)blockIdx.x - sp * nkc) * 64 + threadIdx.x;
     const long long PLM = (long long)P.ny_max * P.nz_max;
     double* out = P.sem_part + ((long long)s * P.sem_split + sp) * 3 * PLM;
     const int e0 = sp * SEM_CHUNK;
@@ -1619,6 +1620,11 @@ struct DuctState {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   // [0] plain step, [1] with reinitialisation
     bool graph_ok[2] = {false, false};
     int use_graph = 1;
+    // side stream for the launches that run beside the main chain (inflow
+    // turbulence beside the RK stages, point/probe output beside the
+    // statistics), forked from and joined back into the main stream
+    hipStream_t aux = nullptr;
+    hipEvent_t evf[2] = {nullptr, nullptr}, evj[2] = {nullptr, nullptr};
 };
 
 static void grid_dims(const les_params& p, GridDims& g)
@@ -1691,6 +1697,11 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
     DuctState* d = new (std::nothrow) DuctState();
     if (!d) return LES_ERR_HIP;
     c->d = d;
+    if (hipStreamCreateWithFlags(&d->aux, hipStreamNonBlocking) != hipSuccess) return LES_ERR_HIP;
+    for (int a = 0; a < 2; ++a)
+        if (hipEventCreateWithFlags(&d->evf[a], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&d->evj[a], hipEventDisableTiming) != hipSuccess)
+            return LES_ERR_HIP;
     d->precision = cfg.precision;
     d->esize = (cfg.precision == LES_PREC_FP32) ? 4 : 8;
     d->max_slots = cfg.max_slots;
@@ -1926,19 +1937,44 @@ static int stage_launch(les_gpu* c, int stage, real* const* Lout)
 }
 
 // One engine step of every active, unfinished slot; all launches on c->stream.
+// Forks the side stream off the main stream (f) or joins it back (j); in the
+// per-kernel timing mode everything stays on the main stream.
+static int side_fork(les_gpu* c, int f)
+{
+    DuctState* d = c->d;
+    if (c->timing) return LES_OK;
+    DHIPC(hipEventRecord(d->evf[f], c->stream));
+    DHIPC(hipStreamWaitEvent(d->aux, d->evf[f], 0));
+    return LES_OK;
+}
+
+static int side_join(les_gpu* c, int f)
+{
+    DuctState* d = c->d;
+    if (c->timing) return LES_OK;
+    DHIPC(hipEventRecord(d->evj[f], d->aux));
+    DHIPC(hipStreamWaitEvent(c->stream, d->evj[f], 0));
+    return LES_OK;
+}
+
 template <typename real, int TK>
 static int step_launches(les_gpu* c, bool reinit)
 {
     DuctState* d = c->d;
     const int S = d->max_slots;
+    hipStream_t side = c->timing ? c->stream : d->aux;
     hipLaunchKernelGGL(k_duct_dt, dim3(S), dim3(1024), 0, c->stream, d->P);
-    hipLaunchKernelGGL(k_duct_eddy_update, dim3((d->nvort_max + 255) / 256, S), dim3(256), 0, c->stream, d->P);
-    hipLaunchKernelGGL(k_duct_sem_plane, dim3((d->nz_max + 63) / 64, d->ny_max, S * d->P.sem_split), dim3(64), 0,
-                       c->stream, d->P);
-    hipLaunchKernelGGL(k_duct_sem_norm, dim3(3, S), dim3(1024), 0, c->stream, d->P);
+    // the inflow turbulence (eddies -> SEM plane) is read only by the boundary
+    // update after the stages, so it runs beside them
+    int e = side_fork(c, 0);
+    if (e != LES_OK) return e;
+    hipLaunchKernelGGL(k_duct_eddy_update, dim3((d->nvort_max + 255) / 256, S), dim3(256), 0, side, d->P);
+    hipLaunchKernelGGL(k_duct_sem_plane, dim3((d->nz_max + 63) / 64 * d->P.sem_split, d->ny_max, S), dim3(64), 0,
+                       side, d->P);
+    hipLaunchKernelGGL(k_duct_sem_norm, dim3(3, S), dim3(1024), 0, side, d->P);
     DLAUNCH_CHECK();
     for (int s = 0; s < 3; ++s) {
-        int e = stage_launch<real, TK>(c, s, nullptr);
+        e = stage_launch<real, TK>(c, s, nullptr);
         if (e != LES_OK) return e;
     }
     if (d->max_proj_iters > 0) {
@@ -1950,6 +1986,7 @@ static int step_launches(les_gpu* c, bool reinit)
         hipLaunchKernelGGL(k_duct_proj_final, dim3(S), dim3(1024), 0, c->stream, d->P);
         DLAUNCH_CHECK();
     }
+    if ((e = side_join(c, 0)) != LES_OK) return e;
     hipLaunchKernelGGL((k_duct_bc<real>), dim3(d->nzb, d->ny_max, S), dim3(256), 0, c->stream, d->P);
     DLAUNCH_CHECK();
     if (reinit) {
@@ -1962,11 +1999,14 @@ static int step_launches(les_gpu* c, bool reinit)
         }
         DLAUNCH_CHECK();
     }
+    // outputs of the new state: point samples and probe rows beside the statistics
+    if ((e = side_fork(c, 1)) != LES_OK) return e;
+    hipLaunchKernelGGL((k_duct_points<real>), dim3(S), dim3(64), 0, side, d->P);
+    if (d->probe_cap > 0)
+        hipLaunchKernelGGL((k_duct_probe<real>), dim3(LES_NPROBE_MAX, S), dim3(256), 0, side, d->P);
     hipLaunchKernelGGL((k_duct_stats<real>), dim3(STAT_CHUNKS, d->nx_max, S), dim3(256), 0, c->stream, d->P);
     hipLaunchKernelGGL(k_duct_stats_final, dim3((d->nx_max * 10 + 255) / 256, S), dim3(256), 0, c->stream, d->P);
-    hipLaunchKernelGGL((k_duct_points<real>), dim3(S), dim3(64), 0, c->stream, d->P);
-    if (d->probe_cap > 0)
-        hipLaunchKernelGGL((k_duct_probe<real>), dim3(LES_NPROBE_MAX, S), dim3(256), 0, c->stream, d->P);
+    if ((e = side_join(c, 1)) != LES_OK) return e;
     hipLaunchKernelGGL(k_duct_step_end, dim3((S + 255) / 256), dim3(256), 0, c->stream, d->P, S);
     DLAUNCH_CHECK();
     return LES_OK;
@@ -2380,6 +2420,14 @@ void duct_destroy(les_gpu* c)
     if (!c->d) return;
     for (auto& g : c->d->gexec)
         if (g) (void)hipGraphExecDestroy(g);
+    if (c->d->aux) {
+        (void)hipStreamSynchronize(c->d->aux);
+        (void)hipStreamDestroy(c->d->aux);
+    }
+    for (int a = 0; a < 2; ++a) {
+        if (c->d->evf[a]) (void)hipEventDestroy(c->d->evf[a]);
+        if (c->d->evj[a]) (void)hipEventDestroy(c->d->evj[a]);
+    }
     delete c->d;
     c->d = nullptr;
 }
