@@ -693,9 +693,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 // momentum RHS (:2552-2691 structure) + SSP-RK3 combine; in the last stage also
 // the per-tile max |u| of the new state (its T was written by the scalar kernel)
 template <typename real, int TK>
-// FP64: at least 5 waves per SIMD (96 VGPRs, a 36-byte spill) measured faster
-// than 4 waves without the spill; FP32 needs fewer registers than the cap
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
+// FP64: at least 4 waves per SIMD (no spill; a 5-wave cap spills about 50
+// bytes per lane, whose scratch traffic reaches HBM, and measured 3% slower);
+// FP32: at least 5 waves
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 4 : 5)))
+k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
 {
     __shared__ double vred[8];
     const int t = tile_of_block(P, a.xcd_map);
