@@ -33,7 +33,8 @@
 
 #define DNF 5
 #define DUCT_NPTS 64
-#define STAT_CHUNKS 16
+#define STAT_CHUNKS 16            // most chunks of one x-plane in the statistics pass
+#define STAT_CHUNK_CELLS 2048     // target cells per chunk (8 per thread)
 
 // ---------------------------------------------------------------------------
 // device-side parameters of the duct scheme (one set per slot)
@@ -1283,10 +1284,18 @@ __device__ __forceinline__ bool stats_due(const SlotDesc& D, const SlotDyn& y)
     return D.k.stats_start >= 0 && y.step >= D.k.stats_start;
 }
 
+// Chunks of one x-plane of a run in the statistics pass: a function of the
+// run's own plane size only, so its sums do not depend on the pool.
+__device__ __forceinline__ int stat_chunks(long long PL)
+{
+    const long long c = (PL + STAT_CHUNK_CELLS - 1) / STAT_CHUNK_CELLS;
+    return c < 1 ? 1 : (c > STAT_CHUNKS ? STAT_CHUNKS : (int)c);
+}
+
 template <typename real>
 __global__ void __launch_bounds__(256) k_duct_stats(Pool P)
 {
-    __shared__ double red[10][256];
+    __shared__ double red[4][10];
     const int s = blockIdx.z;
     const SlotDesc& D = P.desc[s];
     if (!D.active || P.dyn[s].done || !stats_due(D, P.dyn[s])) return;
@@ -1295,7 +1304,9 @@ __global__ void __launch_bounds__(256) k_duct_stats(Pool P)
     if (i >= g.nx) return;
     const DuctConsts& k = D.k;
     const long long PL = (long long)g.ny * g.nz;
-    const long long chunk = (PL + gridDim.x - 1) / gridDim.x;
+    const int nch = stat_chunks(PL);
+    if ((int)blockIdx.x >= nch) return;
+    const long long chunk = (PL + nch - 1) / nch;
     const long long p0 = (long long)blockIdx.x * chunk;
     const long long p1 = p0 + chunk < PL ? p0 + chunk : PL;
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -1321,15 +1332,19 @@ __global__ void __launch_bounds__(256) k_duct_stats(Pool P)
         acc[0] += u; acc[1] += v; acc[2] += w; acc[3] += c;
         acc[4] += u * u; acc[5] += v * v; acc[6] += w * w; acc[7] += u * v; acc[8] += u * w; acc[9] += v * w;
     }
-    for (int a = 0; a < 10; ++a) red[a][threadIdx.x] = acc[a];
+    // fixed-order reduction: butterfly within each wave, then the 4 waves in order
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int a = 0; a < 10; ++a) acc[a] += __shfl_xor(acc[a], o);
+    const int wv = (int)threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int a = 0; a < 10; ++a) red[wv][a] = acc[a];
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            for (int a = 0; a < 10; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + w];
-        __syncthreads();
-    }
     if (threadIdx.x < 10)
-        P.spart[(((long long)s * P.nx_max + i) * STAT_CHUNKS + blockIdx.x) * 10 + threadIdx.x] = red[threadIdx.x][0];
+        P.spart[(((long long)s * P.nx_max + i) * STAT_CHUNKS + blockIdx.x) * 10 + threadIdx.x] =
+            ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
 }
 
 __global__ void k_duct_stats_final(Pool P)
@@ -1341,7 +1356,8 @@ __global__ void k_duct_stats_final(Pool P)
     if (t >= D.g.nx * 10) return;
     int i = t / 10, a = t % 10;
     double sm = 0.0;
-    for (int c = 0; c < STAT_CHUNKS; ++c) sm += P.spart[(((long long)s * P.nx_max + i) * STAT_CHUNKS + c) * 10 + a];
+    const int nch = stat_chunks((long long)D.g.ny * D.g.nz);
+    for (int c = 0; c < nch; ++c) sm += P.spart[(((long long)s * P.nx_max + i) * STAT_CHUNKS + c) * 10 + a];
     P.prof[D.xoff + t] += sm;
 }
 
@@ -2006,7 +2022,11 @@ static int step_launches(les_gpu* c, bool reinit)
     hipLaunchKernelGGL((k_duct_points<real>), dim3(S), dim3(64), 0, side, d->P);
     if (d->probe_cap > 0)
         hipLaunchKernelGGL((k_duct_probe<real>), dim3(LES_NPROBE_MAX, S), dim3(256), 0, side, d->P);
-    hipLaunchKernelGGL((k_duct_stats<real>), dim3(STAT_CHUNKS, d->nx_max, S), dim3(256), 0, c->stream, d->P);
+    {
+        const long long plm = (long long)d->ny_max * d->nz_max;
+        const int sch = (int)std::min<long long>(STAT_CHUNKS, std::max<long long>(1, (plm + STAT_CHUNK_CELLS - 1) / STAT_CHUNK_CELLS));
+        hipLaunchKernelGGL((k_duct_stats<real>), dim3(sch, d->nx_max, S), dim3(256), 0, c->stream, d->P);
+    }
     hipLaunchKernelGGL(k_duct_stats_final, dim3((d->nx_max * 10 + 255) / 256, S), dim3(256), 0, c->stream, d->P);
     if ((e = side_join(c, 1)) != LES_OK) return e;
     hipLaunchKernelGGL(k_duct_step_end, dim3((S + 255) / 256), dim3(256), 0, c->stream, d->P, S);
