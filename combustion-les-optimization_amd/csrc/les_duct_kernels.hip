@@ -39,6 +39,8 @@
 // ---------------------------------------------------------------------------
 // device-side parameters of the duct scheme (one set per slot)
 // ---------------------------------------------------------------------------
+#define DUCT_NRC 19          // host-computed reciprocal constants per run
+
 struct DuctConsts {
     double rho_u_Tu;        // rho_unburnt * T_u  (rho = rho_u_Tu / T)
     double Tu, Tb, U, intensity;
@@ -63,8 +65,8 @@ struct DuctConsts {
     // reciprocals of run constants, computed on the host in FP64 (the value a
     // device FP64 division gives) and rounded to FP32, so the kernels issue no
     // uniform divisions or conversions; indices DUCT_RC_*
-    double rc[14];
-    float rcf[14];
+    double rc[DUCT_NRC];
+    float rcf[DUCT_NRC];
 };
 
 enum {
@@ -75,7 +77,12 @@ enum {
     DUCT_RC_IDEN = 10,  // 1/(6 + filter centre weight)
     DUCT_RC_ISC = 11,   // 0.5/Sc_t
     DUCT_RC_IPR = 12,   // 0.5/Pr_t
-    DUCT_RC_INVD = 13   // 1/(2/dx^2 + 2/dy^2 + 2/dz^2) (Jacobi diagonal)
+    DUCT_RC_INVD = 13,  // 1/(2/dx^2 + 2/dy^2 + 2/dz^2) (Jacobi diagonal)
+    DUCT_RC_ITDT = 14,  // 1/(property table step)
+    DUCT_RC_IPRAN = 15, // 1/Pr
+    DUCT_RC_IW2 = 16,   // 1/(2 w^2), w the flame-acceleration window width
+    DUCT_RC_DF = 17,    // nu_u/(S_L + small): flame thickness of the SSR model
+    DUCT_RC_ITSL = 18   // 1/(tau S_L) (0 when tau S_L <= small)
 };
 
 template <typename real>
@@ -203,16 +210,17 @@ struct M<float> {
     }
 };
 
+// progress variable c = (1 + tanh(phi / (2 dx))) / 2; i2dx = 0.5/dx of the run
 template <typename real>
-__device__ __forceinline__ real progress(real phi, real dx)
+__device__ __forceinline__ real progress(real phi, real i2dx)
 {
-    return (real)0.5 * ((real)1.0 + M<real>::tanh(M<real>::div(phi, (real)2.0 * dx)));
+    return (real)0.5 * ((real)1.0 + M<real>::tanh(phi * i2dx));
 }
 
 template <typename real>
 __device__ __forceinline__ real table_lookup(const double* tab, const DuctConsts& k, real T)
 {
-    real s = M<real>::div(T - (real)k.table_t0, (real)k.table_dt);
+    real s = (T - (real)k.table_t0) * rcp_const<real>(k, DUCT_RC_ITDT);
     if (s < (real)0.0) s = (real)0.0;
     if (s > (real)(k.table_n - 1)) s = (real)(k.table_n - 1);
     int i0 = (int)s;
@@ -517,7 +525,7 @@ __global__ void __launch_bounds__(256) k_duct_sgs(Pool P, int stage, int xcd_map
                 real mu = table_lookup<real>(P.mu_tab, k, T);
                 real cp = table_lookup<real>(P.cp_tab, k, T);
                 real nu = M<real>::div(mu, rf);
-                real lam = M<real>::div(mu * cp, (real)k.prandtl);
+                real lam = mu * cp * rcp_const<real>(k, DUCT_RC_IPRAN);
                 real al = M<real>::div(lam, rf * cp);
                 mn = (double)nu; ma = (double)al;
                 const int xp = q + (int)g.sx, xm = q - (int)g.sx;
@@ -537,7 +545,7 @@ __global__ void __launch_bounds__(256) k_duct_sgs(Pool P, int stage, int xcd_map
                 real ss = (real)2.0 * (S12 * S12 + S13 * S13 + S23 * S23) + du_dx * du_dx + dv_dy * dv_dy + dw_dz * dw_dz;
                 real smag = sqrt((real)2.0 * ss);
                 real div = du_dx + dv_dy + dw_dz;
-                real gg = progress<real>(pf[q], DX);
+                real gg = progress<real>(pf[q], rcp_const<real>(D.k, DUCT_RC_HID));
                 real cs = (real)k.cs_u + ((real)k.cs_b - (real)k.cs_u) * gg;
                 cs += (real)k.cs_f * (real)4.0 * gg * ((real)1.0 - gg);
                 nut[q] = (cs * DX) * (cs * DX) * smag;
@@ -552,10 +560,10 @@ __global__ void __launch_bounds__(256) k_duct_sgs(Pool P, int stage, int xcd_map
                     d_eta = (real)0.2340 * px * (alg > (real)1.0 ? M<real>::log(alg) : (real)0.0);
                 }
                 real eta = M<real>::div(DX, d_eta + (real)LES_SMALL);
-                real dF = M<real>::div((real)k.nu_unburnt, SL + (real)LES_SMALL);
+                const real dF = rcp_const<real>(k, DUCT_RC_DF);
                 real asf = (real)8.0 * M<real>::exp(M<real>::div((real)10.0 * dF, (real)8.0 * eta + (real)LES_SMALL));
-                real wr = M<real>::pow(M<real>::div(asf * eta, DX), (real)2.0 - (real)k.d3);
-                real te = ((real)k.tau * SL > (real)LES_SMALL) ? M<real>::div(DX * fmax((real)0.0, div), (real)k.tau * SL) : (real)0.0;
+                real wr = M<real>::pow(asf * eta * rcp_const<real>(k, DUCT_RC_ID), (real)2.0 - (real)k.d3);
+                real te = DX * fmax((real)0.0, div) * rcp_const<real>(k, DUCT_RC_ITSL);
                 stv[q] = fmax(SL, SL * fmax((real)0.0, wr + te));
             }
         }
@@ -615,7 +623,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         const real* Tf = (const real*)P.F[3] + D.off;
         const real* pf = (const real*)P.F[4] + D.off;
         const real* nut = (const real*)P.nut + D.off;
-        const real DX = (real)g.dx;
         const real rut = (real)k.rho_u_Tu;
         const bool xband = (i < 3 || i > g.nx - 4);
         const int scheme = k.scheme;
@@ -663,13 +670,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             }
             cv[s] = acc; sg[s] = sacc; lap[s] = lacc;
         }
-        const real g0 = progress<real>(pf[q], DX);
-        const real wdt = (real)k.accel_width, ctr = (real)k.accel_center;
+        const real g0 = progress<real>(pf[q], rcp_const<real>(D.k, DUCT_RC_HID));
+        const real ctr = (real)k.accel_center;
         real gmag = sqrt(grad[0] * grad[0] + grad[1] * grad[1] + grad[2] * grad[2]);
         real art = (real)0;
         if (g0 > (real)0.1 && g0 < (real)0.99) {
             real df = g0 - ctr;
-            real win = M<real>::exp(M<real>::div(-(df * df), (real)2.0 * wdt * wdt));
+            real win = M<real>::exp(-(df * df) * rcp_const<real>(k, DUCT_RC_IW2));
             art = (real)5.0 * (nu0 + (real)1.0e-5) * win * lap[0];
         }
         const real* stv = (const real*)P.st + D.off;
@@ -677,7 +684,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         const real rf0 = M<real>::div(rut, T0);
         real mu = table_lookup<real>(P.mu_tab, k, T0);
         real cp = table_lookup<real>(P.cp_tab, k, T0);
-        real lam = M<real>::div(mu * cp, (real)k.prandtl);
+        real lam = mu * cp * rcp_const<real>(k, DUCT_RC_IPRAN);
         real al = M<real>::div(lam, rf0 * cp);
         const real Tu = (real)k.Tu, Tb = (real)k.Tb;
         real Ttg = Tu + g0 * (Tb - Tu);
@@ -735,7 +742,6 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 const real* Tf = (const real*)P.F[3] + D.off;
                 const real* pf = (const real*)P.F[4] + D.off;
                 const real* nut = (const real*)P.nut + D.off;
-                const real DX = (real)g.dx;
                 const real rut = (real)k.rho_u_Tu;
                 const bool xband = (i < 3 || i > g.nx - 4);
                 const int scheme = k.scheme;
@@ -792,15 +798,15 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 }
                 const real rho_avg = (real)ldc(&P.dyn[tp.s].rho_avg);
                 real grav = (rf0 - rho_avg) * (real)LES_G_GRAV;
-                real g0 = progress<real>(pf[q], DX);
+                real g0 = progress<real>(pf[q], rcp_const<real>(D.k, DUCT_RC_HID));
                 real fx = (real)0, fy = (real)0, fz = (real)0;
-                const real wdt = (real)k.accel_width, ctr = (real)k.accel_center, Fm = (real)k.max_force;
+                const real ctr = (real)k.accel_center, Fm = (real)k.max_force;
                 if (g0 > (real)0.01 && g0 < (real)0.99) {
-                    real gx = (progress<real>(pf[xp], DX) - progress<real>(pf[xm], DX)) * rcp_const<real>(k, DUCT_RC_HID);
-                    real gy = (progress<real>(pf[yp], DX) - progress<real>(pf[ym], DX)) * rcp_const<real>(k, DUCT_RC_HID + 1);
-                    real gz = (progress<real>(pf[zp], DX) - progress<real>(pf[zm], DX)) * rcp_const<real>(k, DUCT_RC_HID + 2);
+                    real gx = (progress<real>(pf[xp], rcp_const<real>(D.k, DUCT_RC_HID)) - progress<real>(pf[xm], rcp_const<real>(D.k, DUCT_RC_HID))) * rcp_const<real>(k, DUCT_RC_HID);
+                    real gy = (progress<real>(pf[yp], rcp_const<real>(D.k, DUCT_RC_HID)) - progress<real>(pf[ym], rcp_const<real>(D.k, DUCT_RC_HID))) * rcp_const<real>(k, DUCT_RC_HID + 1);
+                    real gz = (progress<real>(pf[zp], rcp_const<real>(D.k, DUCT_RC_HID)) - progress<real>(pf[zm], rcp_const<real>(D.k, DUCT_RC_HID))) * rcp_const<real>(k, DUCT_RC_HID + 2);
                     real df = g0 - ctr;
-                    real win = M<real>::exp(M<real>::div(-(df * df), (real)2.0 * wdt * wdt));
+                    real win = M<real>::exp(-(df * df) * rcp_const<real>(k, DUCT_RC_IW2));
                     real common = (real)k.accel_strength * rf0 * win;
                     fx = common * gx * (real)k.weight_x;
                     fy = common * gy;
@@ -827,8 +833,8 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                     qo[f][q] = out[f];
                 }
                 if (a.track_vel) {
-                    real rho = M<real>::div(rut, ((const real*)QO[4] + D.off)[q]);
-                    real uu = M<real>::div(out[0], rho), vv = M<real>::div(out[1], rho), ww = M<real>::div(out[2], rho);
+                    const real ri = ((const real*)QO[4] + D.off)[q] * rcp_const<real>(k, DUCT_RC_IRUT);   // 1/rho
+                    real uu = out[0] * ri, vv = out[1] * ri, ww = out[2] * ri;
                     vmax = (double)sqrt(uu * uu + vv * vv + ww * ww);
                 }
             }
@@ -1318,7 +1324,6 @@ __global__ void __launch_bounds__(256) k_duct_stats(Pool P)
     const long long p0 = (long long)blockIdx.x * chunk;
     const long long p1 = p0 + chunk < PL ? p0 + chunk : PL;
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const real dx = (real)g.dx;
     const real rut = (real)k.rho_u_Tu;
     const real* A0 = (const real*)P.A[0] + D.off;
     const real* A1 = (const real*)P.A[1] + D.off;
@@ -1330,7 +1335,7 @@ __global__ void __launch_bounds__(256) k_duct_stats(Pool P)
         long long q = (long long)i * PL + p;
         real rho = rut / A4[q];
         double u = (double)(A0[q] / rho), v = (double)(A1[q] / rho), w = (double)(A2[q] / rho);
-        double c = (double)progress<real>(A3[q], dx);
+        double c = (double)progress<real>(A3[q], rcp_const<real>(D.k, DUCT_RC_HID));
         if (per_cell) {
             long long qa = D.off + q;
             P.stat[0][qa] += u; P.stat[1][qa] += v; P.stat[2][qa] += w; P.stat[3][qa] += c;
@@ -1385,7 +1390,7 @@ __global__ void k_duct_points(Pool P)
     o[0] = (double)(((const real*)P.A[0] + D.off)[q] / rho);
     o[1] = (double)(((const real*)P.A[1] + D.off)[q] / rho);
     o[2] = (double)(((const real*)P.A[2] + D.off)[q] / rho);
-    o[3] = (double)progress<real>(((const real*)P.A[3] + D.off)[q], (real)D.g.dx);
+    o[3] = (double)progress<real>(((const real*)P.A[3] + D.off)[q], rcp_const<real>(D.k, DUCT_RC_HID));
 }
 
 // flame-region probe lines (record_flame_region_data :3403-3471 structure):
@@ -1410,20 +1415,19 @@ __global__ void __launch_bounds__(256) k_duct_probe(Pool P)
     }
     const int kp = g.nz / 2;
     const real* A3 = (const real*)P.A[3] + D.off;
-    const real dxr = (real)g.dx;
     if (threadIdx.x == 0) jmin = 0x7fffffff;
     __syncthreads();
     for (int j = threadIdx.x; j < g.ny - 1; j += blockDim.x) {
-        double g1 = (double)progress<real>(A3[lidx(g, ip, j, kp)], dxr);
-        double g2 = (double)progress<real>(A3[lidx(g, ip, j + 1, kp)], dxr);
+        double g1 = (double)progress<real>(A3[lidx(g, ip, j, kp)], rcp_const<real>(D.k, DUCT_RC_HID));
+        double g2 = (double)progress<real>(A3[lidx(g, ip, j + 1, kp)], rcp_const<real>(D.k, DUCT_RC_HID));
         if ((g1 - 0.5) * (g2 - 0.5) <= 0.0) atomicMin(&jmin, j);
     }
     __syncthreads();
     const int jc = jmin;
     double yf = -1.0;
     if (jc != 0x7fffffff) {
-        double g1 = (double)progress<real>(A3[lidx(g, ip, jc, kp)], dxr);
-        double g2 = (double)progress<real>(A3[lidx(g, ip, jc + 1, kp)], dxr);
+        double g1 = (double)progress<real>(A3[lidx(g, ip, jc, kp)], rcp_const<real>(D.k, DUCT_RC_HID));
+        double g2 = (double)progress<real>(A3[lidx(g, ip, jc + 1, kp)], rcp_const<real>(D.k, DUCT_RC_HID));
         double y1 = jc * g.dy, y2 = (jc + 1) * g.dy;
         if (fabs(g2 - g1) < 1e-9) yf = y1;
         else yf = y1 + (0.5 - g1) / (g2 - g1) * (y2 - y1);
@@ -1444,7 +1448,7 @@ __global__ void __launch_bounds__(256) k_duct_probe(Pool P)
         r[0] = (double)(((const real*)P.A[0] + D.off)[q] / rho);
         r[1] = (double)(((const real*)P.A[1] + D.off)[q] / rho);
         r[2] = (double)(((const real*)P.A[2] + D.off)[q] / rho);
-        r[3] = (double)progress<real>(A3[q], dxr);
+        r[3] = (double)progress<real>(A3[q], rcp_const<real>(D.k, DUCT_RC_HID));
         r[4] = (double)((const real*)P.nut + D.off)[q];
         const int xp = q + (int)g.sx, xm = q - (int)g.sx;
         const int yp = (int)lidx(g, ip, wrapi(j + 1, g.ny), kp), ym = (int)lidx(g, ip, wrapi(j - 1, g.ny), kp);
@@ -1488,7 +1492,7 @@ __global__ void __launch_bounds__(256) k_duct_monitor(Pool P, int s, double* par
         double T = (double)A4[q];
         tm += D.k.rho_u_Tu / T * dV;
         mnT = fmin(mnT, T); mxT = fmax(mxT, T);
-        double c = (double)progress<real>(A3[q], (real)g.dx);
+        double c = (double)progress<real>(A3[q], rcp_const<real>(D.k, DUCT_RC_HID));
         mnG = fmin(mnG, c); mxG = fmax(mxG, c);
         if (!isfinite(T) || !isfinite((double)A0[q]) || !isfinite((double)A3[q])) bad = 1.0;
     }
@@ -1703,7 +1707,12 @@ static DuctConsts make_duct_consts(const les_params& p)
     k.rc[DUCT_RC_ISC] = 0.5 / k.sc_t;
     k.rc[DUCT_RC_IPR] = 0.5 / k.pr_t;
     k.rc[DUCT_RC_INVD] = 1.0 / (2.0 / (g.dx * g.dx) + 2.0 / (g.dy * g.dy) + 2.0 / (g.dz * g.dz));
-    for (int r = 0; r < 14; ++r) k.rcf[r] = (float)k.rc[r];
+    k.rc[DUCT_RC_ITDT] = 1.0 / k.table_dt;
+    k.rc[DUCT_RC_IPRAN] = 1.0 / k.prandtl;
+    k.rc[DUCT_RC_IW2] = 1.0 / (2.0 * k.accel_width * k.accel_width);
+    k.rc[DUCT_RC_DF] = k.nu_unburnt / (k.s_l + LES_SMALL);
+    k.rc[DUCT_RC_ITSL] = (k.tau * k.s_l > LES_SMALL) ? 1.0 / (k.tau * k.s_l) : 0.0;
+    for (int r = 0; r < DUCT_NRC; ++r) k.rcf[r] = (float)k.rc[r];
     return k;
 }
 
