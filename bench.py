@@ -65,8 +65,8 @@ RHS_KERNELS = "k_duct_rhs_scalar + k_duct_rhs_momentum (one RK stage's RHS and c
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=500, help="timed steps (the 128^3 run's 500 steps)")
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="duct128-fp64", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the FP32 and ensemble legs of the default line")
@@ -98,6 +98,7 @@ def cpu_baseline(preset, nx, ny, nz, over, stages, threads, target_s=10.0, max_s
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     if threads > 0:
         os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     import pyoracle
     p = pyoracle.params(preset)
     p.nx, p.ny, p.nz = nx, ny, nz
@@ -272,6 +273,7 @@ def served_line(a, lesgpu, serving):
         bad = sum(srv.results[r]["nonfinite"] for r in range(len(reqs)))
         tune = b.tuning()
     cell_steps = sum(p.nx * p.ny * p.nz * n for _, p, n in reqs)
+    err = served_profile_error(lesgpu, serving, probe, 50)
     return {"metric": "LES runs/hour + p50/p99 latency (96 served requests, FP32)",
             "value": len(reqs) * 3600.0 / wall, "unit": "runs/hour", "n_gpus": 1,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -281,7 +283,29 @@ def served_line(a, lesgpu, serving):
                        "pool_cells": cfg.pool_cells, "hbm_budget_bytes": budget},
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()), "wall_s": wall, "gpu_busy_s": srv.gpu_ms * 1e-3,
-            "pool_steps": srv.steps_run, "cell_steps": cell_steps, "nonfinite_runs": int(bad), "tuning": tune}
+            "pool_steps": srv.steps_run, "cell_steps": cell_steps, "nonfinite_runs": int(bad), "tuning": tune,
+            "reynolds_profile_rel_err_fp32_vs_fp64": err}
+
+
+def served_profile_error(lesgpu, serving, p, steps):
+    """The first request's run (its grid and constants) in the FP32 form vs the
+    plain FP64 path (which equals the FP64 oracle within 1e-10) after `steps`
+    steps: plane-profile errors per column."""
+    profs = {}
+    for prec in (1, 0):
+        q = type(p).from_buffer_copy(p)
+        q.precision = prec
+        g = lesgpu.LesGpu(q, lesgpu.mixture())
+        try:
+            g.init_state()
+            g.step(steps)
+            profs[prec] = serving.reynolds_profiles(g.field("profiles", 10 * q.nx), q.nx, q.ny, q.nz, steps)
+        finally:
+            g.close()
+    out = serving.profile_errors(profs[1], profs[0], p.inlet_velocity, p.turbulence_intensity)
+    out["steps"] = steps
+    out["grid"] = [p.nx, p.ny, p.nz]
+    return out
 
 
 def main():
