@@ -80,6 +80,21 @@ def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
+# MI355X VALU issue: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles
+PEAK_VALU_GWIS = 256 * 4 * 2.4 / 4
+
+
+def valu_from_profiles(workload):
+    """VALU wave-instructions per stage of the RHS pair (rocprofv3 --pmc
+    SQ_INSTS_VALU over the same build), where recorded."""
+    path = os.path.join(ROOT, "profiles", "rhs_valu.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    return rec.get(workload)
+
+
 def traffic_from_profiles(workload):
     """HBM bytes per launch of the RHS kernels as measured with rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), where recorded."""
@@ -174,6 +189,13 @@ def single_line(a, lesgpu, name):
                      "bytes_per_cell_stage": B, "cells": cells,
                      "whole_stage_achieved": cells * B / (stage_ms * 1e-3) / 1e9},
     }
+    vi = valu_from_profiles(name)
+    if vi:
+        # the pair's VALU instruction count (measured) over its live duration:
+        # the bound these kernels actually run against
+        g = vi / (k_ms * 1e-3) / 1e9
+        out["roofline"]["valu_issue"] = {"achieved": g, "peak": PEAK_VALU_GWIS, "unit": "G wave-instr/s",
+                                         "frac": g / PEAK_VALU_GWIS, "insts_per_stage": vi}
     return out, p
 
 

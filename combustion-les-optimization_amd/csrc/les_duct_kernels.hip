@@ -101,6 +101,18 @@ __device__ __forceinline__ real rcp_dir(const DuctConsts& k, int base, int d)
     return d == 0 ? a : (d == 1 ? b : c);
 }
 
+// |u| of a cell from its conservative state (1/rho = T/(rho_u T_u)).  Every
+// kernel that feeds the dt rule uses this one form, with explicit FMAs, so a
+// run resumed from a checkpoint (whose first dt comes from a separate pass)
+// takes bit for bit the dt of a run that never stopped.
+template <typename real>
+__device__ __forceinline__ double cell_speed(real m0, real m1, real m2, real T, const DuctConsts& k)
+{
+    const real ri = T * rcp_const<real>(k, DUCT_RC_IRUT);
+    const real u = m0 * ri, v = m1 * ri, w = m2 * ri;
+    return (double)sqrt(fma(w, w, fma(v, v, u * u)));
+}
+
 // Run descriptor: written by the host when a run is admitted or retired.
 struct SlotDesc {
     GridDims g;
@@ -709,10 +721,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 // momentum RHS (:2552-2691 structure) + SSP-RK3 combine; in the last stage also
 // the per-tile max |u| of the new state (its T was written by the scalar kernel)
 template <typename real, int TK>
-// FP64: at least 4 waves per SIMD (no spill; a 5-wave cap spills about 50
-// bytes per lane, whose scratch traffic reaches HBM, and measured 3% slower);
-// FP32: at least 7 waves (72 VGPRs, no spill; 3% faster than 6 waves)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 4 : 7)))
+// FP64: at least 6 waves per SIMD with the component loops rolled (80 VGPRs
+// and a 12-byte spill, measured 2% faster than 4 waves unrolled at 115 VGPRs);
+// FP32: at least 7 waves (72 VGPRs, no spill)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 6 : 7)))
 k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
 {
     __shared__ double vred[8];
@@ -748,7 +760,10 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 const real* vel[3] = {uf, vf, wf};
                 const real T0 = Tf[q];
                 const real rf0 = M<real>::div(rut, T0);
-                real conv[3] = {(real)0, (real)0, (real)0};
+                // FP64 keeps the component loops rolled (fewer live registers,
+                // 6 waves per SIMD); FP32 unrolls them
+                constexpr int MU = sizeof(real) == 8 ? 1 : 3;
+                real cv0 = (real)0, cv1 = (real)0, cv2 = (real)0;
 #pragma unroll 1
                 for (int d = 0; d < 3; ++d) {
                     const real id = rcp_dir<real>(k, DUCT_RC_ID, d);
@@ -760,9 +775,9 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                     real rm = (real)0.5 * (rf0 + M<real>::div(rut, Tf[qm]));
                     real mp = rp * up, mm = rm * um;
                     const bool band = (d == 0 && xband);
-#pragma unroll
+#pragma unroll MU
                     for (int m = 0; m < 3; ++m) {
-                        const real* V = vel[m];
+                        const real* V = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
                         real fp, fm;
                         if (band) {
                             real v2 = V[qm], v3 = V[q], v4 = V[qp];
@@ -776,9 +791,14 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                             fm = face_minus<real>(scheme, v, um >= (real)0.0);
                         }
                         real term = (mp * fp - mm * fm) * id;
-                        conv[m] = (d == 0) ? term : conv[m] + term;
+                        const real c = m == 0 ? cv0 : (m == 1 ? cv1 : cv2);
+                        const real nc = (d == 0) ? term : c + term;
+                        cv0 = m == 0 ? nc : cv0;
+                        cv1 = m == 1 ? nc : cv1;
+                        cv2 = m == 2 ? nc : cv2;
                     }
                 }
+                const real conv[3] = {cv0, cv1, cv2};
                 real mu = table_lookup<real>(P.mu_tab, k, T0);
                 real nu0 = nut[q];
                 real mu_eff = mu + rf0 * nu0;
@@ -787,24 +807,36 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 const int zp = line_idx(g, bi, j, kk, 2, 1), zm = line_idx(g, bi, j, kk, 2, -1);
                 const real ix2 = rcp_const<real>(k, DUCT_RC_ID2), iy2 = rcp_const<real>(k, DUCT_RC_ID2 + 1), iz2 = rcp_const<real>(k, DUCT_RC_ID2 + 2);
                 real diff[3];
-#pragma unroll
+                real df0 = (real)0, df1 = (real)0, df2 = (real)0;
+#pragma unroll MU
                 for (int m = 0; m < 3; ++m) {
-                    const real* U = vel[m];
+                    const real* U = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
                     real uc = U[q];
                     real lx = (U[xp] - (real)2.0 * uc + U[xm]) * ix2;
                     real ly = (U[yp] - (real)2.0 * uc + U[ym]) * iy2;
                     real lz = (U[zp] - (real)2.0 * uc + U[zm]) * iz2;
-                    diff[m] = mu_eff * (lx + ly + lz);
+                    const real dm = mu_eff * (lx + ly + lz);
+                    df0 = m == 0 ? dm : df0;
+                    df1 = m == 1 ? dm : df1;
+                    df2 = m == 2 ? dm : df2;
                 }
+                diff[0] = df0; diff[1] = df1; diff[2] = df2;
                 const real rho_avg = (real)ldc(&P.dyn[tp.s].rho_avg);
                 real grav = (rf0 - rho_avg) * (real)LES_G_GRAV;
                 real g0 = progress<real>(pf[q], rcp_const<real>(D.k, DUCT_RC_HID));
                 real fx = (real)0, fy = (real)0, fz = (real)0;
                 const real ctr = (real)k.accel_center, Fm = (real)k.max_force;
                 if (g0 > (real)0.01 && g0 < (real)0.99) {
-                    real gx = (progress<real>(pf[xp], rcp_const<real>(D.k, DUCT_RC_HID)) - progress<real>(pf[xm], rcp_const<real>(D.k, DUCT_RC_HID))) * rcp_const<real>(k, DUCT_RC_HID);
-                    real gy = (progress<real>(pf[yp], rcp_const<real>(D.k, DUCT_RC_HID)) - progress<real>(pf[ym], rcp_const<real>(D.k, DUCT_RC_HID))) * rcp_const<real>(k, DUCT_RC_HID + 1);
-                    real gz = (progress<real>(pf[zp], rcp_const<real>(D.k, DUCT_RC_HID)) - progress<real>(pf[zm], rcp_const<real>(D.k, DUCT_RC_HID))) * rcp_const<real>(k, DUCT_RC_HID + 2);
+                    real gx = (real)0, gy = (real)0, gz = (real)0;
+#pragma unroll MU
+                    for (int d = 0; d < 3; ++d) {
+                        const int a = d == 0 ? xp : (d == 1 ? yp : zp), b = d == 0 ? xm : (d == 1 ? ym : zm);
+                        const real gd = (progress<real>(pf[a], rcp_const<real>(D.k, DUCT_RC_HID)) -
+                                         progress<real>(pf[b], rcp_const<real>(D.k, DUCT_RC_HID))) * rcp_dir<real>(k, DUCT_RC_HID, d);
+                        gx = d == 0 ? gd : gx;
+                        gy = d == 1 ? gd : gy;
+                        gz = d == 2 ? gd : gz;
+                    }
                     real df = g0 - ctr;
                     real win = M<real>::exp(-(df * df) * rcp_const<real>(k, DUCT_RC_IW2));
                     real common = (real)k.accel_strength * rf0 * win;
@@ -833,9 +865,7 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                     qo[f][q] = out[f];
                 }
                 if (a.track_vel) {
-                    const real ri = ((const real*)QO[4] + D.off)[q] * rcp_const<real>(k, DUCT_RC_IRUT);   // 1/rho
-                    real uu = out[0] * ri, vv = out[1] * ri, ww = out[2] * ri;
-                    vmax = (double)sqrt(uu * uu + vv * vv + ww * ww);
+                    vmax = cell_speed<real>(out[0], out[1], out[2], ((const real*)QO[4] + D.off)[q], k);
                 }
             }
         }
@@ -903,11 +933,8 @@ __global__ void __launch_bounds__(256) k_duct_velmax(Pool P, int only_slot, int 
         double vm = 0.0;
         if (tp.in && tp.i <= D.g.nx - 2) {
             const int q = (int)lidx(D.g, tp.i, tp.j, tp.kk);
-            real rho = M<real>::div((real)D.k.rho_u_Tu, ((const real*)P.A[4] + D.off)[q]);
-            real u = M<real>::div(((const real*)P.A[0] + D.off)[q], rho);
-            real v = M<real>::div(((const real*)P.A[1] + D.off)[q], rho);
-            real w = M<real>::div(((const real*)P.A[2] + D.off)[q], rho);
-            vm = (double)sqrt(u * u + v * v + w * w);
+            vm = cell_speed<real>(((const real*)P.A[0] + D.off)[q], ((const real*)P.A[1] + D.off)[q],
+                                  ((const real*)P.A[2] + D.off)[q], ((const real*)P.A[4] + D.off)[q], D.k);
         }
         double dummy = 0.0;
         block_max2_256(vm, dummy, red);
@@ -1079,9 +1106,7 @@ __global__ void __launch_bounds__(256) k_duct_bc(Pool P)
         q2[p] = (real)(rho_u * w);
         q3[p] = q3[p + g.sx] - (real)g.dx;
         q4[p] = (real)k.Tu;
-        real rr = M<real>::div((real)k.rho_u_Tu, q4[p]);
-        real uu = M<real>::div(q0[p], rr), vv = M<real>::div(q1[p], rr), ww = M<real>::div(q2[p], rr);
-        vm = (double)sqrt(uu * uu + vv * vv + ww * ww);
+        vm = cell_speed<real>(q0[p], q1[p], q2[p], q4[p], k);
         const long long qn = (long long)(g.nx - 1) * g.sx + p, qm = qn - g.sx;
         q0[qn] = q0[qm]; q1[qn] = q1[qm]; q2[qn] = q2[qm]; q3[qn] = q3[qm]; q4[qn] = q4[qm];
     }
@@ -1250,9 +1275,7 @@ __global__ void __launch_bounds__(256) k_duct_proj_correct(Pool P, int xcd_map)
         const real a1 = m1[q] - (proj_phi(phi, g, i, j + 1, kk) - proj_phi(phi, g, i, j - 1, kk)) * hy;
         const real a2 = m2[q] - (proj_phi(phi, g, i, j, kk + 1) - proj_phi(phi, g, i, j, kk - 1)) * hz;
         m0[q] = a0; m1[q] = a1; m2[q] = a2;
-        const real rho = M<real>::div((real)D.k.rho_u_Tu, ((const real*)P.A[4] + D.off)[q]);
-        const real uu = M<real>::div(a0, rho), vv = M<real>::div(a1, rho), ww = M<real>::div(a2, rho);
-        vm = (double)sqrt(uu * uu + vv * vv + ww * ww);
+        vm = cell_speed<real>(a0, a1, a2, ((const real*)P.A[4] + D.off)[q], D.k);
     }
     double tot = block_sum256(sr, red);
     double dummy = 0.0;

@@ -46,11 +46,14 @@ def test_resume_is_exact(tmp_path, preset, extra):
     assert read(b / "restart_info.txt").split() == [b"DUCT" if preset == "duct64" else b"BINARY", b"30"]
     out = run(common + ["--out", str(b), "--steps", "20"], tmp_path)
     assert "AUTO-RESUME DETECTED (From Checkpoint Step 30)" in out
-    assert read(a / "checkpoint.bin") == read(b / "checkpoint.bin")
-    assert read(a / "stats_backup.bin") == read(b / "stats_backup.bin")
+    # compare to a bool first: pytest's diff of multi-MB byte strings is slow
+    for f in ["checkpoint.bin", "stats_backup.bin"]:
+        same = read(a / f) == read(b / f)
+        assert same, f
     assert csv_files(a) == csv_files(b)
     for f in csv_files(a):
-        assert read(a / f) == read(b / f), f
+        same = read(a / f) == read(b / f)
+        assert same, f
 
 
 def test_output_formats(tmp_path):
