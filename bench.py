@@ -249,6 +249,29 @@ def profile_error(lesgpu, serving, grid, steps):
     return out
 
 
+def fp32_field_error(lesgpu, serving, preset, steps, **over):
+    """FP32 form vs the plain FP64 path (equal to the FP64 oracle within 1e-10)
+    on the same run after `steps` steps: each output field u, v, w, G, T
+    (max |diff| / max |FP64 value|) and the plane profiles."""
+    st, profs = {}, {}
+    for prec in (1, 0):
+        q = lesgpu.params(preset, precision=prec, **over)
+        g = lesgpu.LesGpu(q, lesgpu.mixture())
+        try:
+            g.init_state()
+            g.step(steps)
+            x = g.download_state(5)
+            rho = q.rho_unburnt * q.inlet_temperature / x[4]
+            st[prec] = {"u": x[0] / rho, "v": x[1] / rho, "w": x[2] / rho, "G": x[3], "T": x[4]}
+            profs[prec] = serving.reynolds_profiles(g.field("profiles", 10 * q.nx), q.nx, q.ny, q.nz, steps)
+        finally:
+            g.close()
+    out = {"fields": {k: float(np.abs(st[1][k] - st[0][k]).max() / np.abs(st[0][k]).max()) for k in st[0]},
+           "profiles": serving.profile_errors(profs[1], profs[0], q.inlet_velocity, q.turbulence_intensity),
+           "steps": steps}
+    return out
+
+
 def station_profile_error(lesgpu, serving, p, steps):
     """FP32 form vs the FP64 path (which equals the FP64 oracle within 1e-10):
     mean and Reynolds-stress plane profiles at the run's x-stations after
@@ -362,7 +385,8 @@ def main():
         out["profiles_fp32_vs_fp64"] = station_profile_error(lesgpu, serving, p, a.steps)
     if a.workload == "duct128-fp64" and not a.no_extra:
         f32, _ = single_line(a, lesgpu, "duct128-fp32")
-        out["fp32_form"] = {"value": f32["value"], "ms_per_step": f32["ms_per_step"], "roofline": f32["roofline"]}
+        out["fp32_form"] = {"value": f32["value"], "ms_per_step": f32["ms_per_step"], "roofline": f32["roofline"],
+                            "error_vs_fp64": fp32_field_error(lesgpu, serving, 2, 50)}
         out["ensemble"] = ensemble_leg(lesgpu, serving, a.steps, a.warmup)
     if not a.no_cpu:
         log("cpu baseline")
