@@ -2459,7 +2459,8 @@ int duct_create(les_gpu* c)
     cfg.max_nx = p.nx; cfg.max_ny = p.ny; cfg.max_nz = p.nz;
     cfg.max_vortices = p.num_vortices > 0 ? p.num_vortices : 1;
     cfg.precision = p.precision;
-    cfg.tile_k = 64;
+    // the widest tile row that divides nz (no idle lanes), else flat tiles
+    cfg.tile_k = (p.nz % 64 == 0) ? 64 : ((p.nz % 32 == 0) ? 32 : LES_TILE_FLAT);
     cfg.xcd_map = 1;
     cfg.point_cap = 4096;
     cfg.probe_cap = p.num_probe_x > 0 ? 64 : 0;

@@ -1,10 +1,12 @@
 """Batched duct runs (les_batch_*): a pool of runs advanced by one launch
 sequence per step.
 
-* a run in a pool gives bitwise the same state as the same run in a
-  single-run context (same tile width), whatever else is resident — other
-  grids, parameters, precisions of step counts, runs admitted and retired
-  mid-way (continuous batching);
+* a run in a pool gives bitwise the same state as the same run alone in a
+  pool of the same configuration (tile width, launch mapping), whatever else
+  is resident — other grids, parameters, step counts, runs admitted and
+  retired mid-way (continuous batching).  The tile width groups the
+  per-tile partial sums of the domain-mean density, so runs under different
+  tile widths agree to rounding, not bitwise;
 * every tile shape and launch configuration the autotuner may pick stays
   within the FP64 tolerance (1e-10 of each field's scale) of the oracle;
 * admission refuses runs that do not fit and reuses freed arena ranges.
@@ -26,13 +28,11 @@ def run_params(lesgpu, precision=0, **over):
 
 
 def single_run(lesgpu, p, steps):
-    g = lesgpu.LesGpu(p)
-    try:
-        g.init_state()
-        g.step(steps)
-        return g.download_state(5), g.field("profiles", 10 * p.nx)
-    finally:
-        g.close()
+    """The run alone in a pool of the test pools' configuration."""
+    with pool(lesgpu, p.precision, slots=1, cells=p.nx * p.ny * p.nz) as b:
+        s = b.admit(p, n_steps=steps)
+        b.step(steps)
+        return b.download(s), b.field(s, "profiles", 10 * p.nx)
 
 
 def pool(lesgpu, precision=0, slots=4, cells=None, **over):
