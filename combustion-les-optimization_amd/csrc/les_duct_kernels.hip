@@ -331,6 +331,23 @@ __device__ __forceinline__ T ldc(const T* p)
 #endif
 }
 
+// A run-local array at a block-uniform base (an arena pointer plus the run's
+// cell offset).  Element i is addressed through a 32-bit byte offset, so a
+// load or store takes the SGPR-base + 32-bit VGPR-offset form instead of a
+// 64-bit address per access; admission keeps one run's field below 4 GiB.
+template <typename T>
+struct RunArr {
+    T* p;
+    __device__ __forceinline__ T& operator[](int i) const
+    {
+        using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+        return *(T*)((B*)p + (unsigned)i * (unsigned)sizeof(T));
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ RunArr<T> run_arr(T* p) { return RunArr<T>{p}; }
+
 // The parts of a run descriptor a stencil kernel uses, held in SGPRs.
 struct SlotView {
     GridDims g;
@@ -742,22 +759,24 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
             const int q = bi + j * g.nz + kk;
             void* const* QI = a.qin_is_B ? P.B : P.A;
             void* const* QO = a.qout_is_B ? P.B : P.A;
-            const real* qi[3] = {(const real*)QI[0] + D.off, (const real*)QI[1] + D.off, (const real*)QI[2] + D.off};
-            real* qo[3] = {(real*)QO[0] + D.off, (real*)QO[1] + D.off, (real*)QO[2] + D.off};
+            const RunArr<const real> qi[3] = {run_arr((const real*)QI[0] + D.off), run_arr((const real*)QI[1] + D.off),
+                                              run_arr((const real*)QI[2] + D.off)};
+            const RunArr<real> qo[3] = {run_arr((real*)QO[0] + D.off), run_arr((real*)QO[1] + D.off),
+                                        run_arr((real*)QO[2] + D.off)};
             if (i == 0 || i == g.nx - 1) {
 #pragma unroll
                 for (int f = 0; f < 3; ++f) qo[f][q] = qi[f][q];
             } else {
-                const real* uf = (const real*)P.F[0] + D.off;
-                const real* vf = (const real*)P.F[1] + D.off;
-                const real* wf = (const real*)P.F[2] + D.off;
-                const real* Tf = (const real*)P.F[3] + D.off;
-                const real* pf = (const real*)P.F[4] + D.off;
-                const real* nut = (const real*)P.nut + D.off;
+                const auto uf = run_arr((const real*)P.F[0] + D.off);
+                const auto vf = run_arr((const real*)P.F[1] + D.off);
+                const auto wf = run_arr((const real*)P.F[2] + D.off);
+                const auto Tf = run_arr((const real*)P.F[3] + D.off);
+                const auto pf = run_arr((const real*)P.F[4] + D.off);
+                const auto nut = run_arr((const real*)P.nut + D.off);
                 const real rut = (real)k.rho_u_Tu;
                 const bool xband = (i < 3 || i > g.nx - 4);
                 const int scheme = k.scheme;
-                const real* vel[3] = {uf, vf, wf};
+                const RunArr<const real> vel[3] = {uf, vf, wf};
                 const real T0 = Tf[q];
                 const real rf0 = M<real>::div(rut, T0);
                 // FP64 keeps the component loops rolled (fewer live registers,
@@ -768,7 +787,7 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 for (int d = 0; d < 3; ++d) {
                     const real id = rcp_dir<real>(k, DUCT_RC_ID, d);
                     const int qp = line_idx(g, bi, j, kk, d, 1), qm = line_idx(g, bi, j, kk, d, -1);
-                    const real* U = vel[d];
+                    const RunArr<const real> U = d == 0 ? vel[0] : (d == 1 ? vel[1] : vel[2]);
                     real up = (real)0.5 * (U[qp] + U[q]);
                     real um = (real)0.5 * (U[q] + U[qm]);
                     real rp = (real)0.5 * (M<real>::div(rut, Tf[qp]) + rf0);
@@ -777,7 +796,7 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                     const bool band = (d == 0 && xband);
 #pragma unroll MU
                     for (int m = 0; m < 3; ++m) {
-                        const real* V = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
+                        const RunArr<const real> V = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
                         real fp, fm;
                         if (band) {
                             real v2 = V[qm], v3 = V[q], v4 = V[qp];
@@ -810,7 +829,7 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 real df0 = (real)0, df1 = (real)0, df2 = (real)0;
 #pragma unroll MU
                 for (int m = 0; m < 3; ++m) {
-                    const real* U = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
+                    const RunArr<const real> U = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
                     real uc = U[q];
                     real lx = (U[xp] - (real)2.0 * uc + U[xm]) * ix2;
                     real ly = (U[yp] - (real)2.0 * uc + U[ym]) * iy2;
@@ -861,11 +880,11 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 for (int f = 0; f < 3; ++f) {
                     real y = qi[f][q] + DT * Lq[f];
                     if (a.stage == 0) out[f] = y;
-                    else out[f] = (real)a.ca * ((const real*)P.A[f] + D.off)[q] + (real)a.cb * y;
+                    else out[f] = (real)a.ca * run_arr((const real*)P.A[f] + D.off)[q] + (real)a.cb * y;
                     qo[f][q] = out[f];
                 }
                 if (a.track_vel) {
-                    vmax = cell_speed<real>(out[0], out[1], out[2], ((const real*)QO[4] + D.off)[q], k);
+                    vmax = cell_speed<real>(out[0], out[1], out[2], run_arr((const real*)QO[4] + D.off)[q], k);
                 }
             }
         }
@@ -1896,6 +1915,8 @@ static int pool_admit(les_gpu* c, const les_params& p, int n_steps, int* slot_ou
         p.num_vortices > d->nvort_max || p.reinit_iters < 0 || p.reinit_iters > 64 || p.proj_iters < 0 ||
         p.proj_iters > 256)
         return LES_ERR_ADMISSION;
+    // kernels address a run's field with 32-bit byte offsets (RunArr)
+    if ((long long)p.nx * p.ny * p.nz * (long long)d->esize >= (1ll << 32)) return LES_ERR_PARAM;
     int s = -1;
     for (int a = 0; a < d->max_slots; ++a) if (!d->taken[a]) { s = a; break; }
     if (s < 0) return LES_ERR_ADMISSION;
