@@ -451,11 +451,11 @@ __global__ void __launch_bounds__(256) k_duct_filter(Pool P, int qin_is_B, int x
         const GridDims g = D.g;
         const DuctConsts& k = D.k;
         void* const* Q = qin_is_B ? P.B : P.A;
-        const real* q0 = (const real*)Q[0] + D.off;
-        const real* q1 = (const real*)Q[1] + D.off;
-        const real* q2 = (const real*)Q[2] + D.off;
-        const real* q3 = (const real*)Q[3] + D.off;
-        const real* q4 = (const real*)Q[4] + D.off;
+        const auto q0 = run_arr((const real*)Q[0] + D.off);
+        const auto q1 = run_arr((const real*)Q[1] + D.off);
+        const auto q2 = run_arr((const real*)Q[2] + D.off);
+        const auto q3 = run_arr((const real*)Q[3] + D.off);
+        const auto q4 = run_arr((const real*)Q[4] + D.off);
         double rsum = 0.0;
         if (tp.in) {
             const int i = tp.i, j = tp.j, kk = tp.kk;
@@ -489,11 +489,11 @@ __global__ void __launch_bounds__(256) k_duct_filter(Pool P, int qin_is_B, int x
                 fT = (pT[0] + pT[1] + pT[2] + wc * pT[3] + pT[4] + pT[5] + pT[6]) * iden;
                 fp = (pp[0] + pp[1] + pp[2] + wc * pp[3] + pp[4] + pp[5] + pp[6]) * iden;
             }
-            real* F0 = (real*)P.F[0] + D.off;
-            real* F1 = (real*)P.F[1] + D.off;
-            real* F2 = (real*)P.F[2] + D.off;
-            real* F3 = (real*)P.F[3] + D.off;
-            real* F4 = (real*)P.F[4] + D.off;
+            const auto F0 = run_arr((real*)P.F[0] + D.off);
+            const auto F1 = run_arr((real*)P.F[1] + D.off);
+            const auto F2 = run_arr((real*)P.F[2] + D.off);
+            const auto F3 = run_arr((real*)P.F[3] + D.off);
+            const auto F4 = run_arr((real*)P.F[4] + D.off);
             F0[q] = fu; F1[q] = fv; F2[q] = fw; F3[q] = fT; F4[q] = fp;
             rsum = (double)(rut / fT);
         }
@@ -538,13 +538,13 @@ __global__ void __launch_bounds__(256) k_duct_sgs(Pool P, int stage, int xcd_map
         if (tp.in) {
             const int i = tp.i, j = tp.j, kk = tp.kk;
             const int q = (int)lidx(g, i, j, kk);
-            const real* uf = (const real*)P.F[0] + D.off;
-            const real* vf = (const real*)P.F[1] + D.off;
-            const real* wf = (const real*)P.F[2] + D.off;
-            const real* Tf = (const real*)P.F[3] + D.off;
-            const real* pf = (const real*)P.F[4] + D.off;
-            real* nut = (real*)P.nut + D.off;
-            real* stv = (real*)P.st + D.off;
+            const auto uf = run_arr((const real*)P.F[0] + D.off);
+            const auto vf = run_arr((const real*)P.F[1] + D.off);
+            const auto wf = run_arr((const real*)P.F[2] + D.off);
+            const auto Tf = run_arr((const real*)P.F[3] + D.off);
+            const auto pf = run_arr((const real*)P.F[4] + D.off);
+            const auto nut = run_arr((real*)P.nut + D.off);
+            const auto stv = run_arr((real*)P.st + D.off);
             const real SL = (real)k.s_l;
             if (i == 0 || i == g.nx - 1) {
                 nut[q] = (real)0.0; stv[q] = SL;
@@ -637,35 +637,35 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         const int q = bi + j * g.nz + kk;
         void* const* QI = a.qin_is_B ? P.B : P.A;
         void* const* QO = a.qout_is_B ? P.B : P.A;
-        const real* qin3 = (const real*)QI[3] + D.off;
-        const real* qin4 = (const real*)QI[4] + D.off;
-        real* qo3 = (real*)QO[3] + D.off;
-        real* qo4 = (real*)QO[4] + D.off;
+        const auto qin3 = run_arr((const real*)QI[3] + D.off);
+        const auto qin4 = run_arr((const real*)QI[4] + D.off);
+        const auto qo3 = run_arr((real*)QO[3] + D.off);
+        const auto qo4 = run_arr((real*)QO[4] + D.off);
         if (i == 0 || i == g.nx - 1) {
             qo3[q] = qin3[q];
             qo4[q] = qin4[q];
             return;
         }
-        const real* uf = (const real*)P.F[0] + D.off;
-        const real* vf = (const real*)P.F[1] + D.off;
-        const real* wf = (const real*)P.F[2] + D.off;
-        const real* Tf = (const real*)P.F[3] + D.off;
-        const real* pf = (const real*)P.F[4] + D.off;
-        const real* nut = (const real*)P.nut + D.off;
+        const auto uf = run_arr((const real*)P.F[0] + D.off);
+        const auto vf = run_arr((const real*)P.F[1] + D.off);
+        const auto wf = run_arr((const real*)P.F[2] + D.off);
+        const auto Tf = run_arr((const real*)P.F[3] + D.off);
+        const auto pf = run_arr((const real*)P.F[4] + D.off);
+        const auto nut = run_arr((const real*)P.nut + D.off);
         const real rut = (real)k.rho_u_Tu;
         const bool xband = (i < 3 || i > g.nx - 4);
         const int scheme = k.scheme;
-        const real* vel[3] = {uf, vf, wf};
+        const RunArr<const real> vel[3] = {uf, vf, wf};
         const real T0 = Tf[q];
         const real nu0 = nut[q];
         const int xp = q + (int)g.sx, xm = q - (int)g.sx;
         const int yp = line_idx(g, bi, j, kk, 1, 1), ym = line_idx(g, bi, j, kk, 1, -1);
         const int zp = line_idx(g, bi, j, kk, 2, 1), zm = line_idx(g, bi, j, kk, 2, -1);
-        const real* S2[2] = {pf, Tf};
+        const RunArr<const real> S2[2] = {pf, Tf};
         real cv[2], sg[2], lap[2], grad[3];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const real* Fs = S2[s];
+            const RunArr<const real> Fs = S2[s];
             const real isc2 = rcp_const<real>(k, (s == 0) ? DUCT_RC_ISC : DUCT_RC_IPR);
             real acc = (real)0, sacc = (real)0, lacc = (real)0;
 #pragma unroll
@@ -708,7 +708,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             real win = M<real>::exp(-(df * df) * rcp_const<real>(k, DUCT_RC_IW2));
             art = (real)5.0 * (nu0 + (real)1.0e-5) * win * lap[0];
         }
-        const real* stv = (const real*)P.st + D.off;
+        const auto stv = run_arr((const real*)P.st + D.off);
         real L3 = -cv[0] + stv[q] * gmag + sg[0] + art;
         const real rf0 = M<real>::div(rut, T0);
         real mu = table_lookup<real>(P.mu_tab, k, T0);
@@ -724,8 +724,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         real v3, v4;
         if (a.stage == 0) { v3 = y3; v4 = y4; }
         else {
-            const real* qa3 = (const real*)P.A[3] + D.off;
-            const real* qa4 = (const real*)P.A[4] + D.off;
+            const auto qa3 = run_arr((const real*)P.A[3] + D.off);
+            const auto qa4 = run_arr((const real*)P.A[4] + D.off);
             v3 = (real)a.ca * qa3[q] + (real)a.cb * y3;
             v4 = (real)a.ca * qa4[q] + (real)a.cb * y4;
         }
