@@ -1156,9 +1156,9 @@ __global__ void __launch_bounds__(256) k_duct_reinit(Pool P, int phase, int iter
         const GridDims g = D.g;
         const int i = tp.i, j = tp.j, kk = tp.kk;
         const int q = (int)lidx(g, i, j, kk);
-        real* phi = (real*)P.A[3] + D.off;
-        real* phi0 = (real*)P.B[3] + D.off;
-        real* tmp = (real*)P.B[4] + D.off;
+        const auto phi = run_arr((real*)P.A[3] + D.off);
+        const auto phi0 = run_arr((real*)P.B[3] + D.off);
+        const auto tmp = run_arr((real*)P.B[4] + D.off);
         if (phase == 0) {             // save phi0 (first iteration only)
             if (iter == 0) phi0[q] = phi[q];
             return;
@@ -1168,13 +1168,15 @@ __global__ void __launch_bounds__(256) k_duct_reinit(Pool P, int phase, int iter
             return;
         }
         if (i == 0 || i == g.nx - 1) { tmp[q] = phi[q]; return; }
-        const real dx = (real)g.dx, dy = (real)g.dy, dz = (real)g.dz, dtau = (real)0.3 * dx;
+        const real dx = (real)g.dx, dtau = (real)0.3 * dx;
+        const real idx = rcp_const<real>(D.k, DUCT_RC_ID), idy = rcp_const<real>(D.k, DUCT_RC_ID + 1),
+                   idz = rcp_const<real>(D.k, DUCT_RC_ID + 2);
         real p0 = phi0[q];
-        real sgn = p0 / sqrt(p0 * p0 + dx * dx);
+        real sgn = M<real>::div(p0, sqrt(p0 * p0 + dx * dx));
         real c = phi[q];
-        real ax = (c - phi[q - (int)g.sx]) / dx, bx = (phi[q + (int)g.sx] - c) / dx;
-        real ay = (c - phi[lidx(g, i, wrapi(j - 1, g.ny), kk)]) / dy, by = (phi[lidx(g, i, wrapi(j + 1, g.ny), kk)] - c) / dy;
-        real az = (c - phi[lidx(g, i, j, wrapi(kk - 1, g.nz))]) / dz, bz = (phi[lidx(g, i, j, wrapi(kk + 1, g.nz))] - c) / dz;
+        real ax = (c - phi[q - (int)g.sx]) * idx, bx = (phi[q + (int)g.sx] - c) * idx;
+        real ay = (c - phi[(int)lidx(g, i, wrapi(j - 1, g.ny), kk)]) * idy, by = (phi[(int)lidx(g, i, wrapi(j + 1, g.ny), kk)] - c) * idy;
+        real az = (c - phi[(int)lidx(g, i, j, wrapi(kk - 1, g.nz))]) * idz, bz = (phi[(int)lidx(g, i, j, wrapi(kk + 1, g.nz))] - c) * idz;
         real gx, gy, gz;
         if (p0 > (real)0.0) {
             real a1 = fmax(ax, (real)0), b1 = fmin(bx, (real)0), a2 = fmax(ay, (real)0), b2 = fmin(by, (real)0);
@@ -1199,22 +1201,22 @@ __global__ void __launch_bounds__(256) k_duct_reinit(Pool P, int phase, int iter
 // (B is dead after the last stage).  FP64 per-tile sums of b^2 and of the
 // final residual (b - L phi)^2, reduced per run in a fixed order.
 // ---------------------------------------------------------------------------
-template <typename real>
-__device__ __forceinline__ real proj_phi(const real* phi, const GridDims& g, int i, int j, int kk)
+template <typename real, typename Arr>
+__device__ __forceinline__ real proj_phi(const Arr& phi, const GridDims& g, int i, int j, int kk)
 {
     if (i <= 0) i = 1;
     if (i >= g.nx - 1) return (real)0.0;
     return phi[i * (int)g.sx + wrapi(j, g.ny) * g.nz + wrapi(kk, g.nz)];
 }
 
-template <typename real>
-__device__ __forceinline__ real proj_lap(const real* phi, const GridDims& g, const DuctConsts& k, int i, int j, int kk)
+template <typename real, typename Arr>
+__device__ __forceinline__ real proj_lap(const Arr& phi, const GridDims& g, const DuctConsts& k, int i, int j, int kk)
 {
     const real ix2 = rcp_const<real>(k, DUCT_RC_ID2), iy2 = rcp_const<real>(k, DUCT_RC_ID2 + 1), iz2 = rcp_const<real>(k, DUCT_RC_ID2 + 2);
     const real p0 = phi[i * (int)g.sx + j * g.nz + kk];
-    real lx = (proj_phi(phi, g, i + 1, j, kk) - (real)2.0 * p0 + proj_phi(phi, g, i - 1, j, kk)) * ix2;
-    real ly = (proj_phi(phi, g, i, j + 1, kk) - (real)2.0 * p0 + proj_phi(phi, g, i, j - 1, kk)) * iy2;
-    real lz = (proj_phi(phi, g, i, j, kk + 1) - (real)2.0 * p0 + proj_phi(phi, g, i, j, kk - 1)) * iz2;
+    real lx = (proj_phi<real>(phi, g, i + 1, j, kk) - (real)2.0 * p0 + proj_phi<real>(phi, g, i - 1, j, kk)) * ix2;
+    real ly = (proj_phi<real>(phi, g, i, j + 1, kk) - (real)2.0 * p0 + proj_phi<real>(phi, g, i, j - 1, kk)) * iy2;
+    real lz = (proj_phi<real>(phi, g, i, j, kk + 1) - (real)2.0 * p0 + proj_phi<real>(phi, g, i, j, kk - 1)) * iz2;
     return lx + ly + lz;
 }
 
@@ -1232,9 +1234,9 @@ __global__ void __launch_bounds__(256) k_duct_proj_div(Pool P, int xcd_map)
     double sb = 0.0;
     if (tp.in && i > 0 && i < g.nx - 1) {
         const int q = i * (int)g.sx + j * g.nz + kk;
-        const real* m0 = (const real*)P.A[0] + D.off;
-        const real* m1 = (const real*)P.A[1] + D.off;
-        const real* m2 = (const real*)P.A[2] + D.off;
+        const auto m0 = run_arr((const real*)P.A[0] + D.off);
+        const auto m1 = run_arr((const real*)P.A[1] + D.off);
+        const auto m2 = run_arr((const real*)P.A[2] + D.off);
         const real hx = rcp_const<real>(D.k, DUCT_RC_HID), hy = rcp_const<real>(D.k, DUCT_RC_HID + 1), hz = rcp_const<real>(D.k, DUCT_RC_HID + 2);
         real v = (m0[q + (int)g.sx] - m0[q - (int)g.sx]) * hx +
                  (m1[i * (int)g.sx + wrapi(j + 1, g.ny) * g.nz + kk] - m1[i * (int)g.sx + wrapi(j - 1, g.ny) * g.nz + kk]) * hy +
@@ -1260,8 +1262,8 @@ __global__ void __launch_bounds__(256) k_duct_proj_sweep(Pool P, int it, int xcd
     const int i = tp.i;
     if (it >= D.k.proj_iters || !tp.in || i == 0 || i == g.nx - 1) return;
     const int q = i * (int)g.sx + tp.j * g.nz + tp.kk;
-    const real* po = (const real*)P.B[1 + (it & 1)] + D.off;
-    real* pn = (real*)P.B[1 + ((it + 1) & 1)] + D.off;
+    const auto po = run_arr((const real*)P.B[1 + (it & 1)] + D.off);
+    const auto pn = run_arr((real*)P.B[1 + ((it + 1) & 1)] + D.off);
     const real b = ((const real*)P.B[0] + D.off)[q];
     const real inv_d = rcp_const<real>(D.k, DUCT_RC_INVD);
     if (it == 0) pn[q] = ((real)0.0 - b) * inv_d;           // phi = 0 before the first sweep
@@ -1282,17 +1284,17 @@ __global__ void __launch_bounds__(256) k_duct_proj_correct(Pool P, int xcd_map)
     double sr = 0.0, vm = 0.0;
     if (tp.in && i > 0 && i < g.nx - 1) {
         const int q = i * (int)g.sx + j * g.nz + kk;
-        const real* phi = (const real*)P.B[1 + (D.k.proj_iters & 1)] + D.off;
+        const auto phi = run_arr((const real*)P.B[1 + (D.k.proj_iters & 1)] + D.off);
         const real b = ((const real*)P.B[0] + D.off)[q];
         const real r = b - proj_lap<real>(phi, g, D.k, i, j, kk);
         sr = (double)r * (double)r;
         const real hx = rcp_const<real>(D.k, DUCT_RC_HID), hy = rcp_const<real>(D.k, DUCT_RC_HID + 1), hz = rcp_const<real>(D.k, DUCT_RC_HID + 2);
-        real* m0 = (real*)P.A[0] + D.off;
-        real* m1 = (real*)P.A[1] + D.off;
-        real* m2 = (real*)P.A[2] + D.off;
-        const real a0 = m0[q] - (proj_phi(phi, g, i + 1, j, kk) - proj_phi(phi, g, i - 1, j, kk)) * hx;
-        const real a1 = m1[q] - (proj_phi(phi, g, i, j + 1, kk) - proj_phi(phi, g, i, j - 1, kk)) * hy;
-        const real a2 = m2[q] - (proj_phi(phi, g, i, j, kk + 1) - proj_phi(phi, g, i, j, kk - 1)) * hz;
+        const auto m0 = run_arr((real*)P.A[0] + D.off);
+        const auto m1 = run_arr((real*)P.A[1] + D.off);
+        const auto m2 = run_arr((real*)P.A[2] + D.off);
+        const real a0 = m0[q] - (proj_phi<real>(phi, g, i + 1, j, kk) - proj_phi<real>(phi, g, i - 1, j, kk)) * hx;
+        const real a1 = m1[q] - (proj_phi<real>(phi, g, i, j + 1, kk) - proj_phi<real>(phi, g, i, j - 1, kk)) * hy;
+        const real a2 = m2[q] - (proj_phi<real>(phi, g, i, j, kk + 1) - proj_phi<real>(phi, g, i, j, kk - 1)) * hz;
         m0[q] = a0; m1[q] = a1; m2[q] = a2;
         vm = cell_speed<real>(a0, a1, a2, ((const real*)P.A[4] + D.off)[q], D.k);
     }
@@ -1367,11 +1369,11 @@ __global__ void __launch_bounds__(256) k_duct_stats(Pool P)
     const long long p1 = p0 + chunk < PL ? p0 + chunk : PL;
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const real rut = (real)k.rho_u_Tu;
-    const real* A0 = (const real*)P.A[0] + D.off;
-    const real* A1 = (const real*)P.A[1] + D.off;
-    const real* A2 = (const real*)P.A[2] + D.off;
-    const real* A3 = (const real*)P.A[3] + D.off;
-    const real* A4 = (const real*)P.A[4] + D.off;
+    const auto A0 = run_arr((const real*)P.A[0] + D.off);
+    const auto A1 = run_arr((const real*)P.A[1] + D.off);
+    const auto A2 = run_arr((const real*)P.A[2] + D.off);
+    const auto A3 = run_arr((const real*)P.A[3] + D.off);
+    const auto A4 = run_arr((const real*)P.A[4] + D.off);
     const bool per_cell = k.stats_per_cell && P.stat[0];
     for (long long p = p0 + threadIdx.x; p < p1; p += 256) {
         long long q = (long long)i * PL + p;
