@@ -348,6 +348,45 @@ struct RunArr {
 template <typename T>
 __device__ __forceinline__ RunArr<T> run_arr(T* p) { return RunArr<T>{p}; }
 
+// A run-local array that the kernel only reads, behind a buffer resource (base
+// and size in SGPRs): element i is one buffer load at a 32-bit byte offset,
+// constant neighbour offsets fold into the instruction, and a read outside
+// the run's n elements returns 0.  Only for arrays no thread of the kernel
+// writes (buffer loads are not ordered against the kernel's global stores).
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+struct RunBuf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ T operator[](int i) const
+    {
+        static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit elements");
+        const int off = (int)((unsigned)i * (unsigned)sizeof(T));
+        if constexpr (sizeof(T) == 8)
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+        else
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    }
+};
+
+// 0x00020000: the gfx9-family buffer descriptor word 3 for raw 32-bit data
+template <typename T>
+__device__ __forceinline__ RunBuf<T> run_buf(const T* p, long long n)
+{
+    RunBuf<T> b;
+    b.r = __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (long long)((unsigned long long)n * sizeof(T)),
+                                            0x00020000);
+    return b;
+}
+#else
+template <typename T>
+struct RunBuf {                                      // host pass only, never executed
+    const T* p;
+    __device__ T operator[](int i) const { return p[i]; }
+};
+template <typename T>
+__device__ RunBuf<T> run_buf(const T* p, long long) { return RunBuf<T>{p}; }
+#endif
+
 // The parts of a run descriptor a stencil kernel uses, held in SGPRs.
 struct SlotView {
     GridDims g;
@@ -646,26 +685,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             qo4[q] = qin4[q];
             return;
         }
-        const auto uf = run_arr((const real*)P.F[0] + D.off);
-        const auto vf = run_arr((const real*)P.F[1] + D.off);
-        const auto wf = run_arr((const real*)P.F[2] + D.off);
-        const auto Tf = run_arr((const real*)P.F[3] + D.off);
-        const auto pf = run_arr((const real*)P.F[4] + D.off);
-        const auto nut = run_arr((const real*)P.nut + D.off);
+        const auto uf = run_buf((const real*)P.F[0] + D.off, g.n);
+        const auto vf = run_buf((const real*)P.F[1] + D.off, g.n);
+        const auto wf = run_buf((const real*)P.F[2] + D.off, g.n);
+        const auto Tf = run_buf((const real*)P.F[3] + D.off, g.n);
+        const auto pf = run_buf((const real*)P.F[4] + D.off, g.n);
+        const auto nut = run_buf((const real*)P.nut + D.off, g.n);
         const real rut = (real)k.rho_u_Tu;
         const bool xband = (i < 3 || i > g.nx - 4);
         const int scheme = k.scheme;
-        const RunArr<const real> vel[3] = {uf, vf, wf};
+        const RunBuf<real> vel[3] = {uf, vf, wf};
         const real T0 = Tf[q];
         const real nu0 = nut[q];
         const int xp = q + (int)g.sx, xm = q - (int)g.sx;
         const int yp = line_idx(g, bi, j, kk, 1, 1), ym = line_idx(g, bi, j, kk, 1, -1);
         const int zp = line_idx(g, bi, j, kk, 2, 1), zm = line_idx(g, bi, j, kk, 2, -1);
-        const RunArr<const real> S2[2] = {pf, Tf};
+        const RunBuf<real> S2[2] = {pf, Tf};
         real cv[2], sg[2], lap[2], grad[3];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const RunArr<const real> Fs = S2[s];
+            const RunBuf<real> Fs = S2[s];
             const real isc2 = rcp_const<real>(k, (s == 0) ? DUCT_RC_ISC : DUCT_RC_IPR);
             real acc = (real)0, sacc = (real)0, lacc = (real)0;
 #pragma unroll
@@ -708,7 +747,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             real win = M<real>::exp(-(df * df) * rcp_const<real>(k, DUCT_RC_IW2));
             art = (real)5.0 * (nu0 + (real)1.0e-5) * win * lap[0];
         }
-        const auto stv = run_arr((const real*)P.st + D.off);
+        const auto stv = run_buf((const real*)P.st + D.off, g.n);
         real L3 = -cv[0] + stv[q] * gmag + sg[0] + art;
         const real rf0 = M<real>::div(rut, T0);
         real mu = table_lookup<real>(P.mu_tab, k, T0);
@@ -767,16 +806,16 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
 #pragma unroll
                 for (int f = 0; f < 3; ++f) qo[f][q] = qi[f][q];
             } else {
-                const auto uf = run_arr((const real*)P.F[0] + D.off);
-                const auto vf = run_arr((const real*)P.F[1] + D.off);
-                const auto wf = run_arr((const real*)P.F[2] + D.off);
-                const auto Tf = run_arr((const real*)P.F[3] + D.off);
-                const auto pf = run_arr((const real*)P.F[4] + D.off);
-                const auto nut = run_arr((const real*)P.nut + D.off);
+                const auto uf = run_buf((const real*)P.F[0] + D.off, g.n);
+                const auto vf = run_buf((const real*)P.F[1] + D.off, g.n);
+                const auto wf = run_buf((const real*)P.F[2] + D.off, g.n);
+                const auto Tf = run_buf((const real*)P.F[3] + D.off, g.n);
+                const auto pf = run_buf((const real*)P.F[4] + D.off, g.n);
+                const auto nut = run_buf((const real*)P.nut + D.off, g.n);
                 const real rut = (real)k.rho_u_Tu;
                 const bool xband = (i < 3 || i > g.nx - 4);
                 const int scheme = k.scheme;
-                const RunArr<const real> vel[3] = {uf, vf, wf};
+                const RunBuf<real> vel[3] = {uf, vf, wf};
                 const real T0 = Tf[q];
                 const real rf0 = M<real>::div(rut, T0);
                 // FP64 keeps the component loops rolled (fewer live registers,
@@ -787,7 +826,7 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 for (int d = 0; d < 3; ++d) {
                     const real id = rcp_dir<real>(k, DUCT_RC_ID, d);
                     const int qp = line_idx(g, bi, j, kk, d, 1), qm = line_idx(g, bi, j, kk, d, -1);
-                    const RunArr<const real> U = d == 0 ? vel[0] : (d == 1 ? vel[1] : vel[2]);
+                    const RunBuf<real> U = d == 0 ? vel[0] : (d == 1 ? vel[1] : vel[2]);
                     real up = (real)0.5 * (U[qp] + U[q]);
                     real um = (real)0.5 * (U[q] + U[qm]);
                     real rp = (real)0.5 * (M<real>::div(rut, Tf[qp]) + rf0);
@@ -796,7 +835,7 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                     const bool band = (d == 0 && xband);
 #pragma unroll MU
                     for (int m = 0; m < 3; ++m) {
-                        const RunArr<const real> V = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
+                        const RunBuf<real> V = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
                         real fp, fm;
                         if (band) {
                             real v2 = V[qm], v3 = V[q], v4 = V[qp];
@@ -829,7 +868,7 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 real df0 = (real)0, df1 = (real)0, df2 = (real)0;
 #pragma unroll MU
                 for (int m = 0; m < 3; ++m) {
-                    const RunArr<const real> U = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
+                    const RunBuf<real> U = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
                     real uc = U[q];
                     real lx = (U[xp] - (real)2.0 * uc + U[xm]) * ix2;
                     real ly = (U[yp] - (real)2.0 * uc + U[ym]) * iy2;
