@@ -490,11 +490,11 @@ __global__ void __launch_bounds__(256) k_duct_filter(Pool P, int qin_is_B, int x
         const GridDims g = D.g;
         const DuctConsts& k = D.k;
         void* const* Q = qin_is_B ? P.B : P.A;
-        const auto q0 = run_arr((const real*)Q[0] + D.off);
-        const auto q1 = run_arr((const real*)Q[1] + D.off);
-        const auto q2 = run_arr((const real*)Q[2] + D.off);
-        const auto q3 = run_arr((const real*)Q[3] + D.off);
-        const auto q4 = run_arr((const real*)Q[4] + D.off);
+        const auto q0 = run_buf((const real*)Q[0] + D.off, g.n);
+        const auto q1 = run_buf((const real*)Q[1] + D.off, g.n);
+        const auto q2 = run_buf((const real*)Q[2] + D.off, g.n);
+        const auto q3 = run_buf((const real*)Q[3] + D.off, g.n);
+        const auto q4 = run_buf((const real*)Q[4] + D.off, g.n);
         double rsum = 0.0;
         if (tp.in) {
             const int i = tp.i, j = tp.j, kk = tp.kk;
@@ -577,11 +577,11 @@ __global__ void __launch_bounds__(256) k_duct_sgs(Pool P, int stage, int xcd_map
         if (tp.in) {
             const int i = tp.i, j = tp.j, kk = tp.kk;
             const int q = (int)lidx(g, i, j, kk);
-            const auto uf = run_arr((const real*)P.F[0] + D.off);
-            const auto vf = run_arr((const real*)P.F[1] + D.off);
-            const auto wf = run_arr((const real*)P.F[2] + D.off);
-            const auto Tf = run_arr((const real*)P.F[3] + D.off);
-            const auto pf = run_arr((const real*)P.F[4] + D.off);
+            const auto uf = run_buf((const real*)P.F[0] + D.off, g.n);
+            const auto vf = run_buf((const real*)P.F[1] + D.off, g.n);
+            const auto wf = run_buf((const real*)P.F[2] + D.off, g.n);
+            const auto Tf = run_buf((const real*)P.F[3] + D.off, g.n);
+            const auto pf = run_buf((const real*)P.F[4] + D.off, g.n);
             const auto nut = run_arr((real*)P.nut + D.off);
             const auto stv = run_arr((real*)P.st + D.off);
             const real SL = (real)k.s_l;
