@@ -660,8 +660,10 @@ struct RhsArgs {
 
 // scalar RHS (phi: G equation :2706-2818; T :2912-2955) + SSP-RK3 combine
 template <typename real, int TK>
-// FP64: at least 4 waves per SIMD (128 VGPRs, a 12-byte spill), measured faster than 3
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_duct_rhs_scalar(Pool P, RhsArgs<real> a)
+// FP64: at least 4 waves per SIMD (122 VGPRs; a 5-wave cap spills 108 bytes);
+// FP32: 8 waves with flat tiles, 7 with row tiles (8 would spill 16 bytes)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 4 : (TK == 0 ? 8 : 7))))
+k_duct_rhs_scalar(Pool P, RhsArgs<real> a)
 {
     const int t = tile_of_block(P, a.xcd_map);
     TilePos tp;
@@ -777,10 +779,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 // momentum RHS (:2552-2691 structure) + SSP-RK3 combine; in the last stage also
 // the per-tile max |u| of the new state (its T was written by the scalar kernel)
 template <typename real, int TK>
-// FP64: at least 6 waves per SIMD with the component loops rolled (80 VGPRs
-// and a 12-byte spill, measured 2% faster than 4 waves unrolled at 115 VGPRs);
-// FP32: at least 7 waves (72 VGPRs, no spill)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 6 : 7)))
+// FP64: at least 6 waves per SIMD with the component loops rolled (79 VGPRs;
+// unrolled it needs 115 at 4 waves); FP32: 8 waves (64 VGPRs, no spill)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 6 : 8)))
 k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
 {
     __shared__ double vred[8];
