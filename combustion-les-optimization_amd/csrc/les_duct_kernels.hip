@@ -39,7 +39,7 @@
 // ---------------------------------------------------------------------------
 // device-side parameters of the duct scheme (one set per slot)
 // ---------------------------------------------------------------------------
-#define DUCT_NRC 19          // host-computed reciprocal constants per run
+#define DUCT_NRC 21          // host-computed reciprocal constants per run
 
 struct DuctConsts {
     double rho_u_Tu;        // rho_unburnt * T_u  (rho = rho_u_Tu / T)
@@ -82,7 +82,9 @@ enum {
     DUCT_RC_IPRAN = 15, // 1/Pr
     DUCT_RC_IW2 = 16,   // 1/(2 w^2), w the flame-acceleration window width
     DUCT_RC_DF = 17,    // nu_u/(S_L + small): flame thickness of the SSR model
-    DUCT_RC_ITSL = 18   // 1/(tau S_L) (0 when tau S_L <= small)
+    DUCT_RC_ITSL = 18,  // 1/(tau S_L) (0 when tau S_L <= small)
+    DUCT_RC_ILY = 19,   // 1/ly, 1/lz: periodic images of the SEM eddies
+    DUCT_RC_ILZ = 20
 };
 
 template <typename real>
@@ -1077,9 +1079,9 @@ __global__ void __launch_bounds__(64) k_duct_sem_plane(Pool P)
                 const double* E = se + 7 * e;
                 double ddx = 0.0 - E[0];
                 double ddy = y - E[1];
-                ddy -= k.ly * rint(ddy / k.ly);
+                ddy -= k.ly * rint(ddy * k.rc[DUCT_RC_ILY]);     // nearest periodic image
                 double ddz = z - E[2];
-                ddz -= k.lz * rint(ddz / k.lz);
+                ddz -= k.lz * rint(ddz * k.rc[DUCT_RC_ILZ]);
                 double f = exp(-M_PI * (ddx * ddx + ddy * ddy + ddz * ddz) * inv_l2);
                 su += E[3] * f; sv += E[4] * f; sw += E[5] * f;
             }
@@ -1796,6 +1798,8 @@ static DuctConsts make_duct_consts(const les_params& p)
     k.rc[DUCT_RC_IW2] = 1.0 / (2.0 * k.accel_width * k.accel_width);
     k.rc[DUCT_RC_DF] = k.nu_unburnt / (k.s_l + LES_SMALL);
     k.rc[DUCT_RC_ITSL] = (k.tau * k.s_l > LES_SMALL) ? 1.0 / (k.tau * k.s_l) : 0.0;
+    k.rc[DUCT_RC_ILY] = 1.0 / k.ly;
+    k.rc[DUCT_RC_ILZ] = 1.0 / k.lz;
     for (int r = 0; r < DUCT_NRC; ++r) k.rcf[r] = (float)k.rc[r];
     return k;
 }
