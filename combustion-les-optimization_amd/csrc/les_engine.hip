@@ -20,6 +20,7 @@
 #include "les_engine.hpp"
 #include "les_ref_kernels.hip"
 #include "les_duct_kernels.hip"
+#include "les_duct_host.hip"
 
 // ---------------------------------------------------------------------------
 // error plumbing
