@@ -1074,54 +1074,58 @@ static int batch_config_check(const les_batch_config* cfg)
 
 extern "C" int les_batch_pool_bytes(const les_batch_config* cfg, size_t* bytes)
 {
-    if (batch_config_check(cfg) != LES_OK || !bytes) return LES_ERR_PARAM;
-    const size_t es = cfg->precision == LES_PREC_FP32 ? 4 : 8;
-    const size_t n = (size_t)cfg->pool_cells, S = (size_t)cfg->max_slots;
-    const size_t tiles = n / 256 + S * cfg->max_nx * (cfg->max_ny / 2 + cfg->max_nz / 32 + 2) + 64;
-    size_t b = 17 * n * es;                                   // A, B, F (5 each), nu_t, S_T
-    if (cfg->stats_per_cell) b += 80 * n;
-    if (cfg->precision == LES_PREC_FP32) b += 8 * n;           // FP64 staging of one run
-    b += tiles * (8 + 8 + 48 + 16 + 4);                       // rho, |u|, nu/alpha, residual partials, tile list
-    const size_t nv = cfg->max_vortices > 0 ? cfg->max_vortices : 1;
-    b += S * 8 * (7 * nv + 3 * (size_t)cfg->max_ny * cfg->max_nz + 10 * (size_t)cfg->max_nx +
-                  4 * (size_t)DUCT_NPTS * cfg->point_cap + (size_t)cfg->max_ny * ((cfg->max_nz + 255) / 256) +
-                  10 * (size_t)cfg->max_nx * STAT_CHUNKS);
-    b += S * (sizeof(SlotDesc) + sizeof(SlotDyn)) + 16 * (size_t)cfg->table_n + 8 * 6 * 1024;
-    b += S * 8 * 3 * (size_t)cfg->max_ny * cfg->max_nz * ((nv + SEM_CHUNK - 1) / SEM_CHUNK);   // SEM partials
-    *bytes = b;
-    return LES_OK;
+    return abi_guard(nullptr, [&]() -> int {
+        if (batch_config_check(cfg) != LES_OK || !bytes) return LES_ERR_PARAM;
+        const size_t es = cfg->precision == LES_PREC_FP32 ? 4 : 8;
+        const size_t n = (size_t)cfg->pool_cells, S = (size_t)cfg->max_slots;
+        const size_t tiles = n / 256 + S * cfg->max_nx * (cfg->max_ny / 2 + cfg->max_nz / 32 + 2) + 64;
+        size_t b = 17 * n * es;                                   // A, B, F (5 each), nu_t, S_T
+        if (cfg->stats_per_cell) b += 80 * n;
+        if (cfg->precision == LES_PREC_FP32) b += 8 * n;           // FP64 staging of one run
+        b += tiles * (8 + 8 + 48 + 16 + 4);                       // rho, |u|, nu/alpha, residual partials, tile list
+        const size_t nv = cfg->max_vortices > 0 ? cfg->max_vortices : 1;
+        b += S * 8 * (7 * nv + 3 * (size_t)cfg->max_ny * cfg->max_nz + 10 * (size_t)cfg->max_nx +
+                      4 * (size_t)DUCT_NPTS * cfg->point_cap + (size_t)cfg->max_ny * ((cfg->max_nz + 255) / 256) +
+                      10 * (size_t)cfg->max_nx * STAT_CHUNKS);
+        b += S * (sizeof(SlotDesc) + sizeof(SlotDyn)) + 16 * (size_t)cfg->table_n + 8 * 6 * 1024;
+        b += S * 8 * 3 * (size_t)cfg->max_ny * cfg->max_nz * ((nv + SEM_CHUNK - 1) / SEM_CHUNK);   // SEM partials
+        *bytes = b;
+        return LES_OK;
+    });
 }
 
 extern "C" int les_batch_create(const les_batch_config* cfg, const les_params* probe, les_batch** out)
 {
-    if (!out) return LES_ERR_PARAM;
-    *out = nullptr;
-    if (batch_config_check(cfg) != LES_OK) return LES_ERR_PARAM;
-    if (cfg->autotune && (!probe || les_params_validate(probe) != LES_OK || probe->mode != LES_MODE_DUCT))
-        return LES_ERR_PARAM;
-    les_batch* b = new (std::nothrow) les_batch();
-    if (!b) return LES_ERR_HIP;
-    les_gpu* c = new (std::nothrow) les_gpu();
-    if (!c) { delete b; return LES_ERR_HIP; }
-    b->c = c;
-    if (probe) c->p = *probe;
-    else les_params_default(&c->p, LES_PRESET_ENSEMBLE);
-    c->p.precision = cfg->precision;
-    les_mixture_synthetic(&c->mix);
-    int rc = LES_OK;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = LES_ERR_HIP;
-    for (int a = 0; a < 4 && rc == LES_OK; ++a)
-        if (hipEventCreate(&c->ev[a]) != hipSuccess) rc = LES_ERR_HIP;
-    if (rc == LES_OK) rc = pool_create(c, *cfg);
-    if (rc == LES_OK && cfg->autotune) rc = pool_autotune(c, *probe);
-    if (rc != LES_OK) {
-        fprintf(stderr, "les_batch_create: %s\n", c->err);
-        les_gpu_destroy(c);
-        delete b;
-        return rc;
-    }
-    *out = b;
-    return LES_OK;
+    return abi_guard(nullptr, [&]() -> int {
+        if (!out) return LES_ERR_PARAM;
+        *out = nullptr;
+        if (batch_config_check(cfg) != LES_OK) return LES_ERR_PARAM;
+        if (cfg->autotune && (!probe || les_params_validate(probe) != LES_OK || probe->mode != LES_MODE_DUCT))
+            return LES_ERR_PARAM;
+        les_batch* b = new (std::nothrow) les_batch();
+        if (!b) return LES_ERR_HIP;
+        les_gpu* c = new (std::nothrow) les_gpu();
+        if (!c) { delete b; return LES_ERR_HIP; }
+        b->c = c;
+        if (probe) c->p = *probe;
+        else les_params_default(&c->p, LES_PRESET_ENSEMBLE);
+        c->p.precision = cfg->precision;
+        les_mixture_synthetic(&c->mix);
+        int rc = LES_OK;
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = LES_ERR_HIP;
+        for (int a = 0; a < 4 && rc == LES_OK; ++a)
+            if (hipEventCreate(&c->ev[a]) != hipSuccess) rc = LES_ERR_HIP;
+        if (rc == LES_OK) rc = pool_create(c, *cfg);
+        if (rc == LES_OK && cfg->autotune) rc = pool_autotune(c, *probe);
+        if (rc != LES_OK) {
+            fprintf(stderr, "les_batch_create: %s\n", c->err);
+            les_gpu_destroy(c);
+            delete b;
+            return rc;
+        }
+        *out = b;
+        return LES_OK;
+    });
 }
 
 extern "C" void les_batch_destroy(les_batch* b)
@@ -1135,13 +1139,15 @@ extern "C" const char* les_batch_last_error(const les_batch* b) { return b && b-
 
 extern "C" int les_batch_tuning(const les_batch* b, int* tile_k, int* grid_blocks, int* xcd_map, double* tuned_ms)
 {
-    if (!b) return LES_ERR_PARAM;
-    const DuctState* d = b->c->d;
-    if (tile_k) *tile_k = d->tk;
-    if (grid_blocks) *grid_blocks = d->grid;
-    if (xcd_map) *xcd_map = d->xcd_map;
-    if (tuned_ms) *tuned_ms = d->tuned_ms;
-    return LES_OK;
+    return abi_guard(b ? b->c : nullptr, [&]() -> int {
+        if (!b) return LES_ERR_PARAM;
+        const DuctState* d = b->c->d;
+        if (tile_k) *tile_k = d->tk;
+        if (grid_blocks) *grid_blocks = d->grid;
+        if (xcd_map) *xcd_map = d->xcd_map;
+        if (tuned_ms) *tuned_ms = d->tuned_ms;
+        return LES_OK;
+    });
 }
 
 static int batch_slot_ok(les_batch* b, int slot)
@@ -1155,136 +1161,158 @@ static int batch_slot_ok(les_batch* b, int slot)
 
 extern "C" int les_batch_admit(les_batch* b, const les_params* p, int n_steps, int init, int* slot)
 {
-    if (!b || !p || !slot || n_steps < 0) return LES_ERR_PARAM;
-    les_gpu* c = b->c;
-    *slot = -1;
-    if (les_params_validate(p) != LES_OK || p->mode != LES_MODE_DUCT) {
-        snprintf(c->err, sizeof(c->err), "batched runs are duct-mode runs with valid parameters");
-        return LES_ERR_PARAM;
-    }
-    const DuctState* d = c->d;
-    if (p->table_t0 != c->p.table_t0 || p->table_dt != c->p.table_dt || p->table_n != c->p.table_n ||
-        (p->stats_per_cell && !d->stats_cells)) {
-        snprintf(c->err, sizeof(c->err), "run needs a property table or per-cell statistics the pool lacks");
-        return LES_ERR_ADMISSION;
-    }
-    int e = pool_admit(c, *p, n_steps, slot);
-    if (e == LES_ERR_ADMISSION) {
-        snprintf(c->err, sizeof(c->err), "no free slot, cells or tiles for a %dx%dx%d run", p->nx, p->ny, p->nz);
-        return e;
-    }
-    if (e != LES_OK) return e;
-    if (init) {
-        e = slot_init(c, *slot, *p);
-        if (e != LES_OK) { (void)pool_release(c, *slot); *slot = -1; return e; }
-    }
-    return LES_OK;
+    return abi_guard(b ? b->c : nullptr, [&]() -> int {
+        if (!b || !p || !slot || n_steps < 0) return LES_ERR_PARAM;
+        les_gpu* c = b->c;
+        *slot = -1;
+        if (les_params_validate(p) != LES_OK || p->mode != LES_MODE_DUCT) {
+            snprintf(c->err, sizeof(c->err), "batched runs are duct-mode runs with valid parameters");
+            return LES_ERR_PARAM;
+        }
+        const DuctState* d = c->d;
+        if (p->table_t0 != c->p.table_t0 || p->table_dt != c->p.table_dt || p->table_n != c->p.table_n ||
+            (p->stats_per_cell && !d->stats_cells)) {
+            snprintf(c->err, sizeof(c->err), "run needs a property table or per-cell statistics the pool lacks");
+            return LES_ERR_ADMISSION;
+        }
+        int e = pool_admit(c, *p, n_steps, slot);
+        if (e == LES_ERR_ADMISSION) {
+            snprintf(c->err, sizeof(c->err), "no free slot, cells or tiles for a %dx%dx%d run", p->nx, p->ny, p->nz);
+            return e;
+        }
+        if (e != LES_OK) return e;
+        if (init) {
+            e = slot_init(c, *slot, *p);
+            if (e != LES_OK) { (void)pool_release(c, *slot); *slot = -1; return e; }
+        }
+        return LES_OK;
+    });
 }
 
 extern "C" int les_batch_release(les_batch* b, int slot)
 {
-    if (!batch_slot_ok(b, slot)) return LES_ERR_PARAM;
-    return pool_release(b->c, slot);
+    return abi_guard(b ? b->c : nullptr, [&]() -> int {
+        if (!batch_slot_ok(b, slot)) return LES_ERR_PARAM;
+        return pool_release(b->c, slot);
+    });
 }
 
 extern "C" int les_batch_step(les_batch* b, int n_steps, double* ms_per_step)
 {
-    if (!b || n_steps < 0) return LES_ERR_PARAM;
-    les_gpu* c = b->c;
-    DHIPC(hipEventRecord(c->ev[2], c->stream));
-    int e = pool_steps(c, n_steps);
-    if (e != LES_OK) return e;
-    DHIPC(hipEventRecord(c->ev[3], c->stream));
-    DHIPC(hipEventSynchronize(c->ev[3]));
-    float ms = 0.f;
-    DHIPC(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-    c->step_ms = n_steps > 0 ? ms / n_steps : 0.0;
-    if (ms_per_step) *ms_per_step = c->step_ms;
-    return LES_OK;
+    return abi_guard(b ? b->c : nullptr, [&]() -> int {
+        if (!b || n_steps < 0) return LES_ERR_PARAM;
+        les_gpu* c = b->c;
+        DHIPC(hipEventRecord(c->ev[2], c->stream));
+        int e = pool_steps(c, n_steps);
+        if (e != LES_OK) return e;
+        DHIPC(hipEventRecord(c->ev[3], c->stream));
+        DHIPC(hipEventSynchronize(c->ev[3]));
+        float ms = 0.f;
+        DHIPC(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+        c->step_ms = n_steps > 0 ? ms / n_steps : 0.0;
+        if (ms_per_step) *ms_per_step = c->step_ms;
+        return LES_OK;
+    });
 }
 
 extern "C" int les_batch_slot_info(les_batch* b, int slot, les_slot_info* info)
 {
-    if (!b || !info || slot < 0 || slot >= b->c->d->max_slots) return LES_ERR_PARAM;
-    les_gpu* c = b->c;
-    DuctState* d = c->d;
-    memset(info, 0, sizeof(*info));
-    if (!d->taken[slot]) return LES_OK;
-    const SlotDesc& D = d->desc[slot];
-    SlotDyn y;
-    DHIPC(hipMemcpyAsync(&y, d->d_dyn + slot, sizeof(y), hipMemcpyDeviceToHost, c->stream));
-    DHIPC(hipStreamSynchronize(c->stream));
-    info->active = D.active;
-    info->done = y.done;
-    info->step = y.step;
-    info->n_steps = D.n_steps;
-    info->time = y.time;
-    info->dt = y.dt;
-    info->max_velocity = y.maxvel;
-    info->nx = D.g.nx; info->ny = D.g.ny; info->nz = D.g.nz;
-    info->cells = D.g.n;
-    return LES_OK;
+    return abi_guard(b ? b->c : nullptr, [&]() -> int {
+        if (!b || !info || slot < 0 || slot >= b->c->d->max_slots) return LES_ERR_PARAM;
+        les_gpu* c = b->c;
+        DuctState* d = c->d;
+        memset(info, 0, sizeof(*info));
+        if (!d->taken[slot]) return LES_OK;
+        const SlotDesc& D = d->desc[slot];
+        SlotDyn y;
+        DHIPC(hipMemcpyAsync(&y, d->d_dyn + slot, sizeof(y), hipMemcpyDeviceToHost, c->stream));
+        DHIPC(hipStreamSynchronize(c->stream));
+        info->active = D.active;
+        info->done = y.done;
+        info->step = y.step;
+        info->n_steps = D.n_steps;
+        info->time = y.time;
+        info->dt = y.dt;
+        info->max_velocity = y.maxvel;
+        info->nx = D.g.nx; info->ny = D.g.ny; info->nz = D.g.nz;
+        info->cells = D.g.n;
+        return LES_OK;
+    });
 }
 
 extern "C" int les_batch_poll(les_batch* b, int* slots, int max_slots, int* n_done)
 {
-    if (!b || !n_done || (max_slots > 0 && !slots)) return LES_ERR_PARAM;
-    const DuctState* d = b->c->d;
-    int n = 0;
-    for (int s = 0; s < d->max_slots; ++s) {
-        const SlotDesc& D = d->desc[s];
-        if (d->taken[s] && D.active && D.n_steps > 0 && d->host_step[s] >= D.n_steps) {
-            if (n < max_slots) slots[n] = s;
-            ++n;
+    return abi_guard(b ? b->c : nullptr, [&]() -> int {
+        if (!b || !n_done || (max_slots > 0 && !slots)) return LES_ERR_PARAM;
+        const DuctState* d = b->c->d;
+        int n = 0;
+        for (int s = 0; s < d->max_slots; ++s) {
+            const SlotDesc& D = d->desc[s];
+            if (d->taken[s] && D.active && D.n_steps > 0 && d->host_step[s] >= D.n_steps) {
+                if (n < max_slots) slots[n] = s;
+                ++n;
+            }
         }
-    }
-    *n_done = n;
-    return LES_OK;
+        *n_done = n;
+        return LES_OK;
+    });
 }
 
 extern "C" int les_batch_upload(les_batch* b, int slot, const double* const* fields, int n_fields)
 {
-    if (!batch_slot_ok(b, slot) || !fields) return LES_ERR_PARAM;
-    return slot_upload(b->c, slot, fields, n_fields);
+    return abi_guard(b ? b->c : nullptr, [&]() -> int {
+        if (!batch_slot_ok(b, slot) || !fields) return LES_ERR_PARAM;
+        return slot_upload(b->c, slot, fields, n_fields);
+    });
 }
 
 extern "C" int les_batch_download(les_batch* b, int slot, double* const* fields, int n_fields)
 {
-    if (!batch_slot_ok(b, slot) || !fields) return LES_ERR_PARAM;
-    return slot_download(b->c, slot, fields, n_fields);
+    return abi_guard(b ? b->c : nullptr, [&]() -> int {
+        if (!batch_slot_ok(b, slot) || !fields) return LES_ERR_PARAM;
+        return slot_download(b->c, slot, fields, n_fields);
+    });
 }
 
 extern "C" int les_batch_download_field(les_batch* b, int slot, const char* name, double* out)
 {
-    if (!batch_slot_ok(b, slot) || !name || !out) return LES_ERR_PARAM;
-    return slot_download_field(b->c, slot, name, out);
+    return abi_guard(b ? b->c : nullptr, [&]() -> int {
+        if (!batch_slot_ok(b, slot) || !name || !out) return LES_ERR_PARAM;
+        return slot_download_field(b->c, slot, name, out);
+    });
 }
 
 extern "C" int les_batch_report(les_batch* b, int slot, les_step_report* rep)
 {
-    if (!batch_slot_ok(b, slot) || !rep) return LES_ERR_PARAM;
-    return slot_monitor(b->c, slot, rep);
+    return abi_guard(b ? b->c : nullptr, [&]() -> int {
+        if (!batch_slot_ok(b, slot) || !rep) return LES_ERR_PARAM;
+        return slot_monitor(b->c, slot, rep);
+    });
 }
 
 extern "C" int les_batch_probe_fetch(les_batch* b, int slot, les_probe_row* rows, int max_rows, int* n_rows)
 {
-    if (!batch_slot_ok(b, slot) || !rows || !n_rows || max_rows < 0) return LES_ERR_PARAM;
-    les_gpu* c = b->c;
-    DuctState* d = c->d;
-    std::vector<les_probe_row>& v = d->probe_rows[slot];
-    int e = slot_probe_drain(c, slot, d->probe_x[slot].data(), v);
-    if (e != LES_OK) return e;
-    const int n = (int)v.size() < max_rows ? (int)v.size() : max_rows;
-    if (n > 0) memcpy(rows, v.data(), (size_t)n * sizeof(les_probe_row));
-    v.erase(v.begin(), v.begin() + n);
-    *n_rows = n;
-    return LES_OK;
+    return abi_guard(b ? b->c : nullptr, [&]() -> int {
+        if (!batch_slot_ok(b, slot) || !rows || !n_rows || max_rows < 0) return LES_ERR_PARAM;
+        les_gpu* c = b->c;
+        DuctState* d = c->d;
+        std::vector<les_probe_row>& v = d->probe_rows[slot];
+        int e = slot_probe_drain(c, slot, d->probe_x[slot].data(), v);
+        if (e != LES_OK) return e;
+        const int n = (int)v.size() < max_rows ? (int)v.size() : max_rows;
+        if (n > 0) memcpy(rows, v.data(), (size_t)n * sizeof(les_probe_row));
+        v.erase(v.begin(), v.begin() + n);
+        *n_rows = n;
+        return LES_OK;
+    });
 }
 
 extern "C" int les_batch_point_fetch(les_batch* b, int slot, double* out, int max_steps, int* n_steps)
 {
-    if (!batch_slot_ok(b, slot) || !out || !n_steps || max_steps < 0) return LES_ERR_PARAM;
-    return slot_point_fetch(b->c, slot, out, max_steps, n_steps);
+    return abi_guard(b ? b->c : nullptr, [&]() -> int {
+        if (!batch_slot_ok(b, slot) || !out || !n_steps || max_steps < 0) return LES_ERR_PARAM;
+        return slot_point_fetch(b->c, slot, out, max_steps, n_steps);
+    });
 }
 
 // ---------------------------------------------------------------------------

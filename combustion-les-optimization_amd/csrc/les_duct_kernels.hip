@@ -306,7 +306,7 @@ __device__ __forceinline__ real face_minus(int scheme, const real* v, bool posit
 
 
 // ---------------------------------------------------------------------------
-// persistent tile loop (XCD-aware) and tile decoding
+// tile-to-block mapping (XCD-aware) and tile decoding
 // ---------------------------------------------------------------------------
 #define DUCT_CAS __attribute__((address_space(4)))
 

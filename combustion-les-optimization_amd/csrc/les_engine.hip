@@ -269,50 +269,52 @@ static int create_ref(les_gpu* c)
 
 extern "C" int les_gpu_create(const les_params* p, const les_mixture* mix, les_gpu** out)
 {
-    if (!p || !mix || !out) return LES_ERR_PARAM;
-    *out = nullptr;
-    int v = les_params_validate(p);
-    if (v != LES_OK) return v;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return LES_ERR_HIP;
-    les_gpu* c = new (std::nothrow) les_gpu();
-    if (!c) return LES_ERR_HIP;
-    c->p = *p;
-    c->mix = *mix;
-    c->g.nx = p->nx; c->g.ny = p->ny; c->g.nz = p->nz;
-    c->g.n = (long long)p->nx * p->ny * p->nz;
-    c->g.sx = (long long)p->ny * p->nz;
-    c->g.sy = p->nz;
-    c->g.dx = p->lx / (double)p->nx;
-    c->g.dy = p->ly / (double)p->ny;
-    c->g.dz = p->lz / (double)p->nz;
-    memset(&c->rd, 0, sizeof(c->rd));
-    initstate_r((unsigned)p->seed, c->rstate, sizeof(c->rstate), &c->rd);
-    composition(c);
-    int rc = LES_OK;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = LES_ERR_HIP;
-    if (rc == LES_OK) {
-        void* m = nullptr;
-        if (hipMalloc(&m, sizeof(les_mixture)) != hipSuccess) rc = LES_ERR_HIP;
-        else {
-            c->allocs.push_back(m);
-            c->d_mix = (les_mixture*)m;
-            if (hipMemcpy(m, mix, sizeof(les_mixture), hipMemcpyHostToDevice) != hipSuccess) rc = LES_ERR_HIP;
+    return abi_guard(nullptr, [&]() -> int {
+        if (!p || !mix || !out) return LES_ERR_PARAM;
+        *out = nullptr;
+        int v = les_params_validate(p);
+        if (v != LES_OK) return v;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return LES_ERR_HIP;
+        les_gpu* c = new (std::nothrow) les_gpu();
+        if (!c) return LES_ERR_HIP;
+        c->p = *p;
+        c->mix = *mix;
+        c->g.nx = p->nx; c->g.ny = p->ny; c->g.nz = p->nz;
+        c->g.n = (long long)p->nx * p->ny * p->nz;
+        c->g.sx = (long long)p->ny * p->nz;
+        c->g.sy = p->nz;
+        c->g.dx = p->lx / (double)p->nx;
+        c->g.dy = p->ly / (double)p->ny;
+        c->g.dz = p->lz / (double)p->nz;
+        memset(&c->rd, 0, sizeof(c->rd));
+        initstate_r((unsigned)p->seed, c->rstate, sizeof(c->rstate), &c->rd);
+        composition(c);
+        int rc = LES_OK;
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = LES_ERR_HIP;
+        if (rc == LES_OK) {
+            void* m = nullptr;
+            if (hipMalloc(&m, sizeof(les_mixture)) != hipSuccess) rc = LES_ERR_HIP;
+            else {
+                c->allocs.push_back(m);
+                c->d_mix = (les_mixture*)m;
+                if (hipMemcpy(m, mix, sizeof(les_mixture), hipMemcpyHostToDevice) != hipSuccess) rc = LES_ERR_HIP;
+            }
         }
-    }
-    if (rc == LES_OK) {
-        if (hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess ||
-            hipEventCreate(&c->ev[2]) != hipSuccess || hipEventCreate(&c->ev[3]) != hipSuccess)
-            rc = LES_ERR_HIP;
-    }
-    if (rc == LES_OK) rc = (p->mode == LES_MODE_REFERENCE) ? create_ref(c) : duct_create(c);
-    if (rc != LES_OK) {
-        fprintf(stderr, "les_gpu_create: %s\n", c->err);
-        les_gpu_destroy(c);
-        return rc;
-    }
-    *out = c;
-    return LES_OK;
+        if (rc == LES_OK) {
+            if (hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess ||
+                hipEventCreate(&c->ev[2]) != hipSuccess || hipEventCreate(&c->ev[3]) != hipSuccess)
+                rc = LES_ERR_HIP;
+        }
+        if (rc == LES_OK) rc = (p->mode == LES_MODE_REFERENCE) ? create_ref(c) : duct_create(c);
+        if (rc != LES_OK) {
+            fprintf(stderr, "les_gpu_create: %s\n", c->err);
+            les_gpu_destroy(c);
+            return rc;
+        }
+        *out = c;
+        return LES_OK;
+    });
 }
 
 extern "C" void les_gpu_destroy(les_gpu* c)
@@ -537,64 +539,66 @@ int monitor(les_gpu* c, const double* rho, const double* T, const double* G, con
 // ---------------------------------------------------------------------------
 extern "C" int les_gpu_init_state(les_gpu* c)
 {
-    if (!c) return LES_ERR_PARAM;
-    if (c->p.mode == LES_MODE_DUCT) return duct_init_state(c);
-    // initial fields of a new run (setting :1184-1261) on the host, then upload
-    const size_t n = (size_t)c->g.n;
-    const int ny = c->p.ny, nz = c->p.nz;
-    std::vector<double> f(n * LES_NFIELDS);
-    double* F[LES_NFIELDS];
-    for (int a = 0; a < LES_NFIELDS; ++a) F[a] = &f[a * n];
-    for (size_t q = 0; q < n; ++q) {
-        int a = c->attr_h[q];
-        int i = (int)(q / ((size_t)ny * nz));
-        double Yc[5] = {0, 0, 0, 0, 0}, u = 0.0, T = 0.0, G = 0.0;
-        if (a == 1) {
-            u = c->p.inlet_velocity; T = c->p.inlet_temperature;
-            for (int s = 0; s < 5; ++s) Yc[s] = c->inletY[s];
-        } else if (a == 4 || a == 2) {
-            T = c->p.ambient_temperature;
-            for (int s = 0; s < 5; ++s) Yc[s] = c->ambY[s];
-        } else if (a == 5 || a == 3) {
-            if (i <= c->p.initial_flame_front_i) {
-                double cons = c->inletY[0];
-                G = 1.0; T = c->p.burnt_temperature;
-                Yc[0] = c->inletY[0] - cons;
-                Yc[1] = c->inletY[1] - cons * c->st_o2;
-                Yc[2] = c->inletY[2];
-                Yc[3] = c->inletY[3] + cons * c->st_co2;
-                Yc[4] = c->inletY[4] + cons * c->st_h2o;
-            } else {
-                T = c->p.inlet_temperature;
+    return abi_guard(c, [&]() -> int {
+        if (!c) return LES_ERR_PARAM;
+        if (c->p.mode == LES_MODE_DUCT) return duct_init_state(c);
+        // initial fields of a new run (setting :1184-1261) on the host, then upload
+        const size_t n = (size_t)c->g.n;
+        const int ny = c->p.ny, nz = c->p.nz;
+        std::vector<double> f(n * LES_NFIELDS);
+        double* F[LES_NFIELDS];
+        for (int a = 0; a < LES_NFIELDS; ++a) F[a] = &f[a * n];
+        for (size_t q = 0; q < n; ++q) {
+            int a = c->attr_h[q];
+            int i = (int)(q / ((size_t)ny * nz));
+            double Yc[5] = {0, 0, 0, 0, 0}, u = 0.0, T = 0.0, G = 0.0;
+            if (a == 1) {
+                u = c->p.inlet_velocity; T = c->p.inlet_temperature;
                 for (int s = 0; s < 5; ++s) Yc[s] = c->inletY[s];
+            } else if (a == 4 || a == 2) {
+                T = c->p.ambient_temperature;
+                for (int s = 0; s < 5; ++s) Yc[s] = c->ambY[s];
+            } else if (a == 5 || a == 3) {
+                if (i <= c->p.initial_flame_front_i) {
+                    double cons = c->inletY[0];
+                    G = 1.0; T = c->p.burnt_temperature;
+                    Yc[0] = c->inletY[0] - cons;
+                    Yc[1] = c->inletY[1] - cons * c->st_o2;
+                    Yc[2] = c->inletY[2];
+                    Yc[3] = c->inletY[3] + cons * c->st_co2;
+                    Yc[4] = c->inletY[4] + cons * c->st_h2o;
+                } else {
+                    T = c->p.inlet_temperature;
+                    for (int s = 0; s < 5; ++s) Yc[s] = c->inletY[s];
+                }
             }
+            F[0][q] = u; F[1][q] = 0.0; F[2][q] = 0.0; F[3][q] = T; F[4][q] = G;
+            double sy = 0.0;
+            for (int s = 0; s < 5; ++s) sy += Yc[s] / (c->mix.sp[s].mw / 1000.0);
+            if (sy > LES_SMALL && T > 1.0) {
+                double wmix = 1.0 / sy;
+                double rmix = LES_R_UNIV / wmix;
+                F[5][q] = LES_P_CONST / (rmix * T);
+            } else F[5][q] = 1.2;
+            for (int s = 0; s < 5; ++s) F[6 + s][q] = Yc[s];
         }
-        F[0][q] = u; F[1][q] = 0.0; F[2][q] = 0.0; F[3][q] = T; F[4][q] = G;
-        double sy = 0.0;
-        for (int s = 0; s < 5; ++s) sy += Yc[s] / (c->mix.sp[s].mw / 1000.0);
-        if (sy > LES_SMALL && T > 1.0) {
-            double wmix = 1.0 / sy;
-            double rmix = LES_R_UNIV / wmix;
-            F[5][q] = LES_P_CONST / (rmix * T);
-        } else F[5][q] = 1.2;
-        for (int s = 0; s < 5; ++s) F[6 + s][q] = Yc[s];
-    }
-    int e = les_gpu_upload_state(c, (const double* const*)F, LES_NFIELDS);
-    if (e != LES_OK) return e;
-    R& r = c->r;
-    size_t b = n * sizeof(double);
-    HIPC(hipMemsetAsync(r.nu, 0, b, c->stream));
-    HIPC(hipMemsetAsync(r.alpha, 0, b, c->stream));
-    HIPC(hipMemsetAsync(r.nu_t, 0, b, c->stream));
-    HIPC(hipMemsetAsync(r.S_mag, 0, b, c->stream));
-    HIPC(hipMemsetAsync(r.rhot, 0, b, c->stream));
-    for (int a = 0; a < 10; ++a) HIPC(hipMemsetAsync(r.stat[a], 0, b, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
-    init_vortices(c);
-    c->step = 0;
-    c->stat_count = 0;
-    c->probe.clear();
-    return LES_OK;
+        int e = les_gpu_upload_state(c, (const double* const*)F, LES_NFIELDS);
+        if (e != LES_OK) return e;
+        R& r = c->r;
+        size_t b = n * sizeof(double);
+        HIPC(hipMemsetAsync(r.nu, 0, b, c->stream));
+        HIPC(hipMemsetAsync(r.alpha, 0, b, c->stream));
+        HIPC(hipMemsetAsync(r.nu_t, 0, b, c->stream));
+        HIPC(hipMemsetAsync(r.S_mag, 0, b, c->stream));
+        HIPC(hipMemsetAsync(r.rhot, 0, b, c->stream));
+        for (int a = 0; a < 10; ++a) HIPC(hipMemsetAsync(r.stat[a], 0, b, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        init_vortices(c);
+        c->step = 0;
+        c->stat_count = 0;
+        c->probe.clear();
+        return LES_OK;
+    });
 }
 
 __global__ void k_momentum_from_prim(long long n, const double* rho, const double* u, const double* v,
@@ -607,204 +611,228 @@ __global__ void k_momentum_from_prim(long long n, const double* rho, const doubl
 
 extern "C" int les_gpu_upload_state(les_gpu* c, const double* const* f, int nf)
 {
-    if (!c || !f) return LES_ERR_PARAM;
-    if (c->p.mode == LES_MODE_DUCT) return duct_upload_state(c, f, nf);
-    if (nf < LES_NFIELDS) return set_err(c, LES_ERR_PARAM, "need %d fields", LES_NFIELDS);
-    R& r = c->r;
-    size_t b = (size_t)c->g.n * sizeof(double);
-    double* dst[LES_NFIELDS] = {r.u, r.v, r.w, r.T, r.G, r.rho, r.Y[0], r.Y[1], r.Y[2], r.Y[3], r.Y[4]};
-    for (int a = 0; a < LES_NFIELDS; ++a) HIPC(hipMemcpyAsync(dst[a], f[a], b, hipMemcpyHostToDevice, c->stream));
-    int nb = (int)((c->g.n + 255) / 256);
-    hipLaunchKernelGGL(k_momentum_from_prim, dim3(nb), dim3(256), 0, c->stream, c->g.n, r.rho, r.u, r.v, r.w, r.rhou,
-                       r.rhov, r.rhow);
-    LAUNCH_CHECK();
-    HIPC(hipStreamSynchronize(c->stream));
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c || !f) return LES_ERR_PARAM;
+        if (c->p.mode == LES_MODE_DUCT) return duct_upload_state(c, f, nf);
+        if (nf < LES_NFIELDS) return set_err(c, LES_ERR_PARAM, "need %d fields", LES_NFIELDS);
+        R& r = c->r;
+        size_t b = (size_t)c->g.n * sizeof(double);
+        double* dst[LES_NFIELDS] = {r.u, r.v, r.w, r.T, r.G, r.rho, r.Y[0], r.Y[1], r.Y[2], r.Y[3], r.Y[4]};
+        for (int a = 0; a < LES_NFIELDS; ++a) HIPC(hipMemcpyAsync(dst[a], f[a], b, hipMemcpyHostToDevice, c->stream));
+        int nb = (int)((c->g.n + 255) / 256);
+        hipLaunchKernelGGL(k_momentum_from_prim, dim3(nb), dim3(256), 0, c->stream, c->g.n, r.rho, r.u, r.v, r.w, r.rhou,
+                           r.rhov, r.rhow);
+        LAUNCH_CHECK();
+        HIPC(hipStreamSynchronize(c->stream));
+        return LES_OK;
+    });
 }
 
 extern "C" int les_gpu_download_state(les_gpu* c, double* const* f, int nf)
 {
-    if (!c || !f) return LES_ERR_PARAM;
-    if (c->p.mode == LES_MODE_DUCT) return duct_download_state(c, f, nf);
-    R& r = c->r;
-    size_t b = (size_t)c->g.n * sizeof(double);
-    const double* src[LES_NFIELDS] = {r.u, r.v, r.w, r.T, r.G, r.rho, r.Y[0], r.Y[1], r.Y[2], r.Y[3], r.Y[4]};
-    for (int a = 0; a < nf && a < LES_NFIELDS; ++a)
-        HIPC(hipMemcpyAsync(f[a], src[a], b, hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c || !f) return LES_ERR_PARAM;
+        if (c->p.mode == LES_MODE_DUCT) return duct_download_state(c, f, nf);
+        R& r = c->r;
+        size_t b = (size_t)c->g.n * sizeof(double);
+        const double* src[LES_NFIELDS] = {r.u, r.v, r.w, r.T, r.G, r.rho, r.Y[0], r.Y[1], r.Y[2], r.Y[3], r.Y[4]};
+        for (int a = 0; a < nf && a < LES_NFIELDS; ++a)
+            HIPC(hipMemcpyAsync(f[a], src[a], b, hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        return LES_OK;
+    });
 }
 
 extern "C" int les_gpu_download_field(les_gpu* c, const char* name, double* out)
 {
-    if (!c || !name || !out) return LES_ERR_PARAM;
-    if (c->p.mode == LES_MODE_DUCT) return duct_download_field(c, name, out);
-    R& r = c->r;
-    size_t n = (size_t)c->g.n;
-    if (strcmp(name, "attribute") == 0) {
-        for (size_t q = 0; q < n; ++q) out[q] = (double)c->attr_h[q];
-        return LES_OK;
-    }
-    struct { const char* nm; const double* p; } tab[] = {
-        {"nu_t", r.nu_t}, {"S_mag", r.S_mag}, {"nu", r.nu}, {"alpha", r.alpha}, {"mu_mix", r.mu_mix},
-        {"cp_mix", r.cp}, {"lambda_mix", r.lam}, {"ST", r.ST}, {"gradG", r.gradG},
-        {"u_f", r.uf}, {"v_f", r.vf}, {"w_f", r.wf}, {"T_f", r.Tf}, {"G_f", r.Gf}, {"rho_f", r.rhof},
-        {"rho_temp", r.rhot}, {"rhou", r.rhou}, {"rhov", r.rhov}, {"rhow", r.rhow},
-        {"D0", r.Dk[0]}, {"D1", r.Dk[1]}, {"D2", r.Dk[2]}, {"D3", r.Dk[3]}, {"D4", r.Dk[4]},
-        {"inlet", r.inlet}};
-    for (auto& t : tab) {
-        if (strcmp(t.nm, name) == 0) {
-            size_t cnt = strcmp(name, "inlet") == 0 ? 3 * (size_t)c->p.ny * c->p.nz : n;
-            HIPC(hipMemcpyAsync(out, t.p, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return abi_guard(c, [&]() -> int {
+        if (!c || !name || !out) return LES_ERR_PARAM;
+        if (c->p.mode == LES_MODE_DUCT) return duct_download_field(c, name, out);
+        R& r = c->r;
+        size_t n = (size_t)c->g.n;
+        if (strcmp(name, "attribute") == 0) {
+            for (size_t q = 0; q < n; ++q) out[q] = (double)c->attr_h[q];
+            return LES_OK;
+        }
+        struct { const char* nm; const double* p; } tab[] = {
+            {"nu_t", r.nu_t}, {"S_mag", r.S_mag}, {"nu", r.nu}, {"alpha", r.alpha}, {"mu_mix", r.mu_mix},
+            {"cp_mix", r.cp}, {"lambda_mix", r.lam}, {"ST", r.ST}, {"gradG", r.gradG},
+            {"u_f", r.uf}, {"v_f", r.vf}, {"w_f", r.wf}, {"T_f", r.Tf}, {"G_f", r.Gf}, {"rho_f", r.rhof},
+            {"rho_temp", r.rhot}, {"rhou", r.rhou}, {"rhov", r.rhov}, {"rhow", r.rhow},
+            {"D0", r.Dk[0]}, {"D1", r.Dk[1]}, {"D2", r.Dk[2]}, {"D3", r.Dk[3]}, {"D4", r.Dk[4]},
+            {"inlet", r.inlet}};
+        for (auto& t : tab) {
+            if (strcmp(t.nm, name) == 0) {
+                size_t cnt = strcmp(name, "inlet") == 0 ? 3 * (size_t)c->p.ny * c->p.nz : n;
+                HIPC(hipMemcpyAsync(out, t.p, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                HIPC(hipStreamSynchronize(c->stream));
+                return LES_OK;
+            }
+        }
+        if (strncmp(name, "stat", 4) == 0) {
+            int a = atoi(name + 4);
+            if (a < 0 || a > 9) return LES_ERR_PARAM;
+            HIPC(hipMemcpyAsync(out, r.stat[a], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
             HIPC(hipStreamSynchronize(c->stream));
             return LES_OK;
         }
-    }
-    if (strncmp(name, "stat", 4) == 0) {
-        int a = atoi(name + 4);
-        if (a < 0 || a > 9) return LES_ERR_PARAM;
-        HIPC(hipMemcpyAsync(out, r.stat[a], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPC(hipStreamSynchronize(c->stream));
-        return LES_OK;
-    }
-    return set_err(c, LES_ERR_PARAM, "unknown field %s", name);
+        return set_err(c, LES_ERR_PARAM, "unknown field %s", name);
+    });
 }
 
 extern "C" int les_gpu_step(les_gpu* c, int n_steps, les_step_report* rep)
 {
-    if (!c || n_steps < 0) return LES_ERR_PARAM;
-    c->rhs_ms_sum = 0.0;
-    c->rhs_launches = 0;
-    if (c->p.mode == LES_MODE_DUCT) return duct_step(c, n_steps, rep);
-    HIPC(hipEventRecord(c->ev[2], c->stream));
-    les_step_report tmp;
-    for (int s = 0; s < n_steps; ++s) {
-        int e = ref_step(c, rep ? rep : &tmp, s == n_steps - 1);
-        if (e != LES_OK) return e;
-    }
-    HIPC(hipEventRecord(c->ev[3], c->stream));
-    HIPC(hipEventSynchronize(c->ev[3]));
-    float ms = 0.f;
-    HIPC(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-    c->step_ms = n_steps > 0 ? ms / n_steps : 0.0;
-    if (rep && rep->nonfinite) return set_err(c, LES_ERR_NAN, "non-finite value in the state at step %d", rep->step);
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c || n_steps < 0) return LES_ERR_PARAM;
+        c->rhs_ms_sum = 0.0;
+        c->rhs_launches = 0;
+        if (c->p.mode == LES_MODE_DUCT) return duct_step(c, n_steps, rep);
+        HIPC(hipEventRecord(c->ev[2], c->stream));
+        les_step_report tmp;
+        for (int s = 0; s < n_steps; ++s) {
+            int e = ref_step(c, rep ? rep : &tmp, s == n_steps - 1);
+            if (e != LES_OK) return e;
+        }
+        HIPC(hipEventRecord(c->ev[3], c->stream));
+        HIPC(hipEventSynchronize(c->ev[3]));
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+        c->step_ms = n_steps > 0 ? ms / n_steps : 0.0;
+        if (rep && rep->nonfinite) return set_err(c, LES_ERR_NAN, "non-finite value in the state at step %d", rep->step);
+        return LES_OK;
+    });
 }
 
 extern "C" int les_gpu_rhs(les_gpu* c, int t_loop, double* const* out, int n_out)
 {
-    if (!c || !out || n_out < LES_NRHS) return LES_ERR_PARAM;
-    if (c->p.mode == LES_MODE_DUCT) return duct_rhs(c, out, n_out);
-    int e = ref_rhs(c, false, 0, t_loop);
-    if (e != LES_OK) return e;
-    size_t b = (size_t)c->g.n * sizeof(double);
-    for (int f = 0; f < LES_NRHS; ++f) HIPC(hipMemcpyAsync(out[f], c->r.k[0][f], b, hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c || !out || n_out < LES_NRHS) return LES_ERR_PARAM;
+        if (c->p.mode == LES_MODE_DUCT) return duct_rhs(c, out, n_out);
+        int e = ref_rhs(c, false, 0, t_loop);
+        if (e != LES_OK) return e;
+        size_t b = (size_t)c->g.n * sizeof(double);
+        for (int f = 0; f < LES_NRHS; ++f) HIPC(hipMemcpyAsync(out[f], c->r.k[0][f], b, hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        return LES_OK;
+    });
 }
 
 extern "C" int les_gpu_probe_fetch(les_gpu* c, les_probe_row* rows, int max_rows, int* n_rows)
 {
-    if (!c || !rows || !n_rows || max_rows < 0) return LES_ERR_PARAM;
-    int n = (int)c->probe.size() < max_rows ? (int)c->probe.size() : max_rows;
-    if (n > 0) memcpy(rows, c->probe.data(), (size_t)n * sizeof(les_probe_row));
-    c->probe.erase(c->probe.begin(), c->probe.begin() + n);
-    *n_rows = n;
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c || !rows || !n_rows || max_rows < 0) return LES_ERR_PARAM;
+        int n = (int)c->probe.size() < max_rows ? (int)c->probe.size() : max_rows;
+        if (n > 0) memcpy(rows, c->probe.data(), (size_t)n * sizeof(les_probe_row));
+        c->probe.erase(c->probe.begin(), c->probe.begin() + n);
+        *n_rows = n;
+        return LES_OK;
+    });
 }
 
 extern "C" int les_gpu_set_params(les_gpu* c, const les_params* p)
 {
-    if (!c || !p) return LES_ERR_PARAM;
-    if (p->nx != c->p.nx || p->ny != c->p.ny || p->nz != c->p.nz || p->mode != c->p.mode ||
-        p->num_vortices != c->p.num_vortices || p->precision != c->p.precision)
-        return set_err(c, LES_ERR_PARAM, "grid, mode, precision and eddy count are fixed per context");
-    if (les_params_validate(p) != LES_OK) return LES_ERR_PARAM;
-    c->p = *p;
-    if (c->p.mode == LES_MODE_DUCT) duct_params_changed(c);
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c || !p) return LES_ERR_PARAM;
+        if (p->nx != c->p.nx || p->ny != c->p.ny || p->nz != c->p.nz || p->mode != c->p.mode ||
+            p->num_vortices != c->p.num_vortices || p->precision != c->p.precision)
+            return set_err(c, LES_ERR_PARAM, "grid, mode, precision and eddy count are fixed per context");
+        if (les_params_validate(p) != LES_OK) return LES_ERR_PARAM;
+        c->p = *p;
+        if (c->p.mode == LES_MODE_DUCT) duct_params_changed(c);
+        return LES_OK;
+    });
 }
 
 extern "C" int les_gpu_kernel_timing(const les_gpu* c, double* rhs_ms, int* launches, double* step_ms)
 {
-    if (!c) return LES_ERR_PARAM;
-    if (rhs_ms) *rhs_ms = c->rhs_launches ? c->rhs_ms_sum / c->rhs_launches : 0.0;
-    if (launches) *launches = c->rhs_launches;
-    if (step_ms) *step_ms = c->step_ms;
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c) return LES_ERR_PARAM;
+        if (rhs_ms) *rhs_ms = c->rhs_launches ? c->rhs_ms_sum / c->rhs_launches : 0.0;
+        if (launches) *launches = c->rhs_launches;
+        if (step_ms) *step_ms = c->step_ms;
+        return LES_OK;
+    });
 }
 
 extern "C" int les_gpu_set_timing(les_gpu* c, int on)
 {
-    if (!c) return LES_ERR_PARAM;
-    c->timing = on ? 1 : 0;
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c) return LES_ERR_PARAM;
+        c->timing = on ? 1 : 0;
+        return LES_OK;
+    });
 }
 
 extern "C" int les_gpu_stats_count(const les_gpu* c) { return c ? c->stat_count : -1; }
 
 extern "C" int les_gpu_point_fetch(les_gpu* c, double* out, int max_steps, int* n_steps, int* n_points)
 {
-    if (!c || !out || !n_steps || max_steps < 0) return LES_ERR_PARAM;
-    if (c->p.mode == LES_MODE_DUCT) {
-        if (n_points) *n_points = duct_npoints(c);
-        return duct_point_fetch(c, out, max_steps, n_steps);
-    }
-    // reference scheme: the newest pt_cap_r steps since the last fetch
-    const int total = c->step - c->pt_base_r;
-    const int avail = total < c->pt_cap_r ? total : c->pt_cap_r;
-    const int n = avail < max_steps ? avail : max_steps;
-    const int first = c->step - avail;
-    const int np = c->npts_r;
-    for (int a = 0; a < n; ++a) {
-        const double* src = c->r.pt_ring + (size_t)((first + a) % c->pt_cap_r) * LES_NPOINT_MAX * 4;
-        HIPC(hipMemcpyAsync(out + (size_t)a * np * 4, src, (size_t)np * 4 * sizeof(double), hipMemcpyDeviceToHost,
-                            c->stream));
-    }
-    HIPC(hipStreamSynchronize(c->stream));
-    c->pt_base_r = c->step;
-    *n_steps = n;
-    if (n_points) *n_points = np;
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c || !out || !n_steps || max_steps < 0) return LES_ERR_PARAM;
+        if (c->p.mode == LES_MODE_DUCT) {
+            if (n_points) *n_points = duct_npoints(c);
+            return duct_point_fetch(c, out, max_steps, n_steps);
+        }
+        // reference scheme: the newest pt_cap_r steps since the last fetch
+        const int total = c->step - c->pt_base_r;
+        const int avail = total < c->pt_cap_r ? total : c->pt_cap_r;
+        const int n = avail < max_steps ? avail : max_steps;
+        const int first = c->step - avail;
+        const int np = c->npts_r;
+        for (int a = 0; a < n; ++a) {
+            const double* src = c->r.pt_ring + (size_t)((first + a) % c->pt_cap_r) * LES_NPOINT_MAX * 4;
+            HIPC(hipMemcpyAsync(out + (size_t)a * np * 4, src, (size_t)np * 4 * sizeof(double), hipMemcpyDeviceToHost,
+                                c->stream));
+        }
+        HIPC(hipStreamSynchronize(c->stream));
+        c->pt_base_r = c->step;
+        *n_steps = n;
+        if (n_points) *n_points = np;
+        return LES_OK;
+    });
 }
 
 extern "C" int les_gpu_set_points(les_gpu* c, const int* ijk, int n)
 {
-    if (!c || n < 0 || n > LES_NPOINT_MAX || (n > 0 && !ijk)) return LES_ERR_PARAM;
-    int cells[LES_NPOINT_MAX];
-    for (int a = 0; a < n; ++a) {
-        const int i = ijk[3 * a], j = ijk[3 * a + 1], k = ijk[3 * a + 2];
-        if (i < 0 || i >= c->p.nx || j < 0 || j >= c->p.ny || k < 0 || k >= c->p.nz)
-            return set_err(c, LES_ERR_PARAM, "point %d (%d, %d, %d) is outside the grid", a, i, j, k);
-        cells[a] = (int)lidx(c->g, i, j, k);
-    }
-    if (c->p.mode == LES_MODE_DUCT) return duct_set_points(c, cells, n);
-    if (n > 0) HIPC(hipMemcpy(c->r.pt_idx, cells, n * sizeof(int), hipMemcpyHostToDevice));
-    c->npts_r = n;
-    c->pt_base_r = c->step;
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c || n < 0 || n > LES_NPOINT_MAX || (n > 0 && !ijk)) return LES_ERR_PARAM;
+        int cells[LES_NPOINT_MAX];
+        for (int a = 0; a < n; ++a) {
+            const int i = ijk[3 * a], j = ijk[3 * a + 1], k = ijk[3 * a + 2];
+            if (i < 0 || i >= c->p.nx || j < 0 || j >= c->p.ny || k < 0 || k >= c->p.nz)
+                return set_err(c, LES_ERR_PARAM, "point %d (%d, %d, %d) is outside the grid", a, i, j, k);
+            cells[a] = (int)lidx(c->g, i, j, k);
+        }
+        if (c->p.mode == LES_MODE_DUCT) return duct_set_points(c, cells, n);
+        if (n > 0) HIPC(hipMemcpy(c->r.pt_idx, cells, n * sizeof(int), hipMemcpyHostToDevice));
+        c->npts_r = n;
+        c->pt_base_r = c->step;
+        return LES_OK;
+    });
 }
 
 extern "C" int les_gpu_get_points(les_gpu* c, int* ijk, int max_n, int* n)
 {
-    if (!c || !n || (max_n > 0 && !ijk)) return LES_ERR_PARAM;
-    std::vector<int> cells(LES_NPOINT_MAX);
-    int np;
-    if (c->p.mode == LES_MODE_DUCT) {
-        np = duct_npoints(c);
-        memcpy(cells.data(), c->d->desc[0].point_idx, np * sizeof(int));
-    } else {
-        np = c->npts_r;
-        HIPC(hipMemcpy(cells.data(), c->r.pt_idx, np * sizeof(int), hipMemcpyDeviceToHost));
-    }
-    for (int a = 0; a < np && a < max_n; ++a) {
-        const int q = cells[a];
-        ijk[3 * a] = q / (c->p.ny * c->p.nz);
-        ijk[3 * a + 1] = (q / c->p.nz) % c->p.ny;
-        ijk[3 * a + 2] = q % c->p.nz;
-    }
-    *n = np;
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c || !n || (max_n > 0 && !ijk)) return LES_ERR_PARAM;
+        std::vector<int> cells(LES_NPOINT_MAX);
+        int np;
+        if (c->p.mode == LES_MODE_DUCT) {
+            np = duct_npoints(c);
+            memcpy(cells.data(), c->d->desc[0].point_idx, np * sizeof(int));
+        } else {
+            np = c->npts_r;
+            HIPC(hipMemcpy(cells.data(), c->r.pt_idx, np * sizeof(int), hipMemcpyDeviceToHost));
+        }
+        for (int a = 0; a < np && a < max_n; ++a) {
+            const int q = cells[a];
+            ijk[3 * a] = q / (c->p.ny * c->p.nz);
+            ijk[3 * a + 1] = (q / c->p.nz) % c->p.ny;
+            ijk[3 * a + 2] = q % c->p.nz;
+        }
+        *n = np;
+        return LES_OK;
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -813,28 +841,32 @@ extern "C" int les_gpu_get_points(les_gpu* c, int* ijk, int max_n, int* n)
 // ---------------------------------------------------------------------------
 extern "C" int les_gpu_upload_field(les_gpu* c, const char* name, const double* data)
 {
-    if (!c || !name || !data) return LES_ERR_PARAM;
-    if (c->p.mode == LES_MODE_DUCT) return duct_upload_field(c, name, data);
-    const size_t b = (size_t)c->g.n * sizeof(double);
-    double* dst = nullptr;
-    if (strcmp(name, "nu") == 0) dst = c->r.nu;
-    else if (strcmp(name, "alpha") == 0) dst = c->r.alpha;
-    else if (strncmp(name, "stat", 4) == 0) {
-        int a = atoi(name + 4);
-        if (a < 0 || a > 9) return LES_ERR_PARAM;
-        dst = c->r.stat[a];
-    }
-    if (!dst) return set_err(c, LES_ERR_PARAM, "field %s cannot be uploaded", name);
-    HIPC(hipMemcpyAsync(dst, data, b, hipMemcpyHostToDevice, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c || !name || !data) return LES_ERR_PARAM;
+        if (c->p.mode == LES_MODE_DUCT) return duct_upload_field(c, name, data);
+        const size_t b = (size_t)c->g.n * sizeof(double);
+        double* dst = nullptr;
+        if (strcmp(name, "nu") == 0) dst = c->r.nu;
+        else if (strcmp(name, "alpha") == 0) dst = c->r.alpha;
+        else if (strncmp(name, "stat", 4) == 0) {
+            int a = atoi(name + 4);
+            if (a < 0 || a > 9) return LES_ERR_PARAM;
+            dst = c->r.stat[a];
+        }
+        if (!dst) return set_err(c, LES_ERR_PARAM, "field %s cannot be uploaded", name);
+        HIPC(hipMemcpyAsync(dst, data, b, hipMemcpyHostToDevice, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        return LES_OK;
+    });
 }
 
 extern "C" int les_gpu_set_stats_count(les_gpu* c, int count)
 {
-    if (!c || count < 0) return LES_ERR_PARAM;
-    c->stat_count = count;
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c || count < 0) return LES_ERR_PARAM;
+        c->stat_count = count;
+        return LES_OK;
+    });
 }
 
 static const char AUX_MAGIC[8] = {'L', 'E', 'S', 'A', 'U', 'X', '1', 0};
@@ -849,71 +881,79 @@ struct AuxRef {
 
 extern "C" int les_gpu_export_aux(les_gpu* c, void* buf, size_t* bytes)
 {
-    if (!c || !bytes) return LES_ERR_PARAM;
-    if (c->p.mode == LES_MODE_DUCT) return duct_export_aux(c, buf, bytes);
-    const size_t nv = 7 * (size_t)c->p.num_vortices;
-    const size_t need = sizeof(AuxRef) + nv * sizeof(double);
-    if (!buf || *bytes < need) { *bytes = need; return buf ? LES_ERR_PARAM : LES_OK; }
-    AuxRef a;
-    memset(&a, 0, sizeof(a));
-    memcpy(a.magic, AUX_MAGIC, 8);
-    a.mode = LES_MODE_REFERENCE; a.step = c->step; a.stat_count = c->stat_count; a.nvort = c->p.num_vortices;
-    a.rand_type = c->rd.rand_type; a.rand_deg = c->rd.rand_deg; a.rand_sep = c->rd.rand_sep;
-    const char* base = c->rstate;
-    a.fptr = (const char*)c->rd.fptr - base; a.rptr = (const char*)c->rd.rptr - base;
-    a.state = (const char*)c->rd.state - base; a.end = (const char*)c->rd.end_ptr - base;
-    memcpy(a.rstate, c->rstate, sizeof(a.rstate));
-    memcpy(buf, &a, sizeof(a));
-    memcpy((char*)buf + sizeof(a), c->vort.data(), nv * sizeof(double));
-    *bytes = need;
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c || !bytes) return LES_ERR_PARAM;
+        if (c->p.mode == LES_MODE_DUCT) return duct_export_aux(c, buf, bytes);
+        const size_t nv = 7 * (size_t)c->p.num_vortices;
+        const size_t need = sizeof(AuxRef) + nv * sizeof(double);
+        if (!buf || *bytes < need) { *bytes = need; return buf ? LES_ERR_PARAM : LES_OK; }
+        AuxRef a;
+        memset(&a, 0, sizeof(a));
+        memcpy(a.magic, AUX_MAGIC, 8);
+        a.mode = LES_MODE_REFERENCE; a.step = c->step; a.stat_count = c->stat_count; a.nvort = c->p.num_vortices;
+        a.rand_type = c->rd.rand_type; a.rand_deg = c->rd.rand_deg; a.rand_sep = c->rd.rand_sep;
+        const char* base = c->rstate;
+        a.fptr = (const char*)c->rd.fptr - base; a.rptr = (const char*)c->rd.rptr - base;
+        a.state = (const char*)c->rd.state - base; a.end = (const char*)c->rd.end_ptr - base;
+        memcpy(a.rstate, c->rstate, sizeof(a.rstate));
+        memcpy(buf, &a, sizeof(a));
+        memcpy((char*)buf + sizeof(a), c->vort.data(), nv * sizeof(double));
+        *bytes = need;
+        return LES_OK;
+    });
 }
 
 extern "C" int les_gpu_import_aux(les_gpu* c, const void* buf, size_t bytes)
 {
-    if (!c || !buf) return LES_ERR_PARAM;
-    if (c->p.mode == LES_MODE_DUCT) return duct_import_aux(c, buf, bytes);
-    AuxRef a;
-    if (bytes < sizeof(a)) return set_err(c, LES_ERR_PARAM, "aux blob too short");
-    memcpy(&a, buf, sizeof(a));
-    const size_t nv = 7 * (size_t)c->p.num_vortices;
-    if (memcmp(a.magic, AUX_MAGIC, 8) != 0 || a.mode != LES_MODE_REFERENCE || a.nvort != c->p.num_vortices ||
-        bytes != sizeof(a) + nv * sizeof(double))
-        return set_err(c, LES_ERR_PARAM, "aux blob does not match this context");
-    const long long lim = (long long)sizeof(c->rstate);
-    if (a.fptr < 0 || a.fptr >= lim || a.rptr < 0 || a.rptr >= lim || a.state < 0 || a.state >= lim || a.end < 0 ||
-        a.end > lim)
-        return set_err(c, LES_ERR_PARAM, "aux blob holds an invalid random-state layout");
-    memcpy(c->rstate, a.rstate, sizeof(c->rstate));
-    c->rd.rand_type = a.rand_type; c->rd.rand_deg = a.rand_deg; c->rd.rand_sep = a.rand_sep;
-    c->rd.fptr = (int32_t*)(c->rstate + a.fptr); c->rd.rptr = (int32_t*)(c->rstate + a.rptr);
-    c->rd.state = (int32_t*)(c->rstate + a.state); c->rd.end_ptr = (int32_t*)(c->rstate + a.end);
-    memcpy(c->vort.data(), (const char*)buf + sizeof(a), nv * sizeof(double));
-    HIPC(hipMemcpy(c->r.vort, c->vort.data(), nv * sizeof(double), hipMemcpyHostToDevice));
-    c->step = a.step;
-    c->stat_count = a.stat_count;
-    c->pt_base_r = a.step;
-    c->probe.clear();
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c || !buf) return LES_ERR_PARAM;
+        if (c->p.mode == LES_MODE_DUCT) return duct_import_aux(c, buf, bytes);
+        AuxRef a;
+        if (bytes < sizeof(a)) return set_err(c, LES_ERR_PARAM, "aux blob too short");
+        memcpy(&a, buf, sizeof(a));
+        const size_t nv = 7 * (size_t)c->p.num_vortices;
+        if (memcmp(a.magic, AUX_MAGIC, 8) != 0 || a.mode != LES_MODE_REFERENCE || a.nvort != c->p.num_vortices ||
+            bytes != sizeof(a) + nv * sizeof(double))
+            return set_err(c, LES_ERR_PARAM, "aux blob does not match this context");
+        const long long lim = (long long)sizeof(c->rstate);
+        if (a.fptr < 0 || a.fptr >= lim || a.rptr < 0 || a.rptr >= lim || a.state < 0 || a.state >= lim || a.end < 0 ||
+            a.end > lim)
+            return set_err(c, LES_ERR_PARAM, "aux blob holds an invalid random-state layout");
+        memcpy(c->rstate, a.rstate, sizeof(c->rstate));
+        c->rd.rand_type = a.rand_type; c->rd.rand_deg = a.rand_deg; c->rd.rand_sep = a.rand_sep;
+        c->rd.fptr = (int32_t*)(c->rstate + a.fptr); c->rd.rptr = (int32_t*)(c->rstate + a.rptr);
+        c->rd.state = (int32_t*)(c->rstate + a.state); c->rd.end_ptr = (int32_t*)(c->rstate + a.end);
+        memcpy(c->vort.data(), (const char*)buf + sizeof(a), nv * sizeof(double));
+        HIPC(hipMemcpy(c->r.vort, c->vort.data(), nv * sizeof(double), hipMemcpyHostToDevice));
+        c->step = a.step;
+        c->stat_count = a.stat_count;
+        c->pt_base_r = a.step;
+        c->probe.clear();
+        return LES_OK;
+    });
 }
 
 // restart from a state without engine aux state (VTK restart, :4269-4280):
 // step index and fresh SEM eddies (initialize_vortices :861-877)
 extern "C" int les_gpu_set_step(les_gpu* c, int step)
 {
-    if (!c || step < 0) return LES_ERR_PARAM;
-    if (c->p.mode == LES_MODE_DUCT) return duct_set_step(c, step);
-    c->step = step;
-    c->pt_base_r = step;
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c || step < 0) return LES_ERR_PARAM;
+        if (c->p.mode == LES_MODE_DUCT) return duct_set_step(c, step);
+        c->step = step;
+        c->pt_base_r = step;
+        return LES_OK;
+    });
 }
 
 extern "C" int les_gpu_init_sem(les_gpu* c)
 {
-    if (!c) return LES_ERR_PARAM;
-    if (c->p.mode == LES_MODE_DUCT) return duct_init_sem(c);
-    init_vortices(c);
-    return LES_OK;
+    return abi_guard(c, [&]() -> int {
+        if (!c) return LES_ERR_PARAM;
+        if (c->p.mode == LES_MODE_DUCT) return duct_init_sem(c);
+        init_vortices(c);
+        return LES_OK;
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -931,36 +971,38 @@ __global__ void __launch_bounds__(256) k_copy_stream(long long n, const double2*
 
 extern "C" int les_copy_bandwidth(size_t bytes, int reps, double* ms_per_copy, double* gbps)
 {
-    if (bytes < 4096 || reps < 1 || !ms_per_copy || !gbps) return LES_ERR_PARAM;
-    const long long n = (long long)(bytes / 16);
-    void *a = nullptr, *b = nullptr;
-    hipStream_t s = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    int rc = LES_OK;
-    if (hipMalloc(&a, (size_t)n * 16) != hipSuccess || hipMalloc(&b, (size_t)n * 16) != hipSuccess ||
-        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
-        hipEventCreate(&e1) != hipSuccess)
-        rc = LES_ERR_HIP;
-    if (rc == LES_OK && hipMemsetAsync(a, 0, (size_t)n * 16, s) != hipSuccess) rc = LES_ERR_HIP;
-    const int grid = 4096;
-    if (rc == LES_OK) {
-        hipLaunchKernelGGL(k_copy_stream, dim3(grid), dim3(256), 0, s, n, (const double2*)a, (double2*)b);  // warm-up
-        (void)hipEventRecord(e0, s);
-        for (int r = 0; r < reps; ++r)
-            hipLaunchKernelGGL(k_copy_stream, dim3(grid), dim3(256), 0, s, n, (const double2*)a, (double2*)b);
-        (void)hipEventRecord(e1, s);
-        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) rc = LES_ERR_HIP;
-    }
-    if (rc == LES_OK) {
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        *ms_per_copy = ms / reps;
-        *gbps = 2.0 * (double)n * 16 / (*ms_per_copy * 1e-3) / 1e9;
-    }
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    if (s) (void)hipStreamDestroy(s);
-    if (a) (void)hipFree(a);
-    if (b) (void)hipFree(b);
-    return rc;
+    return abi_guard(nullptr, [&]() -> int {
+        if (bytes < 4096 || reps < 1 || !ms_per_copy || !gbps) return LES_ERR_PARAM;
+        const long long n = (long long)(bytes / 16);
+        void *a = nullptr, *b = nullptr;
+        hipStream_t s = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        int rc = LES_OK;
+        if (hipMalloc(&a, (size_t)n * 16) != hipSuccess || hipMalloc(&b, (size_t)n * 16) != hipSuccess ||
+            hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+            hipEventCreate(&e1) != hipSuccess)
+            rc = LES_ERR_HIP;
+        if (rc == LES_OK && hipMemsetAsync(a, 0, (size_t)n * 16, s) != hipSuccess) rc = LES_ERR_HIP;
+        const int grid = 4096;
+        if (rc == LES_OK) {
+            hipLaunchKernelGGL(k_copy_stream, dim3(grid), dim3(256), 0, s, n, (const double2*)a, (double2*)b);  // warm-up
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < reps; ++r)
+                hipLaunchKernelGGL(k_copy_stream, dim3(grid), dim3(256), 0, s, n, (const double2*)a, (double2*)b);
+            (void)hipEventRecord(e1, s);
+            if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) rc = LES_ERR_HIP;
+        }
+        if (rc == LES_OK) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            *ms_per_copy = ms / reps;
+            *gbps = 2.0 * (double)n * 16 / (*ms_per_copy * 1e-3) / 1e9;
+        }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamDestroy(s);
+        if (a) (void)hipFree(a);
+        if (b) (void)hipFree(b);
+        return rc;
+    });
 }

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 
+#include <new>
+#include <stdio.h>
 #include <type_traits>
 #include <vector>
 
@@ -74,7 +76,28 @@ struct les_gpu {
 int monitor(les_gpu* c, const double* rho, const double* T, const double* G, const double* Y0, const double* u,
             les_step_report* rep);
 
-// duct mode (les_duct_kernels.hip)
+// Runs the body of a C ABI entry point: a C++ exception from host code (a
+// failed host allocation) becomes an error code and message instead of
+// crossing the C boundary.
+template <typename F>
+static inline int abi_guard(const les_gpu* c, F&& body)
+{
+    const char* what = nullptr;
+    int code = LES_OK;
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        what = "host memory allocation failed";
+        code = LES_ERR_HIP;
+    } catch (...) {
+        what = "unexpected host error";
+        code = LES_ERR_STATE;
+    }
+    if (c) snprintf(const_cast<les_gpu*>(c)->err, sizeof(c->err), "%s", what);
+    return code;
+}
+
+// duct mode (les_duct_kernels.hip, les_duct_host.hip)
 int  duct_create(les_gpu* c);
 void duct_destroy(les_gpu* c);
 int  duct_init_state(les_gpu* c);
