@@ -132,6 +132,34 @@ def cpu_baseline(preset, nx, ny, nz, over, stages, threads, target_s=10.0, max_s
     return total / steps, cores, steps, total
 
 
+def reference_build_baseline(lesgpu, target_s=8.0, max_steps=400):
+    """The reference solver compiled from its own source (oracle/_ref, OpenMP
+    build) at its fixed 60x20x40 grid and scheme (RK4, TENO5, 5 species),
+    timed on this host's cores over whole steps (about target_s seconds,
+    including its own per-step probe CSV writes), with the engine's FP64
+    reference-scheme path at the same grid beside it.  None when the
+    reference build is absent."""
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    if not os.path.exists(pyoracle.REF_OMP):
+        return None
+    cells, stages = 60 * 20 * 40, 4
+    with tempfile.TemporaryDirectory() as wd:
+        def timed(n):
+            out = pyoracle.run_reference(wd, steps=n, seed=0, dump=[10 ** 9], omp=True, time_it=True)
+            return json.loads(out.strip().splitlines()[-1])
+        cal = timed(3)
+        n = int(min(max_steps, max(10, target_s / max(cal["s_per_step"], 1e-4))))
+        r = timed(n)
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    g = single_run(lesgpu, lesgpu.params(0), 200, 10, timing=False)
+    return {"value": cells * stages / r["s_per_step"], "unit": "cell-updates/s", "cores": cores,
+            "kind": "reference", "grid": [60, 20, 40], "scheme": "reference (RK4, TENO5, 5-species mixing)",
+            "sample": "%d steps of the reference OpenMP build, %.1f s in all" % (r["steps"], r["total_s"]),
+            "gpu_fp64_same_grid": cells * stages / (g["step_ms"] * 1e-3), "gpu_fp64_ms_per_step": g["step_ms"]}
+
+
 def single_run(lesgpu, p, steps, warmup, timing=True):
     g = lesgpu.LesGpu(p, lesgpu.mixture())
     try:
@@ -398,6 +426,11 @@ def main():
                                "sample": "%d full time steps (%d RK stages each) of the same %dx%dx%d workload, "
                                          "oracle C+OpenMP FP64, %.1f s in all" % (n, wl["stages"], wl["nx"], wl["ny"],
                                                                                   wl["nz"], total)}
+        if a.workload == "duct128-fp64":
+            log("reference build baseline")
+            rb = reference_build_baseline(lesgpu)
+            if rb:
+                out["cpu_baseline"]["reference_build"] = rb
     print(json.dumps(out))
 
 
