@@ -1,0 +1,47 @@
+"""bench.py's command-line contract, checked without a GPU: the default
+workload and step counts, the single-GPU guard, the workload table, and the
+profile-derived roofline inputs it reads."""
+import json
+import os
+import subprocess
+import sys
+
+from conftest import ROOT
+
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def test_defaults(monkeypatch):
+    monkeypatch.setattr(sys, "argv", ["bench.py"])
+    a = bench.parse()
+    assert a.gpus == 1 and a.workload == "duct128-fp64"
+    assert a.steps == 500 and a.warmup == 10
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300, cwd=ROOT)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_workloads_have_their_sizes():
+    w = bench.WORKLOADS
+    assert (w["duct128-fp64"]["nx"], w["duct128-fp64"]["ny"], w["duct128-fp64"]["nz"]) == (128, 128, 128)
+    assert w["duct128-fp64"]["dtype"] == "f64" and w["duct128-fp32"]["dtype"] == "f32"
+    assert (w["large256-fp32"]["nx"], w["large256-fp32"]["ny"], w["large256-fp32"]["nz"]) == (256, 192, 192)
+    # algorithmic bytes per cell and RK3 stage: 13.33 fields of 8 (4) bytes
+    assert abs(w["duct128-fp64"]["bytes_per_cell_stage"] - 106.7) < 0.05
+    assert abs(w["duct128-fp32"]["bytes_per_cell_stage"] - 53.3) < 0.05
+
+
+def test_profile_inputs_are_present_and_consistent():
+    with open(os.path.join(ROOT, "profiles", "rhs_traffic.json")) as f:
+        traffic = json.load(f)
+    with open(os.path.join(ROOT, "profiles", "rhs_valu.json")) as f:
+        valu = json.load(f)
+    for wl in ("duct128-fp64", "duct128-fp32"):
+        assert bench.traffic_from_profiles(wl) == traffic[wl] > 0
+        assert bench.valu_from_profiles(wl) == valu[wl] > 0
+    # FP32 moves fewer bytes than FP64 for the same stage
+    assert traffic["duct128-fp32"] < traffic["duct128-fp64"]
