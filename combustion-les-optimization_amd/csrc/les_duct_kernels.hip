@@ -200,17 +200,32 @@ __host__ __device__ __forceinline__ double duniform(unsigned long long seed, uns
 
 __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
-// Arithmetic of the two storage types.  double: IEEE operations and the
-// device libm (the FP64 path is compared with the oracle).  float (the FP32
-// form): the hardware's native reciprocal, exp2 and log2 (v_rcp_f32,
+// Arithmetic of the two storage types.  double (the FP64 path, compared with
+// the oracle to 1e-10): the device libm's exp and log; x^(3/4) as two square
+// roots; divisions by Newton-Raphson from the hardware reciprocal (v_rcp_f64,
+// two refinement steps e = 1 - b r, r += r e, then one residual correction of
+// the quotient q += r (a - b q), Markstein's scheme), within about 1 ulp of
+// the IEEE quotient for the finite, normal, moderate operands these kernels
+// divide (temperatures, positive sums of squares, eddy viscosities) and
+// without the scaling and fix-up steps the IEEE sequence adds.  float (the
+// FP32 form): the hardware's native reciprocal, exp2 and log2 (v_rcp_f32,
 // v_exp_f32, v_log_f32; about 1 ulp), which the FP32 tolerances cover.
 template <typename real>
 struct M {
-    static __device__ __forceinline__ real div(real a, real b) { return a / b; }
+    static __device__ __forceinline__ real div(real a, real b)
+    {
+        real r = __builtin_amdgcn_rcp(b);
+        real e = fma(-b, r, (real)1.0);
+        r = fma(r, e, r);
+        e = fma(-b, r, (real)1.0);
+        r = fma(r, e, r);
+        const real q = a * r;
+        return fma(r, fma(-b, q, a), q);
+    }
     static __device__ __forceinline__ real exp(real x) { return ::exp(x); }
     static __device__ __forceinline__ real log(real x) { return ::log(x); }
     static __device__ __forceinline__ real pow(real x, real y) { return ::pow(x, y); }
-    static __device__ __forceinline__ real tanh(real x) { return ::tanh(x); }
+    static __device__ __forceinline__ real pow34(real x) { return sqrt(x * sqrt(x)); }
 };
 
 template <>
@@ -219,18 +234,16 @@ struct M<float> {
     static __device__ __forceinline__ float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504089f); }
     static __device__ __forceinline__ float log(float x) { return __builtin_amdgcn_logf(x) * 0.69314718056f; }
     static __device__ __forceinline__ float pow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
-    static __device__ __forceinline__ float tanh(float x)
-    {
-        const float t = __builtin_amdgcn_exp2f(-2.88539008178f * fabsf(x));     // exp(-2|x|)
-        return copysignf((1.0f - t) * __builtin_amdgcn_rcpf(1.0f + t), x);
-    }
+    static __device__ __forceinline__ float pow34(float x) { return pow(x, 0.75f); }
 };
 
-// progress variable c = (1 + tanh(phi / (2 dx))) / 2; i2dx = 0.5/dx of the run
+// progress variable c = (1 + tanh(phi / (2 dx))) / 2, evaluated as the equal
+// logistic 1 / (1 + exp(-phi / dx)) (one exponential and one division; no
+// cancellation where c is small); i2dx = 0.5/dx of the run
 template <typename real>
 __device__ __forceinline__ real progress(real phi, real i2dx)
 {
-    return (real)0.5 * ((real)1.0 + M<real>::tanh(phi * i2dx));
+    return M<real>::div((real)1.0, (real)1.0 + M<real>::exp((real)-2.0 * (phi * i2dx)));
 }
 
 template <typename real>
@@ -627,7 +640,7 @@ __global__ void __launch_bounds__(256) k_duct_sgs(Pool P, int stage, int xcd_map
                 real d_eta;
                 if (X < (real)LES_SMALL) d_eta = (real)1.0;
                 else {
-                    real px = M<real>::pow(X, (real)0.75);
+                    real px = M<real>::pow34(X);
                     real alg = (real)0.2187 * px;
                     d_eta = (real)0.2340 * px * (alg > (real)1.0 ? M<real>::log(alg) : (real)0.0);
                 }
