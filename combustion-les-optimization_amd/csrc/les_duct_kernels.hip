@@ -201,8 +201,8 @@ __host__ __device__ __forceinline__ double duniform(unsigned long long seed, uns
 __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
 // Arithmetic of the two storage types.  double (the FP64 path, compared with
-// the oracle to 1e-10): the device libm's exp and log; x^(3/4) as two square
-// roots; divisions by Newton-Raphson from the hardware reciprocal (v_rcp_f64,
+// the oracle to 1e-10): the device libm's exp and log; x^y as exp(y log x);
+// x^(3/4) as two square roots; divisions by Newton-Raphson from the hardware reciprocal (v_rcp_f64,
 // two refinement steps e = 1 - b r, r += r e, then one residual correction of
 // the quotient q += r (a - b q), Markstein's scheme), within about 1 ulp of
 // the IEEE quotient for the finite, normal, moderate operands these kernels
@@ -224,7 +224,9 @@ struct M {
     }
     static __device__ __forceinline__ real exp(real x) { return ::exp(x); }
     static __device__ __forceinline__ real log(real x) { return ::log(x); }
-    static __device__ __forceinline__ real pow(real x, real y) { return ::pow(x, y); }
+    // x^y of positive x as exp(y log x): relative error about |y log x| ulp,
+    // without the special-case handling of the libm pow
+    static __device__ __forceinline__ real pow(real x, real y) { return ::exp(y * ::log(x)); }
     static __device__ __forceinline__ real pow34(real x) { return sqrt(x * sqrt(x)); }
 };
 
