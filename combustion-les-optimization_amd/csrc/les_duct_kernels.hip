@@ -202,9 +202,10 @@ __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >
 
 // Arithmetic of the two storage types.  double (the FP64 path, compared with
 // the oracle to 1e-10): the device libm's exp and log; x^y as exp(y log x);
-// x^(3/4) as two square roots; divisions by Newton-Raphson from the hardware reciprocal (v_rcp_f64,
-// two refinement steps e = 1 - b r, r += r e, then one residual correction of
-// the quotient q += r (a - b q), Markstein's scheme), within about 1 ulp of
+// x^(3/4) as two square roots; divisions by Newton-Raphson from the hardware
+// reciprocal (v_rcp_f64, two refinement steps e = 1 - b r, r += r e, then one
+// residual correction of the quotient q += r (a - b q), Markstein's scheme),
+// within about 1 ulp of
 // the IEEE quotient for the finite, normal, moderate operands these kernels
 // divide (temperatures, positive sums of squares, eddy viscosities) and
 // without the scaling and fix-up steps the IEEE sequence adds.  float (the
