@@ -58,7 +58,13 @@ WORKLOADS = {
     "large256-fp32": dict(preset=4, nx=256, ny=192, nz=192, stages=3, bytes_per_cell_stage=53.3, over={},
                           dtype="f32", desc="256x192x192 duct, FP32 bulk, FP64 projection residuals and "
                                             "statistics, I=15 %, 2000 SEM eddies, 16 x-stations"),
+    # one GPU's share of the 256x192x192 run split into 8 x-slabs: 32 planes
+    # plus two 4-plane halos, computed here as an ordinary 40-plane run
+    "slab32-fp32": dict(preset=4, nx=40, ny=192, nz=192, stages=3, bytes_per_cell_stage=53.3, over=dict(lx=0.040),
+                        dtype="f32", desc="one x-slab of the 256x192x192 duct at 8 ranks: 32 planes + 2 x 4 "
+                                          "halo planes (40x192x192), FP32, with its halo-exchange copies"),
 }
+XGMI_LINK_GBS = 153.0   # one xGMI link, one direction (MI355X: 7 links per GPU)
 RHS_KERNELS = "k_duct_rhs_scalar + k_duct_rhs_momentum (one RK stage's RHS and combine, two launches)"
 
 
@@ -225,6 +231,24 @@ def single_line(a, lesgpu, name):
         out["roofline"]["valu_issue"] = {"achieved": g, "peak": PEAK_VALU_GWIS, "unit": "G wave-instr/s",
                                          "frac": g / PEAK_VALU_GWIS, "insts_per_stage": vi}
     return out, p
+
+
+def slab_halo(lesgpu, p, planes=4, reps=50):
+    """The local part of one x-slab's halo exchange per RK stage: two faces of
+    `planes` x-planes of the five state fields, copied between two resident
+    slabs of one pool (les_batch_xface_copy), beside the transfer time of the
+    same bytes over one xGMI link."""
+    cfg = lesgpu.batch_config(max_slots=2, pool_cells=2 * p.nx * p.ny * p.nz, max_nx=p.nx, max_ny=p.ny,
+                              max_nz=p.nz, max_vortices=p.num_vortices, precision=p.precision, autotune=0)
+    with lesgpu.Batch(cfg, p) as b:
+        s0, s1 = b.admit(p, 0), b.admit(p, 0)
+        up = b.xface_copy(s0, p.nx - 2 * planes, s1, 0, planes, reps)        # to the right neighbour's low halo
+        down = b.xface_copy(s1, planes, s0, p.nx - planes, planes, reps)     # to the left neighbour's high halo
+    esize = 4 if p.precision == 1 else 8
+    face_bytes = planes * p.ny * p.nz * 5 * esize
+    return {"planes": planes, "fields": 5, "bytes_per_face": face_bytes,
+            "local_copy_ms_per_stage": up + down, "local_copy_gbs": 2 * face_bytes / ((up + down) * 1e-3) / 1e9,
+            "xgmi_ms_per_face_estimate": face_bytes / (XGMI_LINK_GBS * 1e9) * 1e3}
 
 
 def ensemble_leg(lesgpu, serving, steps, warmup, members=64, grid=(64, 48, 48), profile_steps=50):
@@ -411,6 +435,13 @@ def main():
     if a.workload == "large256-fp32":
         out["metric"] = "cell-updates/s per RK stage, 256x192x192"
         out["profiles_fp32_vs_fp64"] = station_profile_error(lesgpu, serving, p, a.steps)
+    if a.workload == "slab32-fp32":
+        out["metric"] = "cell-updates/s per RK stage, one x-slab (40x192x192) of the 256x192x192 run"
+        h = slab_halo(lesgpu, p)
+        stage_ms = out["ms_per_step"] / wl["stages"]
+        h["compute_ms_per_stage"] = stage_ms
+        h["xgmi_over_compute"] = 2 * h["xgmi_ms_per_face_estimate"] / stage_ms
+        out["halo_exchange"] = h
     if a.workload == "duct128-fp64" and not a.no_extra:
         f32, _ = single_line(a, lesgpu, "duct128-fp32")
         out["fp32_form"] = {"value": f32["value"], "ms_per_step": f32["ms_per_step"], "roofline": f32["roofline"],

@@ -1315,6 +1315,42 @@ extern "C" int les_batch_point_fetch(les_batch* b, int slot, double* out, int ma
     });
 }
 
+extern "C" int les_batch_xface_copy(les_batch* b, int src_slot, int src_plane, int dst_slot, int dst_plane,
+                                    int planes, int reps, double* ms_per_copy)
+{
+    return abi_guard(b ? b->c : nullptr, [&]() -> int {
+        if (!batch_slot_ok(b, src_slot) || !batch_slot_ok(b, dst_slot)) return LES_ERR_PARAM;
+        les_gpu* c = b->c;
+        DuctState* d = c->d;
+        const GridDims& gs = d->desc[src_slot].g;
+        const GridDims& gd = d->desc[dst_slot].g;
+        const bool overlap = src_slot == dst_slot && src_plane < dst_plane + planes && dst_plane < src_plane + planes;
+        if (planes < 1 || reps < 1 || gs.ny != gd.ny || gs.nz != gd.nz || src_plane < 0 || dst_plane < 0 ||
+            src_plane + planes > gs.nx || dst_plane + planes > gd.nx || overlap) {
+            snprintf(c->err, sizeof(c->err), "x-face copy of %d planes from %d of slot %d (%dx%dx%d) to %d of slot %d "
+                     "(%dx%dx%d) is out of range or overlaps", planes, src_plane, src_slot, gs.nx, gs.ny, gs.nz,
+                     dst_plane, dst_slot, gd.nx, gd.ny, gd.nz);
+            return LES_ERR_PARAM;
+        }
+        const long long plane = (long long)gs.ny * gs.nz, wpe = (long long)d->esize / 4;
+        const long long sw = (d->desc[src_slot].off + src_plane * plane) * wpe;
+        const long long dw = (d->desc[dst_slot].off + dst_plane * plane) * wpe;
+        const long long words = planes * plane * wpe;
+        const long long nb = (words + 255) / 256;
+        const int blocks = (int)(nb < 2048 ? nb : 2048);
+        DHIPC(hipEventRecord(c->ev[2], c->stream));
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_duct_xface_copy, dim3(blocks), dim3(256), 0, c->stream, d->P, sw, dw, words);
+        DLAUNCH_CHECK();
+        DHIPC(hipEventRecord(c->ev[3], c->stream));
+        DHIPC(hipEventSynchronize(c->ev[3]));
+        float ms = 0.f;
+        DHIPC(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+        if (ms_per_copy) *ms_per_copy = ms / reps;
+        return slot_state_changed(c, dst_slot);
+    });
+}
+
 // ---------------------------------------------------------------------------
 // resume support of a single-run duct context (slot 0)
 // ---------------------------------------------------------------------------

@@ -1631,3 +1631,17 @@ __global__ void k_to_double(long long n, const real* src, double* dst)
     if (q < n) dst[q] = (double)src[q];
 }
 
+// x-planes of the five state fields (arena A) from one run's range to
+// another's, as 32-bit words (both ranges start on an element boundary);
+// one launch for the five fields
+__global__ void __launch_bounds__(256) k_duct_xface_copy(Pool P, long long src_word, long long dst_word,
+                                                         long long words)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (int f = 0; f < DNF; ++f) {
+        const unsigned* s = (const unsigned*)P.A[f] + src_word;
+        unsigned* d = (unsigned*)P.A[f] + dst_word;
+        for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) d[w] = s[w];
+    }
+}
+

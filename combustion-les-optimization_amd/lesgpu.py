@@ -71,6 +71,8 @@ def lib():
         L.les_batch_download_field.argtypes = [vp, C.c_int, C.c_char_p, P(C.c_double)]
         L.les_batch_report.argtypes = [vp, C.c_int, P(abi.StepReport)]
         L.les_batch_point_fetch.argtypes = [vp, C.c_int, P(C.c_double), C.c_int, P(C.c_int)]
+        L.les_batch_xface_copy.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           P(C.c_double)]
         _lib = L
     return _lib
 
@@ -316,6 +318,14 @@ class Batch:
         rep = abi.StepReport()
         self._check(lib().les_batch_report(self.h, slot, C.byref(rep)), "les_batch_report")
         return rep
+
+    def xface_copy(self, src, src_plane, dst, dst_plane, planes, reps=1):
+        """Copy x-planes of the state from one run to another (the local part of
+        an x-slab halo exchange); returns the mean ms of one copy."""
+        ms = C.c_double()
+        self._check(lib().les_batch_xface_copy(self.h, src, src_plane, dst, dst_plane, planes, reps, C.byref(ms)),
+                    "les_batch_xface_copy")
+        return ms.value
 
     def point_series(self, slot, max_steps=256):
         out = np.zeros(max_steps * abi.LES_NPOINT_MAX * 4)

@@ -328,6 +328,19 @@ int les_batch_point_fetch(les_batch* b, int slot, double* out, int max_steps, in
 /* Flame-region probe rows of one run recorded since the last fetch (the newest
  * probe_cap steps; record_flame_region_data :3403-3471 rows). */
 int les_batch_probe_fetch(les_batch* b, int slot, les_probe_row* rows, int max_rows, int* n_rows);
+/* x-plane copy between two resident runs of one pool: `planes` x-planes of the
+ * five state fields from planes [src_plane, src_plane + planes) of src_slot to
+ * [dst_plane, dst_plane + planes) of dst_slot (equal ny and nz, ranges inside
+ * each run, no overlap within one run), repeated `reps` (>= 1) times on the
+ * pool's stream; *ms_per_copy (optional) receives the mean duration of one
+ * copy.  The destination's dt maxima are refreshed afterwards, as after an
+ * upload.  No counterpart in the reference, whose domain is one process's
+ * global arrays: this is the local part of an x-slab halo exchange for a run
+ * split across GPUs (an x-face of a run's state is one contiguous range per
+ * field, so a ghost-plane fill copies five contiguous ranges, one launch
+ * here, the ranges an RCCL send/receive would move straight from the arena). */
+int les_batch_xface_copy(les_batch* b, int src_slot, int src_plane, int dst_slot, int dst_plane, int planes,
+                         int reps, double* ms_per_copy);
 
 #ifdef __cplusplus
 }

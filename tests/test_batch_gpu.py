@@ -159,3 +159,33 @@ def test_batch_point_series_and_autotune(lesgpu):
         idx = [((a + 1) * nx // 5 * ny + (2 * c + 1) * ny // 8) * nz + (2 * e + 1) * nz // 8
                for a in range(4) for c in range(4) for e in range(4)]
         assert np.allclose(ser[-1, :, 0], u[idx], rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_xface_copy_fills_ghost_planes(lesgpu, precision):
+    """les_batch_xface_copy moves x-planes of the state between two runs (two
+    x-slabs' halo exchange on one GPU): the copied planes equal the source's
+    bitwise, every other plane is untouched, and bad ranges are refused."""
+    print("case: x-face copy between two runs, precision=%d" % precision, flush=True)
+    with pool(lesgpu, precision) as b:
+        sa = b.admit(run_params(lesgpu, precision, nx=40, ny=24, nz=32, seed=5), n_steps=0)
+        sb = b.admit(run_params(lesgpu, precision, nx=48, ny=24, nz=32, seed=6), n_steps=0)
+        b.step(3)
+        qa, qb = b.download(sa), b.download(sb)
+        ms = b.xface_copy(sa, 4, sb, 44, 4, reps=3)          # a's first interior planes -> b's high ghosts
+        b.xface_copy(sa, 32, sb, 0, 4)                        # a's last interior planes -> b's low ghosts
+        qb2, qa2 = b.download(sb), b.download(sa)
+        assert ms > 0
+        for f in range(5):
+            assert np.array_equal(qa2[f], qa[f])
+            assert np.array_equal(qb2[f][44:48], qa[f][4:8])
+            assert np.array_equal(qb2[f][0:4], qa[f][32:36])
+            assert np.array_equal(qb2[f][4:44], qb[f][4:44])
+        b.step(1)                                             # the runs go on from the new state
+        assert not b.report(sb).nonfinite
+        for bad in [(sa, 38, sb, 0, 4), (sa, 0, sb, 46, 4), (sa, 0, sa, 2, 4), (sa, -1, sb, 0, 2), (sa, 0, sb, 0, 0)]:
+            with pytest.raises(lesgpu.LesError):
+                b.xface_copy(*bad)
+        sc = b.admit(run_params(lesgpu, precision, nx=40, ny=24, nz=24, seed=7), n_steps=0)
+        with pytest.raises(lesgpu.LesError):
+            b.xface_copy(sa, 4, sc, 0, 4)                     # ny, nz differ
