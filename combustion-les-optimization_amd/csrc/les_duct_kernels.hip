@@ -386,6 +386,14 @@ struct RunBuf {
         else
             return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
     }
+    // element at a byte offset (a multiple of sizeof(T))
+    __device__ __forceinline__ T at(int boff) const
+    {
+        if constexpr (sizeof(T) == 8)
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, boff, 0, 0));
+        else
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, boff, 0, 0));
+    }
 };
 
 // 0x00020000: the gfx9-family buffer descriptor word 3 for raw 32-bit data
@@ -402,6 +410,7 @@ template <typename T>
 struct RunBuf {                                      // host pass only, never executed
     const T* p;
     __device__ T operator[](int i) const { return p[i]; }
+    __device__ T at(int boff) const { return *(const T*)((const char*)p + boff); }
 };
 template <typename T>
 __device__ RunBuf<T> run_buf(const T* p, long long) { return RunBuf<T>{p}; }
@@ -799,9 +808,11 @@ k_duct_rhs_scalar(Pool P, RhsArgs<real> a)
 // momentum RHS (:2552-2691 structure) + SSP-RK3 combine; in the last stage also
 // the per-tile max |u| of the new state (its T was written by the scalar kernel)
 template <typename real, int TK>
-// FP64: at least 6 waves per SIMD with the component loops rolled (79 VGPRs;
-// unrolled it needs 115 at 4 waves); FP32: 8 waves (64 VGPRs, no spill)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 6 : 8)))
+// FP64: at least 5 waves per SIMD with the component loops rolled (85 VGPRs;
+// unrolled it needs 115 at 4 waves); FP32: 7 waves (71 VGPRs).  The line
+// offsets held across the component loop cost 7 registers: at 6 (FP64) and 8
+// (FP32) waves they would spill.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 5 : 7)))
 k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
 {
     __shared__ double vred[8];
@@ -846,12 +857,21 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
 #pragma unroll 1
                 for (int d = 0; d < 3; ++d) {
                     const real id = rcp_dir<real>(k, DUCT_RC_ID, d);
-                    const int qp = line_idx(g, bi, j, kk, d, 1), qm = line_idx(g, bi, j, kk, d, -1);
+                    // byte offsets of the 7-point line along d, selected by d once
+                    // per line for every field read on it (held in registers, so
+                    // the selection is not repeated before each load)
+                    int lo[7];
+#pragma unroll
+                    for (int o = 0; o < 7; ++o) {
+                        lo[o] = line_idx(g, bi, j, kk, d, o - 3) * (int)sizeof(real);
+                        asm volatile("" : "+v"(lo[o]));
+                    }
                     const RunBuf<real> U = d == 0 ? vel[0] : (d == 1 ? vel[1] : vel[2]);
-                    real up = (real)0.5 * (U[qp] + U[q]);
-                    real um = (real)0.5 * (U[q] + U[qm]);
-                    real rp = (real)0.5 * (M<real>::div(rut, Tf[qp]) + rf0);
-                    real rm = (real)0.5 * (rf0 + M<real>::div(rut, Tf[qm]));
+                    const real u0 = U.at(lo[3]);
+                    real up = (real)0.5 * (U.at(lo[4]) + u0);
+                    real um = (real)0.5 * (u0 + U.at(lo[2]));
+                    real rp = (real)0.5 * (M<real>::div(rut, Tf.at(lo[4])) + rf0);
+                    real rm = (real)0.5 * (rf0 + M<real>::div(rut, Tf.at(lo[2])));
                     real mp = rp * up, mm = rm * um;
                     const bool band = (d == 0 && xband);
 #pragma unroll MU
@@ -859,13 +879,13 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                         const RunBuf<real> V = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
                         real fp, fm;
                         if (band) {
-                            real v2 = V[qm], v3 = V[q], v4 = V[qp];
+                            real v2 = V.at(lo[2]), v3 = V.at(lo[3]), v4 = V.at(lo[4]);
                             fp = (up >= (real)0.0) ? v3 : v4;
                             fm = (um >= (real)0.0) ? v2 : v3;
                         } else {
                             real v[7];
 #pragma unroll
-                            for (int o = 0; o < 7; ++o) v[o] = V[line_idx(g, bi, j, kk, d, o - 3)];
+                            for (int o = 0; o < 7; ++o) v[o] = V.at(lo[o]);
                             fp = face_plus<real>(scheme, v, up >= (real)0.0);
                             fm = face_minus<real>(scheme, v, um >= (real)0.0);
                         }
