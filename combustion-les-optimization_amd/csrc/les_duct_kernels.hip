@@ -907,7 +907,7 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 const real ix2 = rcp_const<real>(k, DUCT_RC_ID2), iy2 = rcp_const<real>(k, DUCT_RC_ID2 + 1), iz2 = rcp_const<real>(k, DUCT_RC_ID2 + 2);
                 real diff[3];
                 real df0 = (real)0, df1 = (real)0, df2 = (real)0;
-#pragma unroll MU
+#pragma unroll
                 for (int m = 0; m < 3; ++m) {
                     const RunBuf<real> U = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
                     real uc = U[q];
@@ -927,7 +927,7 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 const real ctr = (real)k.accel_center, Fm = (real)k.max_force;
                 if (g0 > (real)0.01 && g0 < (real)0.99) {
                     real gx = (real)0, gy = (real)0, gz = (real)0;
-#pragma unroll MU
+#pragma unroll
                     for (int d = 0; d < 3; ++d) {
                         const int a = d == 0 ? xp : (d == 1 ? yp : zp), b = d == 0 ? xm : (d == 1 ? ym : zm);
                         const real gd = (progress<real>(pf[a], rcp_const<real>(D.k, DUCT_RC_HID)) -
