@@ -138,6 +138,18 @@ def cpu_baseline(preset, nx, ny, nz, over, stages, threads, target_s=10.0, max_s
     return total / steps, cores, steps, total
 
 
+def small_run_baseline(lesgpu, steps=100):
+    """The CPU-runnable configuration whole (64x32x32 duct, WENO5, FP64, 100
+    steps): the oracle on this host's cores and the engine's plain FP64 path,
+    each over the full run."""
+    dt, cores, n, total = cpu_baseline(1, 64, 32, 32, {}, 3, 0, target_s=1e9, max_steps=steps)
+    g = single_run(lesgpu, lesgpu.params(1), steps, 0, timing=False)
+    cells = 64 * 32 * 32
+    return {"grid": [64, 32, 32], "steps": n, "cores": cores, "cpu_s": total,
+            "cpu_value": cells * 3 / dt, "gpu_ms_per_step": g["step_ms"],
+            "gpu_value": cells * 3 / (g["step_ms"] * 1e-3), "unit": "cell-updates/s"}
+
+
 def reference_build_baseline(lesgpu, target_s=8.0, max_steps=400):
     """The reference solver compiled from its own source (oracle/_ref, OpenMP
     build) at its fixed 60x20x40 grid and scheme (RK4, TENO5, 5 species),
@@ -458,6 +470,8 @@ def main():
                                          "oracle C+OpenMP FP64, %.1f s in all" % (n, wl["stages"], wl["nx"], wl["ny"],
                                                                                   wl["nz"], total)}
         if a.workload == "duct128-fp64":
+            log("64x32x32 full run, CPU and GPU")
+            out["cpu_baseline"]["small_run"] = small_run_baseline(lesgpu)
             log("reference build baseline")
             rb = reference_build_baseline(lesgpu)
             if rb:
