@@ -724,15 +724,18 @@ static int slot_monitor(les_gpu* c, int s, les_step_report* rep)
 // ---------------------------------------------------------------------------
 // autotuner: times each (tile width, XCD mapping) candidate on a scratch pool
 // filled with replicas of the probe run (as many as the pool's slots and cells
-// hold, so the timed launches have the pool's occupancy) and keeps the fastest
+// hold, so the timed launches have the pool's occupancy) and keeps the fastest.
+// xcd_only: the pool's tile width stays and only the XCD mapping is tuned (it
+// does not change results: per-tile partial sums are reduced in tile order).
 // ---------------------------------------------------------------------------
-static int pool_autotune(les_gpu* c, const les_params& probe)
+static int pool_autotune(les_gpu* c, const les_params& probe, bool xcd_only = false)
 {
     DuctState* d = c->d;
-    const int tks[4] = {0, 32, 64, 128};
+    int tks[4] = {0, 32, 64, 128};
+    if (xcd_only) tks[0] = d->tk;
     double best = 1e30;
     int btk = d->tk, bx = d->xcd_map;
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < (xcd_only ? 1 : 4); ++a)
         for (int x = 0; x < 2; ++x) {
             const long long pc = (long long)probe.nx * probe.ny * probe.nz;
             long long reps = d->pool_cells / pc;
@@ -918,7 +921,7 @@ int duct_create(les_gpu* c)
     cfg.precision = p.precision;
     // the widest tile row that divides nz (no idle lanes), else flat tiles
     cfg.tile_k = (p.nz % 64 == 0) ? 64 : ((p.nz % 32 == 0) ? 32 : LES_TILE_FLAT);
-    cfg.xcd_map = 1;
+    cfg.xcd_map = 1;        // tuned below
     cfg.point_cap = 4096;
     cfg.probe_cap = p.num_probe_x > 0 ? 64 : 0;
     cfg.stats_per_cell = (p.stats_start_step >= 0 && p.stats_per_cell) ? 1 : 0;
@@ -930,6 +933,14 @@ int duct_create(les_gpu* c)
     if (e != LES_OK) {
         snprintf(c->err, sizeof(c->err), "duct run does not fit its own pool (status %d)", e);
         return e;
+    }
+    // which XCD mapping is faster depends on the grid and precision (FP64 at
+    // 128^3 prefers round-robin tiles, smaller grids and FP32 contiguous
+    // ranges per XCD); results do not depend on it, so each run tunes it (a
+    // failed trial, e.g. no memory for the scratch pool, keeps the default)
+    if (pool_autotune(c, p, true) != LES_OK) {
+        c->d->xcd_map = 1;
+        c->err[0] = 0;
     }
     c->attr_h.assign((size_t)c->g.n, 3);
     return LES_OK;
