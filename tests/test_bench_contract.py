@@ -30,6 +30,10 @@ def test_workloads_have_their_sizes():
     assert (w["duct128-fp64"]["nx"], w["duct128-fp64"]["ny"], w["duct128-fp64"]["nz"]) == (128, 128, 128)
     assert w["duct128-fp64"]["dtype"] == "f64" and w["duct128-fp32"]["dtype"] == "f32"
     assert (w["large256-fp32"]["nx"], w["large256-fp32"]["ny"], w["large256-fp32"]["nz"]) == (256, 192, 192)
+    # one of 8 x-slabs of the large run: 32 planes + two 4-plane halos, 1 mm cells
+    slab = w["slab32-fp32"]
+    assert (slab["nx"], slab["ny"], slab["nz"]) == (256 // 8 + 8, 192, 192)
+    assert abs(slab["over"]["lx"] - slab["nx"] * 1e-3) < 1e-12
     # algorithmic bytes per cell and RK3 stage: 13.33 fields of 8 (4) bytes
     assert abs(w["duct128-fp64"]["bytes_per_cell_stage"] - 106.7) < 0.05
     assert abs(w["duct128-fp32"]["bytes_per_cell_stage"] - 53.3) < 0.05
