@@ -263,7 +263,7 @@ def slab_halo(lesgpu, p, planes=4, reps=50):
             "xgmi_ms_per_face_estimate": face_bytes / (XGMI_LINK_GBS * 1e9) * 1e3}
 
 
-def ensemble_leg(lesgpu, serving, steps, warmup, members=64, grid=(64, 48, 48), profile_steps=50):
+def ensemble_leg(lesgpu, serving, steps, warmup, members=64, grid=(64, 48, 48), profile_steps=1000):
     """64 members in one FP32 pool vs the plain FP64 path one run at a time."""
     ms = serving.ensemble_members(members, grid, seed=0)
     cells = grid[0] * grid[1] * grid[2]
@@ -285,21 +285,28 @@ def ensemble_leg(lesgpu, serving, steps, warmup, members=64, grid=(64, 48, 48), 
     r64 = single_run(lesgpu, p64, steps, warmup, timing=False)
     plain_rph = 3600.0 / (1000 * r64["step_ms"] * 1e-3)
     err = profile_error(lesgpu, serving, grid, profile_steps)
+    err_fixed = profile_error(lesgpu, serving, grid, profile_steps, fixed_dt=FIXED_DT)
     return {"metric": "LES runs/hour (64 members x 1000 steps, 64x48x48, FP32, one pool)",
             "value": runs_per_hour, "ms_per_pool_step": step_ms, "wall_s": wall,
             "cell_updates_per_s_stage": cu, "tuning": tune,
             "plain_fp64_runs_per_hour": plain_rph, "plain_fp64_ms_per_step": r64["step_ms"],
             "ratio_vs_plain_fp64": runs_per_hour / plain_rph,
-            "reynolds_profile_rel_err_fp32_vs_fp64": err}
+            "reynolds_profile_rel_err_fp32_vs_fp64": err,
+            "reynolds_profile_rel_err_fp32_vs_fp64_fixed_dt": err_fixed}
 
 
-def profile_error(lesgpu, serving, grid, steps):
+FIXED_DT = 1.0e-5      # below the CFL step of every 64x48x48 member (1.2e-5 ... 2.6e-5 s)
+
+
+def profile_error(lesgpu, serving, grid, steps, fixed_dt=0.0):
     """Max relative error (per profile column, against its own scale) of the
     FP32 member's mean/Reynolds-stress plane profiles vs the FP64 path after
-    `steps` steps (the FP64 GPU path equals the FP64 oracle within 1e-10)."""
+    `steps` steps (the FP64 GPU path equals the FP64 oracle within 1e-10);
+    fixed_dt > 0 gives both runs that time step."""
     profs = {}
     for prec in (1, 0):
         p = serving.ensemble_members(1, grid, seed=0, precision=prec)[0]
+        p.fixed_dt = fixed_dt
         g = lesgpu.LesGpu(p, lesgpu.mixture())
         try:
             g.init_state()
@@ -310,6 +317,7 @@ def profile_error(lesgpu, serving, grid, steps):
     p = serving.ensemble_members(1, grid, seed=0, precision=0)[0]
     out = serving.profile_errors(profs[1], profs[0], p.inlet_velocity, p.turbulence_intensity)
     out["steps"] = steps
+    out["dt"] = "fixed %.1e s" % fixed_dt if fixed_dt > 0 else "CFL rule, each precision its own"
     return out
 
 
@@ -382,7 +390,8 @@ def served_line(a, lesgpu, serving):
         bad = sum(srv.results[r]["nonfinite"] for r in range(len(reqs)))
         tune = b.tuning()
     cell_steps = sum(p.nx * p.ny * p.nz * n for _, p, n in reqs)
-    err = served_profile_error(lesgpu, serving, probe, 50)
+    err = served_profile_error(lesgpu, serving, probe, 1000)
+    err["fixed_dt"] = served_profile_error(lesgpu, serving, probe, 1000, fixed_dt=FIXED_DT)
     return {"metric": "LES runs/hour + p50/p99 latency (96 served requests, FP32)",
             "value": len(reqs) * 3600.0 / wall, "unit": "runs/hour", "n_gpus": 1,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -396,14 +405,15 @@ def served_line(a, lesgpu, serving):
             "reynolds_profile_rel_err_fp32_vs_fp64": err}
 
 
-def served_profile_error(lesgpu, serving, p, steps):
+def served_profile_error(lesgpu, serving, p, steps, fixed_dt=0.0):
     """The first request's run (its grid and constants) in the FP32 form vs the
     plain FP64 path (which equals the FP64 oracle within 1e-10) after `steps`
-    steps: plane-profile errors per column."""
+    steps: plane-profile errors per column (fixed_dt > 0: both runs take it)."""
     profs = {}
     for prec in (1, 0):
         q = type(p).from_buffer_copy(p)
         q.precision = prec
+        q.fixed_dt = fixed_dt
         g = lesgpu.LesGpu(q, lesgpu.mixture())
         try:
             g.init_state()

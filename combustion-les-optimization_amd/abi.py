@@ -99,6 +99,7 @@ class Params(C.Structure):
         ("stats_start_step", C.c_int),
         ("stats_per_cell", C.c_int),
         ("seed", C.c_uint64),
+        ("fixed_dt", C.c_double),
     ]
 
 

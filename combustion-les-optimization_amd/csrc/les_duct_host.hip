@@ -160,6 +160,7 @@ static DuctConsts make_duct_consts(const les_params& p)
     k.accel_center = p.flame_accel_center; k.weight_x = p.weight_x; k.max_force = p.max_flame_force;
     k.relaxation = p.relaxation_coeff;
     k.courant = p.courant; k.courant_diff = p.courant_diffusion;
+    k.fixed_dt = p.fixed_dt;
     k.table_t0 = p.table_t0; k.table_dt = p.table_dt; k.table_n = p.table_n;
     k.scheme = p.advection;
     k.nvort = p.num_vortices;

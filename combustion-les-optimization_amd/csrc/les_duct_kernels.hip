@@ -53,6 +53,7 @@ struct DuctConsts {
     double accel_strength, accel_width, accel_center, weight_x, max_force;
     double relaxation;
     double courant, courant_diff;
+    double fixed_dt;        // > 0: prescribed DT_sim
     double table_t0, table_dt;
     int table_n;
     int scheme;             // LES_ADV_WENO5 or LES_ADV_MUSCL_MINMOD
@@ -763,7 +764,7 @@ k_duct_rhs_scalar(Pool P, RhsArgs<real> a)
                 real gp = (ntp + nu0) * isc2, gm = (nu0 + ntm) * isc2;
                 real t2 = (gp * (vp1 - v0) - gm * (v0 - vm1)) * id2;
                 sacc = (d == 0) ? t2 : sacc + t2;
-                real t3 = (vp1 - (real)2.0 * v0 + vm1) * id2;
+                real t3 = ((vp1 - v0) - (v0 - vm1)) * id2;
                 lacc = (d == 0) ? t3 : lacc + t3;
                 if (s == 0) grad[d] = (vp1 - vm1) * hid;
             }
@@ -911,9 +912,12 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 for (int m = 0; m < 3; ++m) {
                     const RunBuf<real> U = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
                     real uc = U[q];
-                    real lx = (U[xp] - (real)2.0 * uc + U[xm]) * ix2;
-                    real ly = (U[yp] - (real)2.0 * uc + U[ym]) * iy2;
-                    real lz = (U[zp] - (real)2.0 * uc + U[zm]) * iz2;
+                    // second differences as differences of neighbour differences:
+                    // exact subtractions of close values, so a smooth field's
+                    // small Laplacian keeps its relative precision in FP32
+                    real lx = ((U[xp] - uc) - (uc - U[xm])) * ix2;
+                    real ly = ((U[yp] - uc) - (uc - U[ym])) * iy2;
+                    real lz = ((U[zp] - uc) - (uc - U[zm])) * iz2;
                     const real dm = mu_eff * (lx + ly + lz);
                     df0 = m == 0 ? dm : df0;
                     df1 = m == 1 ? dm : df1;
@@ -1014,7 +1018,7 @@ __global__ void __launch_bounds__(1024) k_duct_dt(Pool P)
     double dtc = k.courant * dx / (maxvel + LES_SMALL);
     double dtn = k.courant_diff * (dx * dx) / (mn + LES_SMALL);
     double dta = k.courant_diff * (dx * dx) / (ma + LES_SMALL);
-    P.dyn[s].dt = fmin(dtc, fmin(dtn, dta));
+    P.dyn[s].dt = k.fixed_dt > 0 ? k.fixed_dt : fmin(dtc, fmin(dtn, dta));
     P.dyn[s].maxvel = maxvel;
 }
 
@@ -1294,9 +1298,9 @@ __device__ __forceinline__ real proj_lap(const Arr& phi, const GridDims& g, cons
 {
     const real ix2 = rcp_const<real>(k, DUCT_RC_ID2), iy2 = rcp_const<real>(k, DUCT_RC_ID2 + 1), iz2 = rcp_const<real>(k, DUCT_RC_ID2 + 2);
     const real p0 = phi[i * (int)g.sx + j * g.nz + kk];
-    real lx = (proj_phi<real>(phi, g, i + 1, j, kk) - (real)2.0 * p0 + proj_phi<real>(phi, g, i - 1, j, kk)) * ix2;
-    real ly = (proj_phi<real>(phi, g, i, j + 1, kk) - (real)2.0 * p0 + proj_phi<real>(phi, g, i, j - 1, kk)) * iy2;
-    real lz = (proj_phi<real>(phi, g, i, j, kk + 1) - (real)2.0 * p0 + proj_phi<real>(phi, g, i, j, kk - 1)) * iz2;
+    real lx = ((proj_phi<real>(phi, g, i + 1, j, kk) - p0) - (p0 - proj_phi<real>(phi, g, i - 1, j, kk))) * ix2;
+    real ly = ((proj_phi<real>(phi, g, i, j + 1, kk) - p0) - (p0 - proj_phi<real>(phi, g, i, j - 1, kk))) * iy2;
+    real lz = ((proj_phi<real>(phi, g, i, j, kk + 1) - p0) - (p0 - proj_phi<real>(phi, g, i, j, kk - 1))) * iz2;
     return lx + ly + lz;
 }
 

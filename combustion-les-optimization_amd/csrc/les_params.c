@@ -146,6 +146,8 @@ int les_params_validate(const les_params* p)
     if (p->num_vortices < 0 || p->num_vortices > 65536) return LES_ERR_PARAM;
     if (p->num_probe_x < 0 || p->num_probe_x > LES_NPROBE_MAX) return LES_ERR_PARAM;
     if (!(p->courant > 0) || !(p->courant_diffusion > 0)) return LES_ERR_PARAM;
+    if (!(p->fixed_dt >= 0) || p->fixed_dt > 1.0) return LES_ERR_PARAM;
+    if (p->mode == LES_MODE_REFERENCE && p->fixed_dt != 0) return LES_ERR_PARAM;
     if (p->mode == LES_MODE_DUCT) {
         if (p->table_n < 2 || !(p->table_dt > 0)) return LES_ERR_PARAM;
         if (!(p->rho_unburnt > 0) || !(p->prandtl > 0)) return LES_ERR_PARAM;

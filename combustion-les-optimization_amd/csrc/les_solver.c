@@ -97,6 +97,7 @@ static const param_name PARAMS[] = {
     PI("reinit_interval", reinit_interval), PI("reinit_iters", reinit_iters), PI("proj_iters", proj_iters),
     PI("stats_start_step", stats_start_step), PI("stats_per_cell", stats_per_cell),
     {"seed", offsetof(les_params, seed), 'u'},
+    PD("fixed_dt", fixed_dt), PD("FIXED_DT", fixed_dt),
 };
 
 static int set_param(les_params* p, const char* name, const char* value)

@@ -144,6 +144,11 @@ typedef struct les_params {
     int    stats_per_cell;       /* 1: also the per-cell accumulators of :3167-3192 */
 
     uint64_t seed;               /* SEM / noise seed */
+
+    /* duct mode: > 0 gives every step this DT_sim instead of the CFL rule of
+     * :4399-4411 (runs that must share a time-step sequence, e.g. an FP32 run
+     * checked against its FP64 twin); 0: the rule */
+    double fixed_dt;
 } les_params;
 
 /* Presets for les_params_default(). */

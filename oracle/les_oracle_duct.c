@@ -711,6 +711,7 @@ static double duct_dt(ora_ctx* c, double* maxvel_out)
     double dtn = c->p.courant_diffusion * (c->dx * c->dx) / (d->max_nu + SMALL);
     double dta = c->p.courant_diffusion * (c->dx * c->dx) / (d->max_alpha + SMALL);
     *maxvel_out = maxvel;
+    if (c->p.fixed_dt > 0) return c->p.fixed_dt;      /* prescribed DT_sim */
     return fmin(dtc, fmin(dtn, dta));
 }
 

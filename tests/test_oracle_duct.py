@@ -188,3 +188,27 @@ def test_reinit_fixed_point_and_convergence(oracle):
     assert np.all(np.abs(g[i0 - 4: i0 + 5] - 1.0) < 0.05)   # slope relaxes to 1 near the front
     # the zero crossing stays in the same cell
     assert np.sign(phi[i0 - 1, 3, 3]) != np.sign(phi[i0 + 1, 3, 3])
+
+
+def test_fixed_dt(oracle):
+    """A prescribed DT_sim replaces the CFL rule: one step takes exactly that
+    dt, and the stiff nudging follows its closed form with it."""
+    T0, fdt = 2000.0, 2.0e-5
+    o, p, st = uniform_duct(oracle, T=T0, fixed_dt=fdt)
+    rep = o.step(1)
+    assert rep.dt == fdt
+    z = -p.relaxation_coeff * fdt
+    want = 2100.0 + (T0 - 2100.0) * (1 + z + z * z / 2 + z ** 3 / 6)
+    assert np.allclose(o.get_state(5)[4][12:-12], want, rtol=0, atol=1e-9 * 2100.0)
+    assert o.step(3).dt == fdt
+
+
+def test_fixed_dt_validation(lesgpu):
+    """fixed_dt must be finite and non-negative, and the reference scheme keeps
+    its own DT rule."""
+    L = lesgpu.lib()
+    ok = lesgpu.params(1, fixed_dt=1e-5)
+    assert L.les_params_validate(C.byref(ok)) == 0
+    for bad in (lesgpu.params(1, fixed_dt=-1e-5), lesgpu.params(1, fixed_dt=float("nan")),
+                lesgpu.params(0, fixed_dt=1e-5)):
+        assert L.les_params_validate(C.byref(bad)) != 0
