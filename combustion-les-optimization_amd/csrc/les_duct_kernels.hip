@@ -875,18 +875,33 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                     real rm = (real)0.5 * (rf0 + M<real>::div(rut, Tf.at(lo[2])));
                     real mp = rp * up, mm = rm * um;
                     const bool band = (d == 0 && xband);
+                    // the line of component m + 1 is loaded while component m is
+                    // reconstructed (in the rolled FP64 loop the loads of the next
+                    // iteration would otherwise wait behind this one's arithmetic);
+                    // the first-order band reads only offsets -1..1, so there the
+                    // outer offsets repeat them and every load stays inside the run
+                    if (band) {
+                        lo[0] = lo[1] = lo[2];
+                        lo[5] = lo[6] = lo[4];
+                    }
+                    real nxt[7];
+#pragma unroll
+                    for (int o = 0; o < 7; ++o) nxt[o] = vel[0].at(lo[o]);
 #pragma unroll MU
                     for (int m = 0; m < 3; ++m) {
-                        const RunBuf<real> V = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
+                        real v[7];
+#pragma unroll
+                        for (int o = 0; o < 7; ++o) v[o] = nxt[o];
+                        if (m < 2) {
+                            const RunBuf<real> Vn = m == 0 ? vel[1] : vel[2];
+#pragma unroll
+                            for (int o = 0; o < 7; ++o) nxt[o] = Vn.at(lo[o]);
+                        }
                         real fp, fm;
                         if (band) {
-                            real v2 = V.at(lo[2]), v3 = V.at(lo[3]), v4 = V.at(lo[4]);
-                            fp = (up >= (real)0.0) ? v3 : v4;
-                            fm = (um >= (real)0.0) ? v2 : v3;
+                            fp = (up >= (real)0.0) ? v[3] : v[4];
+                            fm = (um >= (real)0.0) ? v[2] : v[3];
                         } else {
-                            real v[7];
-#pragma unroll
-                            for (int o = 0; o < 7; ++o) v[o] = V.at(lo[o]);
                             fp = face_plus<real>(scheme, v, up >= (real)0.0);
                             fm = face_minus<real>(scheme, v, um >= (real)0.0);
                         }
