@@ -809,11 +809,10 @@ k_duct_rhs_scalar(Pool P, RhsArgs<real> a)
 // momentum RHS (:2552-2691 structure) + SSP-RK3 combine; in the last stage also
 // the per-tile max |u| of the new state (its T was written by the scalar kernel)
 template <typename real, int TK>
-// FP64: at least 5 waves per SIMD with the component loops rolled (85 VGPRs;
-// unrolled it needs 115 at 4 waves); FP32: 7 waves (71 VGPRs).  The line
-// offsets held across the component loop cost 7 registers: at 6 (FP64) and 8
-// (FP32) waves they would spill.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 5 : 7)))
+// FP64: 4 waves per SIMD with the component loops rolled (102 VGPRs, the
+// line offsets and the prefetched next line included; a 5-wave cap spills 12
+// bytes and is no faster); FP32: 7 waves (71 VGPRs; 8 would spill).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 4 : 7)))
 k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
 {
     __shared__ double vred[8];
