@@ -809,9 +809,8 @@ k_duct_rhs_scalar(Pool P, RhsArgs<real> a)
 // momentum RHS (:2552-2691 structure) + SSP-RK3 combine; in the last stage also
 // the per-tile max |u| of the new state (its T was written by the scalar kernel)
 template <typename real, int TK>
-// FP64: 4 waves per SIMD with the component loops rolled (102 VGPRs, the
-// line offsets and the prefetched next line included; a 5-wave cap spills 12
-// bytes and is no faster); FP32: 7 waves (71 VGPRs; 8 would spill).
+// FP64: 4 waves per SIMD (103 VGPRs, the line offsets and the prefetched next
+// line included); FP32: 7 waves (71 VGPRs; 8 would spill).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 4 : 7)))
 k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
 {
@@ -850,9 +849,8 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 const RunBuf<real> vel[3] = {uf, vf, wf};
                 const real T0 = Tf[q];
                 const real rf0 = M<real>::div(rut, T0);
-                // FP64 keeps the component loops rolled (fewer live registers,
-                // 6 waves per SIMD); FP32 unrolls them
-                constexpr int MU = sizeof(real) == 8 ? 1 : 3;
+                // the direction loop stays rolled (one copy of the component
+                // loop's code and registers); the component loop is unrolled
                 real cv0 = (real)0, cv1 = (real)0, cv2 = (real)0;
 #pragma unroll 1
                 for (int d = 0; d < 3; ++d) {
@@ -875,9 +873,8 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                     real mp = rp * up, mm = rm * um;
                     const bool band = (d == 0 && xband);
                     // the line of component m + 1 is loaded while component m is
-                    // reconstructed (in the rolled FP64 loop the loads of the next
-                    // iteration would otherwise wait behind this one's arithmetic);
-                    // the first-order band reads only offsets -1..1, so there the
+                    // reconstructed, so its loads do not wait behind this one's
+                    // arithmetic; the first-order band reads only offsets -1..1, so there the
                     // outer offsets repeat them and every load stays inside the run
                     if (band) {
                         lo[0] = lo[1] = lo[2];
@@ -886,7 +883,7 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                     real nxt[7];
 #pragma unroll
                     for (int o = 0; o < 7; ++o) nxt[o] = vel[0].at(lo[o]);
-#pragma unroll MU
+#pragma unroll
                     for (int m = 0; m < 3; ++m) {
                         real v[7];
 #pragma unroll
