@@ -690,7 +690,7 @@ struct RhsArgs {
 
 // scalar RHS (phi: G equation :2706-2818; T :2912-2955) + SSP-RK3 combine
 template <typename real, int TK>
-// FP64: at least 4 waves per SIMD (122 VGPRs; a 5-wave cap spills 108 bytes);
+// FP64: at least 4 waves per SIMD (116 VGPRs; a 5-wave cap spills);
 // FP32: 8 waves with flat tiles, 7 with row tiles (8 would spill 16 bytes)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 4 : (TK == 0 ? 8 : 7))))
 k_duct_rhs_scalar(Pool P, RhsArgs<real> a)
