@@ -1463,17 +1463,17 @@ __global__ void __launch_bounds__(256) k_duct_stats(Pool P)
     const long long p0 = (long long)blockIdx.x * chunk;
     const long long p1 = p0 + chunk < PL ? p0 + chunk : PL;
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const real rut = (real)k.rho_u_Tu;
     const auto A0 = run_arr((const real*)P.A[0] + D.off);
     const auto A1 = run_arr((const real*)P.A[1] + D.off);
     const auto A2 = run_arr((const real*)P.A[2] + D.off);
     const auto A3 = run_arr((const real*)P.A[3] + D.off);
     const auto A4 = run_arr((const real*)P.A[4] + D.off);
     const bool per_cell = k.stats_per_cell && P.stat[0];
+    const real irut = rcp_const<real>(k, DUCT_RC_IRUT);
     for (long long p = p0 + threadIdx.x; p < p1; p += 256) {
         long long q = (long long)i * PL + p;
-        real rho = rut / A4[q];
-        double u = (double)(A0[q] / rho), v = (double)(A1[q] / rho), w = (double)(A2[q] / rho);
+        const real ri = A4[q] * irut;                  // 1 / rho = T / (rho_u T_u)
+        double u = (double)(A0[q] * ri), v = (double)(A1[q] * ri), w = (double)(A2[q] * ri);
         double c = (double)progress<real>(A3[q], rcp_const<real>(D.k, DUCT_RC_HID));
         if (per_cell) {
             long long qa = D.off + q;
