@@ -1,7 +1,7 @@
 """Parity of the HIP engine's reference scheme (FP64) against the CPU oracle.
 
-Tolerance: 1e-10 relative to each field's own scale (the north-star bound for
-the plain FP64 path); differences may come only from reduction order (the
+Tolerance: 1e-10 relative to each field's own scale (the bound set for the
+plain FP64 path); differences may come only from reduction order (the
 domain-mean density) and device-libm ulps in exp/log/pow.  Initial states are
 compared bit-exactly.
 """
