@@ -482,15 +482,16 @@ __device__ __forceinline__ int line_idx(const GridDims& g, int bi, int j, int kk
     return bi + j * g.nz + wrapi(kk + o, g.nz);
 }
 
+// Fixed-order sum of one value per thread of a 256-thread block: butterfly
+// within each wave (lane 0's association order is fixed), then the four wave
+// sums in order; every thread gets the total.
 __device__ __forceinline__ double block_sum256(double v, double* red)
 {
-    red[threadIdx.x] = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    double r = red[0];
+    const double r = ((red[0] + red[1]) + red[2]) + red[3];
     __syncthreads();
     return r;
 }
