@@ -437,7 +437,7 @@ static int ref_step(les_gpu* c, les_step_report* rep, bool last)
     HIPC(hipMemcpyAsync(r.noise, c->noise_h.data(), 3 * plane * sizeof(double), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_ref_sem_plane, dim3((g.nz + 63) / 64, g.ny), dim3(64), 0, c->stream, g, r.attr, r.vort,
                        c->p.num_vortices, r.inlet);
-    hipLaunchKernelGGL(k_ref_sem_normalise, dim3(1), dim3(64), 0, c->stream, g, r.attr, r.inlet,
+    hipLaunchKernelGGL(k_ref_sem_normalise, dim3(1), dim3(SEM_NORM_THREADS), 0, c->stream, g, r.attr, r.inlet,
                        c->p.inlet_velocity * c->p.turbulence_intensity);
     LAUNCH_CHECK();
 

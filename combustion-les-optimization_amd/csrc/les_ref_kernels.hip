@@ -850,32 +850,75 @@ __global__ void __launch_bounds__(256) k_ref_sem_plane(GridDims g, const int* at
 
 // Serial (j, k)-order sums, as in the reference, by one thread: the plane has
 // ny*nz cells (800 on the reference grid), so order fidelity is cheap here.
-__global__ void k_ref_sem_normalise(GridDims g, const int* attr, double* inlet, double target)
+// The reference's serial sums over the inlet cells, kept bit for bit (each
+// component's sum runs in plane-index order, as :1017-1062 do): the block
+// stages the plane in LDS chunks with all its threads and threads 0..2 each
+// run one component's sum over the chunk; the mean subtraction and the rms
+// scaling, elementwise, are spread over the block.  One block of
+// SEM_NORM_THREADS threads.
+#define SEM_NORM_THREADS 256
+#define SEM_NORM_CHUNK 2048
+__global__ void __launch_bounds__(SEM_NORM_THREADS) k_ref_sem_normalise(GridDims g, const int* attr, double* inlet,
+                                                                       double target)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    long long P = (long long)g.ny * g.nz;
-    double* U = inlet;
-    double* V = inlet + P;
-    double* W = inlet + 2 * P;
-    double tu = 0, tv = 0, tw = 0;
-    int nin = 0;
-    for (long long pl = 0; pl < P; ++pl)
-        if (attr[pl] == 1) { tu += U[pl]; tv += V[pl]; tw += W[pl]; nin++; }
-    if (nin > 0) {
-        double au = tu / nin, av = tv / nin, aw = tw / nin;
-        for (long long pl = 0; pl < P; ++pl)
-            if (attr[pl] == 1) { U[pl] -= au; V[pl] -= av; W[pl] -= aw; }
-    }
-    double qu = 0, qv = 0, qw = 0;
-    for (long long pl = 0; pl < P; ++pl)
-        if (attr[pl] == 1) { qu += U[pl] * U[pl]; qv += V[pl] * V[pl]; qw += W[pl] * W[pl]; }
-    if (nin > 0) {
-        double ru = sqrt(qu / nin), rv = sqrt(qv / nin), rw = sqrt(qw / nin);
-        double su = (ru > LES_SMALL) ? target / ru : 0.0;
-        double sv = (rv > LES_SMALL) ? target / rv : 0.0;
-        double sw = (rw > LES_SMALL) ? target / rw : 0.0;
-        for (long long pl = 0; pl < P; ++pl)
-            if (attr[pl] == 1) { U[pl] *= su; V[pl] *= sv; W[pl] *= sw; }
+    __shared__ double sv[3][SEM_NORM_CHUNK];
+    __shared__ int sa[SEM_NORM_CHUNK];
+    __shared__ double tot[3];
+    __shared__ int nin_s;
+    const long long P = (long long)g.ny * g.nz;
+    double* const C[3] = {inlet, inlet + P, inlet + 2 * P};
+    const int t = (int)threadIdx.x;
+    for (int pass = 0; pass < 2; ++pass) {
+        // pass 0: sums of u', v', w' and the inlet-cell count; pass 1: sums of squares
+        double acc = 0.0;
+        int nin = 0;
+        for (long long c0 = 0; c0 < P; c0 += SEM_NORM_CHUNK) {
+            const int n = (int)(P - c0 < SEM_NORM_CHUNK ? P - c0 : SEM_NORM_CHUNK);
+            for (int e = t; e < n; e += SEM_NORM_THREADS) {
+                sa[e] = attr[c0 + e];
+                sv[0][e] = C[0][c0 + e];
+                sv[1][e] = C[1][c0 + e];
+                sv[2][e] = C[2][c0 + e];
+            }
+            __syncthreads();
+            if (t < 3) {
+                const double* x = sv[t];
+                if (pass == 0) {
+                    for (int e = 0; e < n; ++e)
+                        if (sa[e] == 1) { acc += x[e]; nin++; }
+                } else {
+                    for (int e = 0; e < n; ++e)
+                        if (sa[e] == 1) acc += x[e] * x[e];
+                }
+            }
+            __syncthreads();
+        }
+        if (t < 3) tot[t] = acc;
+        if (t == 0 && pass == 0) nin_s = nin;
+        __syncthreads();
+        const int nn = nin_s;
+        if (nn > 0) {
+            double f[3];
+            if (pass == 0) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) f[c] = tot[c] / nn;                 // means
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double r = sqrt(tot[c] / nn);
+                    f[c] = (r > LES_SMALL) ? target / r : 0.0;              // rms scale factors
+                }
+            }
+            for (long long pl = t; pl < P; pl += SEM_NORM_THREADS)
+                if (attr[pl] == 1) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        if (pass == 0) C[c][pl] -= f[c];
+                        else C[c][pl] *= f[c];
+                    }
+                }
+        }
+        __syncthreads();
     }
 }
 
