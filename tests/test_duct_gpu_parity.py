@@ -174,6 +174,29 @@ def test_duct_projection_fp64(lesgpu, oracle):
         o.close()
 
 
+def test_duct_projection_fp32(lesgpu, oracle):
+    """FP32 form with the projection relaxation and reinitialisation against
+    the FP64 oracle (the residual reductions stay FP64): the residual still
+    falls every step and the state stays within the FP32 bound."""
+    print("case: duct64 FP32 with projection relaxation (5 sweeps), 6 steps", flush=True)
+    g, o = make_pair(lesgpu, oracle, 1, precision=1, proj_iters=5, reinit_interval=3, reinit_iters=2)
+    try:
+        g.init_state()
+        o.init_state()
+        for s in range(6):
+            rg = g.step(1)
+            ro = o.step(1)
+            assert rg.proj_residual < rg.proj_residual0, s
+            assert abs(rg.proj_residual0 - ro.proj_residual0) <= 1e-3 * ro.proj_residual0, s
+        errs = {nm: rel_err(a, b) for nm, a, b in zip(DUCT, g.download_state(5), o.get_state(5))}
+        print("FP32 rel errors with projection after 6 steps:", errs)
+        for nm, e in errs.items():
+            assert e <= 5e-4, (nm, e)
+    finally:
+        g.close()
+        o.close()
+
+
 def wrinkled_front(o, p, amp=3e-3):
     """Duct initial state with the planar front wrinkled along y, so the probe
     lines (k = nz/2) cross G = 0.5."""
