@@ -1073,6 +1073,100 @@ int duct_rhs(les_gpu* c, double* const* out, int n_out)
 }
 
 // ---------------------------------------------------------------------------
+// The same RHS hook without device allocation.  The five RHS fields are staged
+// in `scratch`, a file-scope device array of LES_RHS_NOALLOC_MAX_BYTES that
+// each GPU holds from module load to process end.  In host code the array's
+// name is only a host-side stand-in, so its device address is looked up once
+// per device (hipGetSymbolAddress with that device current) and only that
+// address is handed to kernels and copies.  Every call on one GPU shares the
+// array: the staged fields are cleared by a fill kernel at the start of each
+// call, and calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[LES_RHS_NOALLOC_MAX_BYTES];
+
+static constexpr int kScratchDevices = 64;
+static void* g_scratch_dev[kScratchDevices];   // device address of `scratch` per device
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the device of c's stream.
+static int scratch_address(les_gpu* c, unsigned char** out)
+{
+    int dev = -1;
+    DHIPC(hipStreamGetDevice(c->stream, &dev));
+    if (dev < 0 || dev >= kScratchDevices) {
+        snprintf(c->err, sizeof(c->err), "device %d outside the %d devices the RHS scratch tracks", dev, kScratchDevices);
+        return LES_ERR_STATE;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        int cur = -1;
+        DHIPC(hipGetDevice(&cur));
+        if (cur != dev) DHIPC(hipSetDevice(dev));
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) {
+            snprintf(c->err, sizeof(c->err), "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                     hipGetErrorString(e));
+            return LES_ERR_HIP;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    *out = (unsigned char*)g_scratch_dev[dev];
+    return LES_OK;
+}
+
+__global__ void k_zero_words(long long n_words, unsigned int* dst)
+{
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n_words;
+         q += (long long)gridDim.x * blockDim.x)
+        dst[q] = 0u;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of scratch the duct RHS of this context stages (five fields, each
+// starting on a 256-byte boundary).
+static size_t duct_rhs_scratch_bytes(const les_gpu* c)
+{
+    const DuctState* d = c->d;
+    return DNF * align256((size_t)d->desc[0].g.n * d->esize);
+}
+
+int duct_rhs_noalloc(les_gpu* c, double* const* out, int n_out)
+{
+    if (n_out < DNF) return LES_ERR_PARAM;
+    DuctState* d = c->d;
+    const SlotDesc& D = d->desc[0];
+    const size_t n = (size_t)D.g.n;
+    const size_t need = duct_rhs_scratch_bytes(c);
+    if (need > (size_t)LES_RHS_NOALLOC_MAX_BYTES) {
+        snprintf(c->err, sizeof(c->err),
+                 "les_gpu_rhs_noalloc: a %zu-cell grid stages %zu bytes, above the %zu-byte static scratch "
+                 "(use les_gpu_rhs)", n, need, (size_t)LES_RHS_NOALLOC_MAX_BYTES);
+        return LES_ERR_PARAM;
+    }
+    unsigned char* base = nullptr;
+    int e = scratch_address(c, &base);
+    if (e != LES_OK) return e;
+    // boundary planes are not written by the RHS kernels and read as 0
+    const long long words = (long long)(need / 4);
+    const int zb = (int)std::min<long long>((words + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_zero_words, dim3(zb), dim3(256), 0, c->stream, words, (unsigned int*)base);
+    DLAUNCH_CHECK();
+    void* Lbuf[DNF];
+    for (int f = 0; f < DNF; ++f) Lbuf[f] = base + (size_t)f * align256(n * d->esize);
+    e = tk_dispatch(d->tk, [&](auto K) {
+        constexpr int TK = decltype(K)::value;
+        if (d->precision == LES_PREC_FP32) return stage_launch<float, TK>(c, 0, (float* const*)Lbuf);
+        return stage_launch<double, TK>(c, 0, (double* const*)Lbuf);
+    });
+    for (int f = 0; f < DNF && e == LES_OK; ++f) e = slot_xfer(c, 0, (char*)Lbuf[f] - (size_t)D.off * d->esize, out[f], false);
+    DHIPC(hipStreamSynchronize(c->stream));
+    return e;
+}
+
+// ---------------------------------------------------------------------------
 // batch ABI
 // ---------------------------------------------------------------------------
 struct les_batch {

@@ -15,6 +15,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "les_engine.hpp"
@@ -714,6 +716,17 @@ extern "C" int les_gpu_rhs(les_gpu* c, int t_loop, double* const* out, int n_out
         for (int f = 0; f < LES_NRHS; ++f) HIPC(hipMemcpyAsync(out[f], c->r.k[0][f], b, hipMemcpyDeviceToHost, c->stream));
         HIPC(hipStreamSynchronize(c->stream));
         return LES_OK;
+    });
+}
+
+extern "C" int les_gpu_rhs_noalloc(les_gpu* c, int t_loop, double* const* out, int n_out)
+{
+    return abi_guard(c, [&]() -> int {
+        if (!c || !out || n_out < LES_NRHS) return LES_ERR_PARAM;
+        if (c->p.mode == LES_MODE_DUCT) return duct_rhs_noalloc(c, out, n_out);
+        // the reference scheme evaluates into the context's own stage buffers
+        // and allocates nothing per call: the same path as les_gpu_rhs
+        return les_gpu_rhs(c, t_loop, out, n_out);
     });
 }
 

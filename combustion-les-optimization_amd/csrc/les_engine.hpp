@@ -106,6 +106,7 @@ int  duct_download_state(les_gpu* c, double* const* f, int nf);
 int  duct_download_field(les_gpu* c, const char* name, double* out);
 int  duct_step(les_gpu* c, int n_steps, les_step_report* rep);
 int  duct_rhs(les_gpu* c, double* const* out, int n_out);
+int  duct_rhs_noalloc(les_gpu* c, double* const* out, int n_out);
 void duct_params_changed(les_gpu* c);
 int  duct_point_fetch(les_gpu* c, double* out, int max_steps, int* n_steps);
 int  duct_set_points(les_gpu* c, const int* cells, int n);

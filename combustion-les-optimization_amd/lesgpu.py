@@ -46,6 +46,7 @@ def lib():
         L.les_gpu_download_field.argtypes = [vp, C.c_char_p, P(C.c_double)]
         L.les_gpu_step.argtypes = [vp, C.c_int, P(abi.StepReport)]
         L.les_gpu_rhs.argtypes = [vp, C.c_int, P(P(C.c_double)), C.c_int]
+        L.les_gpu_rhs_noalloc.argtypes = [vp, C.c_int, P(P(C.c_double)), C.c_int]
         L.les_gpu_probe_fetch.argtypes = [vp, P(abi.ProbeRow), C.c_int, P(C.c_int)]
         L.les_gpu_set_params.argtypes = [vp, P(abi.Params)]
         L.les_gpu_get_step.argtypes = [vp]
@@ -173,6 +174,14 @@ class LesGpu:
     def rhs(self, t_loop=0):
         out = [np.zeros(self.n) for _ in range(10)]
         self._check(lib().les_gpu_rhs(self.h, t_loop, _ptrs(out), 10), "les_gpu_rhs")
+        return [o.reshape(self.shape) for o in out]
+
+    def rhs_noalloc(self, t_loop=0, fill=0.0):
+        """les_gpu_rhs_noalloc: the same RHS staged in the library's static
+        device scratch (no device allocation); ``fill`` pre-sets the host
+        outputs so a test can see that every value was written."""
+        out = [np.full(self.n, fill) for _ in range(10)]
+        self._check(lib().les_gpu_rhs_noalloc(self.h, t_loop, _ptrs(out), 10), "les_gpu_rhs_noalloc")
         return [o.reshape(self.shape) for o in out]
 
     def probe_rows(self, max_rows=1 << 16):

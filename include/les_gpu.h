@@ -210,6 +210,22 @@ int les_gpu_step(les_gpu* ctx, int n_steps, les_step_report* rep);
  * 5 + LES_NSPECIES fields in the order drhou, drhov, drhow, dT, dG, dY0..dY4. */
 int les_gpu_rhs(les_gpu* ctx, int t_loop, double* const* out, int n_out);
 
+/* Largest device staging les_gpu_rhs_noalloc uses: 32 MiB, i.e. duct grids up
+ * to 838 860 cells in FP64 or 1 677 721 cells in FP32 (five RHS fields). */
+#define LES_RHS_NOALLOC_MAX_BYTES (32u << 20)
+/* les_gpu_rhs without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory between steps (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  les_gpu_rhs stages the duct RHS in buffers it allocates and
+ * frees on every call; this variant stages it in a static device array of
+ * LES_RHS_NOALLOC_MAX_BYTES that each GPU holds until the process ends, and
+ * returns LES_ERR_PARAM (with a message in les_gpu_last_error) for a grid
+ * whose staging would exceed it.  Results are identical to les_gpu_rhs.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (contexts
+ * used from several host threads at once) race.  The reference scheme
+ * allocates nothing per call already and takes the les_gpu_rhs path. */
+int les_gpu_rhs_noalloc(les_gpu* ctx, int t_loop, double* const* out, int n_out);
+
 /* Probe rows recorded since the last fetch (up to max_rows); *n_rows = count. */
 int les_gpu_probe_fetch(les_gpu* ctx, les_probe_row* rows, int max_rows, int* n_rows);
 /* Fixed-point time series recorded since the last fetch (the newest 4096
