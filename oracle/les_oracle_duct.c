@@ -5,7 +5,9 @@
  * mode has no counterpart in the reference (its configs ask for a periodic
  * duct, SSP-RK3, WENO5/MUSCL-minmod, cp/mu tables, Gaussian SEM eddies and a
  * level-set G), so this file is the specification; every operator is pinned by
- * closed forms in tests/test_oracle_duct.py.  The RHS keeps the reference's
+ * closed forms in tests/test_oracle_duct.py, tests/test_oracle_duct_closed_forms.py
+ * and tests/test_oracle_projection.py, and oracle/mutation_check.py shows that
+ * each of 39 planted mistakes fails at least one of them.  The RHS keeps the reference's
  * structure (calculate_RHS, les_combustion_solver.c:2494-2963): box filter,
  * Smagorinsky nu_t with Cs(G) (:1569-1633), SSR turbulent flame speed
  * (:1913-1965), conservative momentum with buoyancy and the flame-acceleration
