@@ -1,8 +1,9 @@
 """Served and ensemble LES runs on one GPU over the les_batch_* C ABI.
 
-The reference's optimiser runs one simulation at a time: it pastes the six
-tunable constants into the solver's #defines, recompiles and waits for the
-run (bayesian_optimizer.py:189-230, :574-581, :703-716).  Here a request is a
+The reference's optimiser drives one simulation at a time: its interactive
+main (bayesian_optimizer.py:719-844) scores one finished run and prints the
+next six constants as #defines (:703-716, bounds :574-581), which a human
+pastes into the solver, recompiles and runs.  Here a request is a
 parameter set plus a step count; requests are admitted into a pool of
 device-resident runs as slots and arena cells free up (admission by HBM
 footprint: a run needs its cells and tiles in the pool's arenas), every pool

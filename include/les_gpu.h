@@ -275,8 +275,9 @@ int les_abi_struct_sizes(size_t out[6]);
 
 /* ------------------------------------------------------------------------
  * Batched runs (duct mode).  The reference runs one simulation per process
- * (main :4303-4700) and its optimiser drives it as a subprocess
- * (bayesian_optimizer.py:189-230, one trial at a time).  A batch context holds
+ * (main :4016-4632), one optimiser trial at a time: the optimiser's
+ * interactive main (bayesian_optimizer.py:719-844) scores a finished run and
+ * prints the next constants as #defines for the next build.  A batch context holds
  * a POOL of run slots sharing field arenas in device memory; every step
  * launch advances all resident, unfinished runs at once, so an ensemble of
  * trials, or a stream of requests admitted and retired while others run
