@@ -84,8 +84,11 @@ static void composition(les_gpu* c)
     c->st_h2o = (4.0 * sp[4].mw) / sp[0].mw;
 }
 
-// Attribute map of a new run (setting, :1152-1181).
-static void attribute_map(les_gpu* c, std::vector<int>& a)
+// Attribute map of a new run (setting, :1152-1181), or with kind 1 the map
+// main() sets before its restart detection (:4123-4136: no symmetry plane at
+// j = 0, no rim), which the reference's binary resume keeps because it skips
+// setting() (:4184-4190).
+static void attribute_map(les_gpu* c, std::vector<int>& a, int kind = 0)
 {
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
     const double R = c->p.inlet_radius, zc = c->p.lz / 2.0, rrim = 0.005;
@@ -100,8 +103,9 @@ static void attribute_map(les_gpu* c, std::vector<int>& a)
                     if (d2 <= pow(R, 2)) v = 1;
                     else if (d2 <= pow(rrim, 2)) v = 6;
                     else v = 4;
+                    if (kind == 1 && v == 6) v = 4;
                 } else if (i == nx - 1) v = 2;
-                else if (j == 0) v = 5;
+                else if (j == 0) v = (kind == 1) ? 4 : 5;
                 else if (j == ny - 1 || k == 0 || k == nz - 1) v = 4;
                 else v = 3;
                 a[((size_t)i * ny + j) * nz + k] = v;
@@ -539,11 +543,25 @@ int monitor(les_gpu* c, const double* rho, const double* T, const double* G, con
 // ---------------------------------------------------------------------------
 // ABI entry points
 // ---------------------------------------------------------------------------
+// Rebuilds the host and device attribute maps of a reference-scheme run.
+static int set_attribute_map(les_gpu* c, int kind)
+{
+    attribute_map(c, c->attr_h, kind);
+    HIPC(hipMemcpy(c->r.attr, c->attr_h.data(), c->attr_h.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->attr_map = kind;
+    return LES_OK;
+}
+
 extern "C" int les_gpu_init_state(les_gpu* c)
 {
     return abi_guard(c, [&]() -> int {
         if (!c) return LES_ERR_PARAM;
         if (c->p.mode == LES_MODE_DUCT) return duct_init_state(c);
+        // a new run takes setting()'s attribute map (:1152-1181)
+        if (c->attr_map != 0) {
+            int e = set_attribute_map(c, 0);
+            if (e != LES_OK) return e;
+        }
         // initial fields of a new run (setting :1184-1261) on the host, then upload
         const size_t n = (size_t)c->g.n;
         const int ny = c->p.ny, nz = c->p.nz;
@@ -882,12 +900,13 @@ extern "C" int les_gpu_set_stats_count(les_gpu* c, int count)
     });
 }
 
-static const char AUX_MAGIC[8] = {'L', 'E', 'S', 'A', 'U', 'X', '1', 0};
+static const char AUX_MAGIC[8] = {'L', 'E', 'S', 'A', 'U', 'X', '2', 0};
 
+// Every aux blob starts with magic[8] "LESAUX<version>" and int mode, int step.
 struct AuxRef {
     char magic[8];
     int mode, step, stat_count, nvort;
-    int rand_type, rand_deg, rand_sep;
+    int rand_type, rand_deg, rand_sep, attr_map;
     long long fptr, rptr, state, end;     // offsets into the state bytes
     char rstate[128];
 };
@@ -905,6 +924,7 @@ extern "C" int les_gpu_export_aux(les_gpu* c, void* buf, size_t* bytes)
         memcpy(a.magic, AUX_MAGIC, 8);
         a.mode = LES_MODE_REFERENCE; a.step = c->step; a.stat_count = c->stat_count; a.nvort = c->p.num_vortices;
         a.rand_type = c->rd.rand_type; a.rand_deg = c->rd.rand_deg; a.rand_sep = c->rd.rand_sep;
+        a.attr_map = c->attr_map;
         const char* base = c->rstate;
         a.fptr = (const char*)c->rd.fptr - base; a.rptr = (const char*)c->rd.rptr - base;
         a.state = (const char*)c->rd.state - base; a.end = (const char*)c->rd.end_ptr - base;
@@ -932,6 +952,11 @@ extern "C" int les_gpu_import_aux(les_gpu* c, const void* buf, size_t bytes)
         if (a.fptr < 0 || a.fptr >= lim || a.rptr < 0 || a.rptr >= lim || a.state < 0 || a.state >= lim || a.end < 0 ||
             a.end > lim)
             return set_err(c, LES_ERR_PARAM, "aux blob holds an invalid random-state layout");
+        if (a.attr_map != 0 && a.attr_map != 1) return set_err(c, LES_ERR_PARAM, "aux blob holds an invalid attribute map");
+        if (a.attr_map != c->attr_map) {
+            int e = set_attribute_map(c, a.attr_map);
+            if (e != LES_OK) return e;
+        }
         memcpy(c->rstate, a.rstate, sizeof(c->rstate));
         c->rd.rand_type = a.rand_type; c->rd.rand_deg = a.rand_deg; c->rd.rand_sep = a.rand_sep;
         c->rd.fptr = (int32_t*)(c->rstate + a.fptr); c->rd.rptr = (int32_t*)(c->rstate + a.rptr);
@@ -967,6 +992,28 @@ extern "C" int les_gpu_init_sem(les_gpu* c)
         init_vortices(c);
         return LES_OK;
     });
+}
+
+// The reference's restart paths (binary :4184-4190, VTK :4269-4280) skip
+// setting() and keep main()'s attribute map (:4123-4136).
+extern "C" int les_gpu_use_resume_attributes(les_gpu* c)
+{
+    return abi_guard(c, [&]() -> int {
+        if (!c) return LES_ERR_PARAM;
+        if (c->p.mode == LES_MODE_DUCT)
+            return set_err(c, LES_ERR_PARAM, "the duct scheme has one attribute map (inflow, fluid, outflow)");
+        return set_attribute_map(c, 1);
+    });
+}
+
+extern "C" int les_aux_step(const void* buf, size_t bytes, int* step)
+{
+    if (!buf || !step || bytes < 16 || memcmp(buf, "LESAUX", 6) != 0) return LES_ERR_PARAM;
+    int mode = 0;
+    memcpy(&mode, (const char*)buf + 8, sizeof(int));
+    if (mode != LES_MODE_REFERENCE && mode != LES_MODE_DUCT) return LES_ERR_PARAM;
+    memcpy(step, (const char*)buf + 12, sizeof(int));
+    return LES_OK;
 }
 
 // ---------------------------------------------------------------------------

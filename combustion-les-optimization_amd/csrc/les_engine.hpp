@@ -55,6 +55,7 @@ struct les_gpu {
     std::vector<double> vort;      // num_vortices x 7
     std::vector<double> noise_h;   // 3 per inlet-plane cell
     std::vector<int> attr_h;
+    int attr_map = 0;              // 0: setting()'s map (:1152-1181); 1: main()'s, kept by a binary resume (:4123-4136)
     std::vector<les_probe_row> probe;
     int stat_count = 0;
     // timing (HIP events on the engine's stream)

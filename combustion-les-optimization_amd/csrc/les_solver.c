@@ -28,7 +28,10 @@
  *   - checkpoint_aux.bin beside checkpoint.bin holds the state the reference
  *     leaves out (SEM/RNG state, DT history, step): a resumed run continues
  *     exactly; the final checkpoint records the real step (the reference
- *     writes step 0 there, :4623);
+ *     writes step 0 there, :4623); a checkpoint.bin without a matching
+ *     checkpoint_aux.bin (one the reference wrote, or a stale aux blob)
+ *     resumes as the reference's own resume does: main()'s attribute map
+ *     (:4123-4136), fresh SEM eddies (:4190), no DT history;
  *   - duct-mode runs (mode B) write the same files; their checkpoint.bin
  *     holds rho*u, rho*v, rho*w, phi, T and restart_info.txt starts "DUCT".
  */
@@ -324,13 +327,23 @@ static int load_checkpoint(run* r, const options* o)
     size_t ab = 0;
     les_gpu_export_aux(r->g, NULL, &ab);
     size_t extra = duct(r) ? 0 : 2 * r->n * 8;
-    if (aux && bytes == ab + extra && les_gpu_import_aux(r->g, aux, ab) == LES_OK) {
+    int aux_step = -1;
+    if (aux && bytes == ab + extra && les_aux_step(aux, ab, &aux_step) == LES_OK && aux_step == step &&
+        les_gpu_import_aux(r->g, aux, ab) == LES_OK) {
         if (!duct(r)) {
             les_gpu_upload_field(r->g, "nu", (const double*)(aux + ab));
             les_gpu_upload_field(r->g, "alpha", (const double*)(aux + ab + r->n * 8));
         }
     } else {
+        /* a checkpoint without (or with a stale) aux blob, e.g. one the
+         * reference wrote: resume as the reference does (:4184-4190) --
+         * main()'s attribute map, fresh SEM eddies, no DT history */
+        if (aux && aux_step >= 0 && aux_step != step)
+            printf("checkpoint_aux.bin is from step %d, checkpoint.bin from step %d: ignoring the aux state.\n",
+                   aux_step, step);
         printf("No matching checkpoint_aux.bin: SEM state and DT history start fresh.\n");
+        if (!duct(r)) les_gpu_use_resume_attributes(r->g);
+        les_gpu_init_sem(r->g);
     }
     free(aux);
     return step;
@@ -759,6 +772,7 @@ int main(int argc, char* argv[])
         } else {
             load_stats(&r, &o);
             les_gpu_set_step(r.g, start_step);
+            if (!duct(&r)) les_gpu_use_resume_attributes(r.g);   /* setting() is skipped (:4269-4280) */
             les_gpu_init_sem(r.g);                        /* no SEM state in a VTK: fresh eddies */
         }
     }

@@ -47,6 +47,8 @@ def lib():
         L.les_gpu_step.argtypes = [vp, C.c_int, P(abi.StepReport)]
         L.les_gpu_rhs.argtypes = [vp, C.c_int, P(P(C.c_double)), C.c_int]
         L.les_gpu_rhs_noalloc.argtypes = [vp, C.c_int, P(P(C.c_double)), C.c_int]
+        L.les_gpu_use_resume_attributes.argtypes = [vp]
+        L.les_aux_step.argtypes = [vp, C.c_size_t, P(C.c_int)]
         L.les_gpu_probe_fetch.argtypes = [vp, P(abi.ProbeRow), C.c_int, P(C.c_int)]
         L.les_gpu_set_params.argtypes = [vp, P(abi.Params)]
         L.les_gpu_get_step.argtypes = [vp]

@@ -254,6 +254,17 @@ int les_gpu_import_aux(les_gpu* ctx, const void* buf, size_t bytes);
  * eddies (initialize_vortices :861-877, as the reference's restart does). */
 int les_gpu_set_step(les_gpu* ctx, int step);
 int les_gpu_init_sem(les_gpu* ctx);
+/* Reference scheme: give the run the attribute map main() sets before its
+ * restart detection (:4123-4136: the j = 0 plane is a wall, not the symmetry
+ * plane of setting :1171-1172), which the reference's restart paths keep
+ * because they skip setting() (binary :4184-4190, VTK :4269-4280).  A resume
+ * from files the reference wrote (no aux blob) calls it to continue as the
+ * reference does; les_gpu_init_state restores setting()'s map, and the aux
+ * blob records which map a run uses.  LES_ERR_PARAM in duct mode. */
+int les_gpu_use_resume_attributes(les_gpu* ctx);
+/* Step recorded in a les_gpu_export_aux blob, to match it to the step in
+ * checkpoint.bin (load_quick_checkpoint :3701-3737) before importing it. */
+int les_aux_step(const void* buf, size_t bytes, int* step);
 /* Change the optimiser-tunable constants between steps. */
 int les_gpu_set_params(les_gpu* ctx, const les_params* p);
 /* Host-visible SEM state (mode R keeps the eddies and glibc rand() stream on the host). */

@@ -13,7 +13,13 @@ harness oracle/ref_harness.c and stores small vectors:
   the 10 RHS fields and 8 intermediates at the sampled cells plus sums;
 * the probe CSV rows record_flame_region_data writes for that state.
 
-Usage:  python tests/golden/make_golden.py     (needs oracle/_ref built)
+With the argument ``probe-csv`` it writes tests/golden/reference_probe_csv.npz
+instead: the text of the seven probe_data_flame_region_loc{n}.csv files and of
+simulation_monitor.csv that the reference writes over a fresh 10-step run of
+seed 0 (rows at steps 3-9: the initial state is burnt everywhere, :69, so the
+G = 0.5 crossing first appears after the inflow has entered).
+
+Usage:  python tests/golden/make_golden.py [probe-csv]    (needs oracle/_ref built)
 """
 import glob
 import os
@@ -91,5 +97,22 @@ def main():
     print("wrote %s (%d bytes)" % (OUT, os.path.getsize(OUT)))
 
 
+def probe_csv_fixture(steps=10):
+    out_path = os.path.join(HERE, "reference_probe_csv.npz")
+    with tempfile.TemporaryDirectory() as wd:
+        po.run_reference(wd, steps=steps, seed=0, dump=[])
+        out = {"steps": np.array([steps]), "seed": np.array([0])}
+        for f in sorted(glob.glob(os.path.join(wd, "*probe_data_flame_region_loc*.csv"))):
+            n = int(f.rsplit("loc", 1)[1].split(".")[0])
+            out["loc%d" % n] = np.array(open(f).read())
+        mon = glob.glob(os.path.join(wd, "*simulation_monitor.csv"))
+        out["monitor"] = np.array(open(mon[0]).read())
+    np.savez_compressed(out_path, **out)
+    print("wrote %s (%d bytes)" % (out_path, os.path.getsize(out_path)))
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "probe-csv":
+        probe_csv_fixture()
+    else:
+        main()
