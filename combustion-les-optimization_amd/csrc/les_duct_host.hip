@@ -758,6 +758,15 @@ static int pool_autotune(les_gpu* c, const les_params& probe, bool xcd_only = fa
             t->p = probe;
             t->stream = c->stream;
             int e = pool_create(t, cfg);
+            if (e != LES_OK) {
+                // the candidate's scratch pool does not fit beside the real
+                // one: skip it (the pool keeps its configured launch shape
+                // if no candidate fits)
+                t->stream = nullptr;
+                les_gpu_destroy(t);
+                (void)hipGetLastError();
+                continue;
+            }
             for (long long r = 0; r < reps && e == LES_OK; ++r) {
                 int slot = -1;
                 e = pool_admit(t, probe, 0, &slot);
@@ -1200,6 +1209,8 @@ extern "C" int les_batch_pool_bytes(const les_batch_config* cfg, size_t* bytes)
                       10 * (size_t)cfg->max_nx * STAT_CHUNKS);
         b += S * (sizeof(SlotDesc) + sizeof(SlotDyn)) + 16 * (size_t)cfg->table_n + 8 * 6 * 1024;
         b += S * 8 * 3 * (size_t)cfg->max_ny * cfg->max_nz * ((nv + SEM_CHUNK - 1) / SEM_CHUNK);   // SEM partials
+        if (cfg->probe_cap > 0)                                   // flame-region probe rings
+            b += S * 8 * (size_t)cfg->probe_cap * LES_NPROBE_MAX * (2 + 6 * (size_t)cfg->max_ny);
         *bytes = b;
         return LES_OK;
     });
@@ -1313,14 +1324,31 @@ extern "C" int les_batch_step(les_batch* b, int n_steps, double* ms_per_step)
     return abi_guard(b ? b->c : nullptr, [&]() -> int {
         if (!b || n_steps < 0) return LES_ERR_PARAM;
         les_gpu* c = b->c;
-        DHIPC(hipEventRecord(c->ev[2], c->stream));
-        int e = pool_steps(c, n_steps);
-        if (e != LES_OK) return e;
-        DHIPC(hipEventRecord(c->ev[3], c->stream));
-        DHIPC(hipEventSynchronize(c->ev[3]));
-        float ms = 0.f;
-        DHIPC(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-        c->step_ms = n_steps > 0 ? ms / n_steps : 0.0;
+        DuctState* d = c->d;
+        // each run's probe ring holds probe_cap steps: drain every resident
+        // run's ring into its row list at least that often (the step time
+        // excludes the drains)
+        const int chunk = d->probe_cap > 0 ? d->probe_cap : (n_steps > 0 ? n_steps : 1);
+        double ms_sum = 0.0;
+        for (int done = 0; done < n_steps;) {
+            const int n = n_steps - done < chunk ? n_steps - done : chunk;
+            DHIPC(hipEventRecord(c->ev[2], c->stream));
+            int e = pool_steps(c, n);
+            if (e != LES_OK) return e;
+            DHIPC(hipEventRecord(c->ev[3], c->stream));
+            DHIPC(hipEventSynchronize(c->ev[3]));
+            float ms = 0.f;
+            DHIPC(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+            ms_sum += ms;
+            done += n;
+            if (d->probe_cap > 0 && done < n_steps)
+                for (int s = 0; s < d->max_slots; ++s)
+                    if (d->desc[s].active) {
+                        e = slot_probe_drain(c, s, d->probe_x[s].data(), d->probe_rows[s]);
+                        if (e != LES_OK) return e;
+                    }
+        }
+        c->step_ms = n_steps > 0 ? ms_sum / n_steps : 0.0;
         if (ms_per_step) *ms_per_step = c->step_ms;
         return LES_OK;
     });

@@ -767,6 +767,25 @@ extern "C" int les_gpu_set_params(les_gpu* c, const les_params* p)
         if (p->nx != c->p.nx || p->ny != c->p.ny || p->nz != c->p.nz || p->mode != c->p.mode ||
             p->num_vortices != c->p.num_vortices || p->precision != c->p.precision)
             return set_err(c, LES_ERR_PARAM, "grid, mode, precision and eddy count are fixed per context");
+        // quantities the context derives once at creation: cell sizes, the
+        // RNG seed, the attribute map and inlet composition (mode R), the
+        // property tables (duct)
+        if (p->lx != c->p.lx || p->ly != c->p.ly || p->lz != c->p.lz || p->seed != c->p.seed)
+            return set_err(c, LES_ERR_PARAM, "domain size and seed are fixed per context");
+        if (p->mode == LES_MODE_REFERENCE &&
+            (p->inlet_radius != c->p.inlet_radius || p->equivalence_ratio != c->p.equivalence_ratio))
+            return set_err(c, LES_ERR_PARAM, "inlet radius and equivalence ratio are fixed per context");
+        if (p->mode == LES_MODE_DUCT &&
+            (p->table_t0 != c->p.table_t0 || p->table_dt != c->p.table_dt || p->table_n != c->p.table_n))
+            return set_err(c, LES_ERR_PARAM, "the cp/mu tables are fixed per context");
+        if (p->integrator != c->p.integrator || p->advection != c->p.advection) {
+            // a duct run may switch between its two reconstructions; nothing else
+            // changes the scheme of a live context
+            const bool duct_ok = p->mode == LES_MODE_DUCT && p->integrator == LES_INT_SSPRK3 &&
+                                 (p->advection == LES_ADV_WENO5 || p->advection == LES_ADV_MUSCL_MINMOD);
+            if (!duct_ok) return set_err(c, LES_ERR_PARAM, "integrator %d / advection %d not available to this context",
+                                         p->integrator, p->advection);
+        }
         if (les_params_validate(p) != LES_OK) return LES_ERR_PARAM;
         c->p = *p;
         if (c->p.mode == LES_MODE_DUCT) duct_params_changed(c);

@@ -74,6 +74,7 @@ def lib():
         L.les_batch_download_field.argtypes = [vp, C.c_int, C.c_char_p, P(C.c_double)]
         L.les_batch_report.argtypes = [vp, C.c_int, P(abi.StepReport)]
         L.les_batch_point_fetch.argtypes = [vp, C.c_int, P(C.c_double), C.c_int, P(C.c_int)]
+        L.les_batch_probe_fetch.argtypes = [vp, C.c_int, P(abi.ProbeRow), C.c_int, P(C.c_int)]
         L.les_batch_xface_copy.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                            P(C.c_double)]
         _lib = L
@@ -337,6 +338,13 @@ class Batch:
         self._check(lib().les_batch_xface_copy(self.h, src, src_plane, dst, dst_plane, planes, reps, C.byref(ms)),
                     "les_batch_xface_copy")
         return ms.value
+
+    def probe_rows(self, slot, max_rows=1 << 16):
+        """Flame-region probe rows of one run recorded since the last fetch."""
+        rows = (abi.ProbeRow * max_rows)()
+        n = C.c_int(0)
+        self._check(lib().les_batch_probe_fetch(self.h, slot, rows, max_rows, C.byref(n)), "les_batch_probe_fetch")
+        return [rows[i] for i in range(n.value)]
 
     def point_series(self, slot, max_steps=256):
         out = np.zeros(max_steps * abi.LES_NPOINT_MAX * 4)

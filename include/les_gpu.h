@@ -328,7 +328,11 @@ typedef struct les_batch les_batch;   /* opaque pool context */
 
 int les_batch_config_default(les_batch_config* cfg);
 /* Device bytes a pool of this configuration allocates (admission budgeting:
- * size pool_cells from the free HBM before creating the pool). */
+ * size pool_cells from the free HBM before creating the pool).  With
+ * autotune = 1, creation also holds one scratch pool of at most the same
+ * arena size per candidate while it times it; a candidate whose scratch pool
+ * does not fit is skipped, and with none timed the configured tile width and
+ * mapping stay. */
 int les_batch_pool_bytes(const les_batch_config* cfg, size_t* bytes);
 /* probe (may be NULL) is the run shape the autotuner times. */
 int  les_batch_create(const les_batch_config* cfg, const les_params* probe, les_batch** out);
@@ -346,7 +350,10 @@ int les_batch_tuning(const les_batch* b, int* tile_k, int* grid_blocks, int* xcd
 int les_batch_admit(les_batch* b, const les_params* p, int n_steps, int init, int* slot);
 int les_batch_release(les_batch* b, int slot);
 /* Advance every resident, unfinished run by n_steps steps; *ms_per_step
- * (may be NULL) is the device time per step (HIP events). */
+ * (may be NULL) is the device time per step (HIP events).  With probe_cap > 0
+ * the steps run in chunks of probe_cap and each run's probe ring is drained
+ * into its row list between chunks, so no probe row is lost however many
+ * steps one call runs. */
 int les_batch_step(les_batch* b, int n_steps, double* ms_per_step);
 int les_batch_slot_info(les_batch* b, int slot, les_slot_info* info);
 /* Slots whose runs have finished (done and still resident). */

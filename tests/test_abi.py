@@ -72,3 +72,12 @@ def test_create_without_gpu_fails_loudly(lesgpu):
         pytest.skip("a GPU is present")
     with pytest.raises(lesgpu.LesError):
         lesgpu.LesGpu()
+
+
+def test_pool_bytes_counts_probe_rings(lesgpu):
+    """les_batch_pool_bytes includes the flame-region probe rings: 8 bytes x
+    slots x probe_cap x 16 stations x (2 + 6 ny_max) values."""
+    base = dict(max_slots=8, pool_cells=8 * 64 * 48 * 48, max_nx=64, max_ny=48, max_nz=48, precision=1)
+    b0 = lesgpu.pool_bytes(lesgpu.batch_config(probe_cap=0, **base))
+    b1 = lesgpu.pool_bytes(lesgpu.batch_config(probe_cap=64, **base))
+    assert b1 - b0 == 8 * 8 * 64 * 16 * (2 + 6 * 48)

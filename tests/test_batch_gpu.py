@@ -207,3 +207,45 @@ def test_xcd_mapping_does_not_change_results(lesgpu, precision):
     for f in range(5):
         assert np.array_equal(out[0][0][f], out[1][0][f]), f
     assert np.array_equal(out[0][1], out[1][1])
+
+
+def wrinkled_front_state(p, amp=3e-3):
+    """Duct state with the planar front wrinkled along y (the probe lines at
+    k = nz/2 cross G = 0.5 at every station)."""
+    nx, ny, nz = p.nx, p.ny, p.nz
+    dx = p.lx / nx
+    x = np.arange(nx)[:, None, None] * dx
+    y = np.arange(ny)[None, :, None] * (p.ly / ny)
+    phi = (x - p.flame_x0 - amp * np.sin(2 * np.pi * y / p.ly)) * np.ones((nx, ny, nz))
+    c = 0.5 * (1 + np.tanh(phi / (2 * dx)))
+    T = p.inlet_temperature + c * (p.burnt_temperature - p.inlet_temperature)
+    rho = p.rho_unburnt * p.inlet_temperature / T
+    return [rho * p.inlet_velocity, np.zeros_like(T), np.zeros_like(T), phi, T]
+
+
+def test_probe_rows_survive_one_long_step_call(lesgpu):
+    """A pool whose probe rings hold 4 steps, advanced 10 steps in one
+    les_batch_step call, returns the same rows as one advanced a step at a
+    time and fetched after each step: every step's rows, none lost."""
+    print("case: batch probe rings (probe_cap 4) over 10 steps in one call", flush=True)
+    p = run_params(lesgpu, 0, reinit_interval=0)
+    st = wrinkled_front_state(p)
+    got = {}
+    for mode in ("one_call", "per_step"):
+        with pool(lesgpu, 0, slots=2, probe_cap=4) as b:
+            b.admit(run_params(lesgpu, 0, nx=48, ny=40, nz=24, seed=3), n_steps=0)   # a second resident run
+            s = b.admit(p, n_steps=0)
+            b.upload(s, st)
+            rows = []
+            if mode == "one_call":
+                b.step(10)
+                rows = b.probe_rows(s)
+            else:
+                for _ in range(10):
+                    b.step(1)
+                    rows += b.probe_rows(s)
+            got[mode] = [(r.timestep, r.station, r.y_flame, r.y_physical, r.u, r.v, r.w, r.G, r.nu_t, r.S_mag)
+                         for r in rows]
+    steps = sorted({r[0] for r in got["per_step"]})
+    assert len(steps) == 10, steps
+    assert got["one_call"] == got["per_step"]

@@ -340,3 +340,25 @@ def test_fp32_reynolds_profiles_at_fixed_dt(lesgpu):
     e = serving.profile_errors(prof[1], prof[0], p.inlet_velocity, p.turbulence_intensity)
     print("max scaled profile error %.2e" % e["max_scaled"], flush=True)
     assert e["max_scaled"] < 1e-4, e
+
+
+def test_set_params_keeps_creation_quantities(lesgpu):
+    """les_gpu_set_params takes the optimiser's constants between steps but
+    refuses what a context fixes at creation (domain size, seed, tables,
+    inflow geometry and composition in the reference scheme) and schemes the
+    context cannot run; a duct run may switch between WENO5 and minmod."""
+    print("case: set_params validation (duct and reference scheme)", flush=True)
+    with lesgpu.LesGpu(lesgpu.params(1)) as g:
+        g.init_state()
+        ok = lesgpu.params(1, cs_unburned=0.15, cs_burnt=0.15, flame_accel_strength=1.2, advection=2)
+        g.set_params(ok)
+        g.step(1)
+        for bad in (dict(lx=0.07), dict(seed=5), dict(table_n=1001), dict(advection=0), dict(integrator=0)):
+            with pytest.raises(lesgpu.LesError):
+                g.set_params(lesgpu.params(1, **bad))
+    with lesgpu.LesGpu(lesgpu.params(0)) as g:
+        g.init_state()
+        g.set_params(lesgpu.params(0, flame_accel_strength=1.5, cs_unburned=0.12))
+        for bad in (dict(inlet_radius=0.01), dict(equivalence_ratio=1.0), dict(lz=0.1), dict(advection=1)):
+            with pytest.raises(lesgpu.LesError):
+                g.set_params(lesgpu.params(0, **bad))
