@@ -9,9 +9,12 @@ synthetic initial state, with the state resident in HBM.
 Workloads (one JSON line each, rank 0):
   duct128-fp64 (default)  128^3 duct, FP64 plain path: cell-updates/s per RK
                           stage = nx*ny*nz * stages * K / (GPU time of K steps);
-                          also carries the FP32 form of the same run and the
-                          64-member ensemble (runs/hour, ratio to the plain
-                          FP64 path at the same grid, profile error).
+                          also carries the FP32 form of the same run, the
+                          64-member ensemble (runs/hour over whole 1000-step
+                          runs, ratio to the plain FP64 path at the same grid)
+                          and the served stream (arrivals compressed 4x:
+                          runs/hour, p50/p99), each with the profile error of
+                          every run against its FP64 twin.
   duct128-fp32, duct128-muscl-fp64, ref128: single-run variants.
   ensemble64-fp32         64 x 64x48x48 FP32 members in one pool: LES runs/hour.
   served96-fp32           96 requests, Poisson arrivals (mean gap 2 s), uneven
@@ -265,44 +268,89 @@ def slab_halo(lesgpu, p, planes=4, reps=50):
             "xgmi_ms_per_face_estimate": face_bytes / (XGMI_LINK_GBS * 1e9) * 1e3}
 
 
-def ensemble_leg(lesgpu, serving, steps, warmup, members=64, grid=(64, 48, 48), profile_steps=1000, extra=True):
-    """64 members in one FP32 pool vs the plain FP64 path one run at a time
-    (extra=False: the pool alone, without the FP64 leg and the profile errors)."""
-    ms = serving.ensemble_members(members, grid, seed=0)
-    cells = grid[0] * grid[1] * grid[2]
-    cfg = serving.pool_config(members, members * cells, grid, autotune=True)
-    log("ensemble: %d members %s, autotuning" % (members, grid))
-    with lesgpu.Batch(cfg, ms[0]) as b:
+def pool_profiles(lesgpu, serving, params, grid_max, precision, run_steps, autotune=False):
+    """Run every run of `params` to its end in one pool of `precision`
+    (run_steps: one step count for all, or one per run) and return per run the
+    FP64 plane profiles turned into means and Reynolds stresses, the pool's
+    device time per pool step, the wall time and its tuning."""
+    nsteps = list(run_steps) if hasattr(run_steps, "__len__") else [run_steps] * len(params)
+    cells = [p.nx * p.ny * p.nz for p in params]
+    cfg = serving.pool_config(len(params), sum(cells), grid_max, precision=precision, autotune=autotune)
+    out = [None] * len(params)
+    with lesgpu.Batch(cfg, params[0]) as b:
         tune = b.tuning()
-        slots = [b.admit(p, 0, init=True) for p in ms]
-        assert all(s is not None for s in slots)
-        b.step(warmup)
+        slots = []
+        for p, n_steps in zip(params, nsteps):
+            q = type(p).from_buffer_copy(p)
+            q.precision = precision
+            s = b.admit(q, n_steps, init=True)
+            assert s is not None
+            slots.append(s)
         t0 = time.perf_counter()
-        step_ms = b.step(steps)
+        ms_sum, n = 0.0, 0
+        while any(o is None for o in out):
+            ms = b.step(10)
+            ms_sum += ms * 10
+            n += 10
+            for s in b.poll():
+                r = slots.index(s)
+                p, st = params[r], b.info(s).step
+                out[r] = serving.reynolds_profiles(b.field(s, "profiles", 10 * p.nx), p.nx, p.ny, p.nz, st)
+                b.release(s)
         wall = time.perf_counter() - t0
-    runs_per_hour = members * 3600.0 / (1000 * step_ms * 1e-3)
+    return out, ms_sum / max(n, 1), wall, tune
+
+
+def profile_error_over_runs(serving, params, prof32, prof64):
+    """Per-run max scaled profile error of the FP32 form against its FP64
+    twin, and the worst run's per-column errors."""
+    errs = [serving.profile_errors(a, b, p.inlet_velocity, p.turbulence_intensity)
+            for p, a, b in zip(params, prof32, prof64)]
+    worst = int(np.argmax([e["max_scaled"] for e in errs]))
+    vals = np.array([e["max_scaled"] for e in errs])
+    return {"runs": len(errs), "max_scaled_max": float(vals.max()), "max_scaled_median": float(np.median(vals)),
+            "worst_run": worst, "worst_run_columns": errs[worst]["scaled"],
+            "runs_above_1e-4": int((vals > 1e-4).sum())}
+
+
+def ensemble_leg(lesgpu, serving, steps, warmup, members=64, grid=(64, 48, 48), run_steps=1000, extra=True):
+    """64 members in one FP32 pool, timed over their whole 1000-step runs,
+    vs the plain FP64 path one run at a time at the same grid.  Every run
+    takes the shared time step FIXED_DT (so the FP32 form and its FP64 twin
+    advance through the same time levels); the profile error is measured on
+    every member of the timed pool against its FP64 twin (an FP64 pool of the
+    same members, which equals the FP64 oracle within 1e-10).
+    extra=False: the pool alone."""
+    ms = serving.ensemble_members(members, grid, seed=0)
+    for p in ms:
+        p.fixed_dt = FIXED_DT
+    cells = grid[0] * grid[1] * grid[2]
+    log("ensemble: %d members %s, %d steps each, autotuning" % (members, grid, run_steps))
+    prof32, step_ms, wall, tune = pool_profiles(lesgpu, serving, ms, grid, 1, run_steps, autotune=True)
+    runs_per_hour = members * 3600.0 / (run_steps * step_ms * 1e-3)
     cu = members * cells * 3 / (step_ms * 1e-3)
+    head = {"metric": "LES runs/hour (64 members x 1000 steps, 64x48x48, FP32, one pool)",
+            "value": runs_per_hour, "ms_per_pool_step": step_ms, "wall_s": wall, "timed_steps": run_steps,
+            "dt": "fixed %.1e s, shared by both precisions" % FIXED_DT,
+            "cell_updates_per_s_stage": cu, "tuning": tune}
     if not extra:
-        return {"metric": "LES runs/hour (64 members x 1000 steps, 64x48x48, FP32, one pool)",
-                "value": runs_per_hour, "ms_per_pool_step": step_ms, "wall_s": wall,
-                "cell_updates_per_s_stage": cu, "tuning": tune}
-    # plain FP64 path, one run at a time, same grid
-    p64 = serving.ensemble_members(1, grid, seed=0, precision=0)[0]
+        return head
+    log("ensemble: FP64 twins of the 64 members")
+    prof64, _, _, _ = pool_profiles(lesgpu, serving, ms, grid, 0, run_steps)
     log("ensemble: plain FP64 single run")
+    p64 = type(ms[0]).from_buffer_copy(ms[0])
+    p64.precision = 0
     r64 = single_run(lesgpu, p64, steps, warmup, timing=False)
-    plain_rph = 3600.0 / (1000 * r64["step_ms"] * 1e-3)
-    err = profile_error(lesgpu, serving, grid, profile_steps)
-    err_fixed = profile_error(lesgpu, serving, grid, profile_steps, fixed_dt=FIXED_DT)
-    return {"metric": "LES runs/hour (64 members x 1000 steps, 64x48x48, FP32, one pool)",
-            "value": runs_per_hour, "ms_per_pool_step": step_ms, "wall_s": wall,
-            "cell_updates_per_s_stage": cu, "tuning": tune,
-            "plain_fp64_runs_per_hour": plain_rph, "plain_fp64_ms_per_step": r64["step_ms"],
-            "ratio_vs_plain_fp64": runs_per_hour / plain_rph,
-            "reynolds_profile_rel_err_fp32_vs_fp64": err,
-            "reynolds_profile_rel_err_fp32_vs_fp64_fixed_dt": err_fixed}
+    plain_rph = 3600.0 / (run_steps * r64["step_ms"] * 1e-3)
+    head.update({"plain_fp64_runs_per_hour": plain_rph, "plain_fp64_ms_per_step": r64["step_ms"],
+                 "ratio_vs_plain_fp64": runs_per_hour / plain_rph,
+                 "reynolds_profile_rel_err_fp32_vs_fp64": profile_error_over_runs(serving, ms, prof32, prof64),
+                 "reynolds_profile_rel_err_cfl_rule_member0": profile_error(lesgpu, serving, grid, run_steps)})
+    return head
 
 
-FIXED_DT = 1.0e-5      # below the CFL step of every 64x48x48 member (1.2e-5 ... 2.6e-5 s)
+FIXED_DT = 1.0e-5      # shared time step of the timed ensemble and served runs: below the CFL step
+                       # of every 64x48x48 member (1.2e-5 ... 2.6e-5 s)
 
 
 def profile_error(lesgpu, serving, grid, steps, fixed_dt=0.0):
@@ -379,58 +427,64 @@ def ensemble_line(a, lesgpu, serving):
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": e["ms_per_pool_step"],
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic", "config": {"workload": WORKLOADS["ensemble64-fp32"]["desc"],
-                                           "members": 64, "grid": [64, 48, 48], "steps_per_run": 1000}}
+                                           "members": 64, "grid": [64, 48, 48], "steps_per_run": 1000,
+                                           "dt": e["dt"]}}
     out.update(e)
     return out
 
 
-def served_line(a, lesgpu, serving):
-    reqs = serving.served_requests(a.requests, 2.0, seed=0)
+SERVED_TIME_SCALE = 0.25   # the default line's served stream: arrivals compressed 4x (mean gap 0.5 s)
+
+
+def served_leg(lesgpu, serving, n_requests, time_scale, extra=True):
+    """The served stream: requests with Poisson arrivals (mean gap 2 s x
+    time_scale), grids drawn from the four served grids, 500-2000 steps each,
+    admitted by HBM footprint into one FP32 pool; every run takes the shared
+    time step FIXED_DT.  p50/p99 latency (arrival -> result) and the profile
+    error of every request against its FP64 twin (all twins in one FP64 pool,
+    which equals the FP64 oracle within 1e-10)."""
+    reqs = serving.served_requests(n_requests, 2.0, seed=0)
+    for _, p, _ in reqs:
+        p.fixed_dt = FIXED_DT
     budget = 8 << 30          # HBM budget of the pool (admission by footprint)
     cfg = serving.pool_for_budget(budget, 64, (128, 64, 64))
     probe = reqs[0][1]
-    log("served: %d requests, pool of %d cells (%.1f GiB budget)" % (len(reqs), cfg.pool_cells, budget / 2 ** 30))
+    log("served: %d requests, arrivals x%.2f, pool of %d cells (%.1f GiB budget)"
+        % (len(reqs), time_scale, cfg.pool_cells, budget / 2 ** 30))
     with lesgpu.Batch(cfg, probe) as b:
         srv = serving.Server(b, chunk=10)
-        wall = serving.run_stream(srv, reqs, a.time_scale, progress=log)
+        wall = serving.run_stream(srv, reqs, time_scale, progress=log)
         lat = np.array([srv.results[r]["latency_s"] for r in range(len(reqs))])
         bad = sum(srv.results[r]["nonfinite"] for r in range(len(reqs)))
         tune = b.tuning()
     cell_steps = sum(p.nx * p.ny * p.nz * n for _, p, n in reqs)
-    err = served_profile_error(lesgpu, serving, probe, 1000)
-    err["fixed_dt"] = served_profile_error(lesgpu, serving, probe, 1000, fixed_dt=FIXED_DT)
-    return {"metric": "LES runs/hour + p50/p99 latency (96 served requests, FP32)",
-            "value": len(reqs) * 3600.0 / wall, "unit": "runs/hour", "n_gpus": 1,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-            "data": "synthetic",
-            "config": {"workload": WORKLOADS["served96-fp32"]["desc"], "requests": len(reqs),
-                       "grids": serving.SERVED_GRIDS, "steps": "U[500,2000]", "time_scale": a.time_scale,
-                       "pool_cells": cfg.pool_cells, "hbm_budget_bytes": budget},
-            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-            "mean_latency_s": float(lat.mean()), "wall_s": wall, "gpu_busy_s": srv.gpu_ms * 1e-3,
-            "pool_steps": srv.steps_run, "cell_steps": cell_steps, "nonfinite_runs": int(bad), "tuning": tune,
-            "reynolds_profile_rel_err_fp32_vs_fp64": err}
+    out = {"metric": "LES runs/hour + p50/p99 latency (%d served requests, FP32)" % len(reqs),
+           "value": len(reqs) * 3600.0 / wall, "unit": "runs/hour",
+           "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+           "mean_latency_s": float(lat.mean()), "wall_s": wall, "gpu_busy_s": srv.gpu_ms * 1e-3,
+           "pool_steps": srv.steps_run, "cell_steps": cell_steps, "nonfinite_runs": int(bad), "tuning": tune,
+           "config": {"requests": len(reqs), "grids": serving.SERVED_GRIDS, "steps": "U[500,2000]",
+                      "mean_gap_s": 2.0 * time_scale, "time_scale": time_scale,
+                      "dt": "fixed %.1e s, shared by both precisions" % FIXED_DT,
+                      "pool_cells": cfg.pool_cells, "hbm_budget_bytes": budget}}
+    if extra:
+        log("served: FP64 twins of the %d requests" % len(reqs))
+        params = [p for _, p, _ in reqs]
+        prof64, _, _, _ = pool_profiles(lesgpu, serving, params, (128, 64, 64), 0, [n for _, _, n in reqs])
+        prof32 = [serving.reynolds_profiles(srv.results[r]["profiles"], p.nx, p.ny, p.nz, srv.results[r]["steps"])
+                  for r, p in enumerate(params)]
+        out["reynolds_profile_rel_err_fp32_vs_fp64"] = profile_error_over_runs(serving, params, prof32, prof64)
+    return out
 
 
-def served_profile_error(lesgpu, serving, p, steps, fixed_dt=0.0):
-    """The first request's run (its grid and constants) in the FP32 form vs the
-    plain FP64 path (which equals the FP64 oracle within 1e-10) after `steps`
-    steps: plane-profile errors per column (fixed_dt > 0: both runs take it)."""
-    profs = {}
-    for prec in (1, 0):
-        q = type(p).from_buffer_copy(p)
-        q.precision = prec
-        q.fixed_dt = fixed_dt
-        g = lesgpu.LesGpu(q, lesgpu.mixture())
-        try:
-            g.init_state()
-            g.step(steps)
-            profs[prec] = serving.reynolds_profiles(g.field("profiles", 10 * q.nx), q.nx, q.ny, q.nz, steps)
-        finally:
-            g.close()
-    out = serving.profile_errors(profs[1], profs[0], p.inlet_velocity, p.turbulence_intensity)
-    out["steps"] = steps
-    out["grid"] = [p.nx, p.ny, p.nz]
+def served_line(a, lesgpu, serving):
+    e = served_leg(lesgpu, serving, a.requests, a.time_scale, extra=not a.no_extra)
+    out = {"metric": e.pop("metric"), "value": e.pop("value"), "unit": e.pop("unit"), "n_gpus": 1,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic"}
+    cfg = e.pop("config")
+    cfg["workload"] = WORKLOADS["served96-fp32"]["desc"]
+    out["config"] = cfg
+    out.update(e)
     return out
 
 
@@ -476,6 +530,7 @@ def main():
         out["fp32_form"] = {"value": f32["value"], "ms_per_step": f32["ms_per_step"], "roofline": f32["roofline"],
                             "error_vs_fp64": fp32_field_error(lesgpu, serving, 2, 50)}
         out["ensemble"] = ensemble_leg(lesgpu, serving, a.steps, a.warmup)
+        out["served"] = served_leg(lesgpu, serving, a.requests, SERVED_TIME_SCALE)
     if not a.no_cpu:
         log("cpu baseline")
         dt, cores, n, total = cpu_baseline(wl["preset"], wl["nx"], wl["ny"], wl["nz"], wl["over"], wl["stages"],
