@@ -112,6 +112,23 @@ static void attribute_map(les_gpu* c, std::vector<int>& a, int kind = 0)
             }
 }
 
+// Plane indices of the inlet cells (attribute 1 on the inflow plane) in
+// plane order: the cells the inflow normalisation sums over.
+static int upload_inlet_list(les_gpu* c)
+{
+    const size_t plane = (size_t)c->p.ny * c->p.nz;
+    std::vector<int> idx;
+    for (size_t pl = 0; pl < plane; ++pl)
+        if (c->attr_h[pl] == 1) idx.push_back((int)pl);
+    c->n_inlet = (int)idx.size();
+    if (!idx.empty() &&
+        hipMemcpy(c->r.inlet_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        snprintf(c->err, sizeof(c->err), "inlet list upload failed");
+        return LES_ERR_HIP;
+    }
+    return LES_OK;
+}
+
 static int rng_draw(les_gpu* c)
 {
     int32_t r = 0;
@@ -245,6 +262,10 @@ static int create_ref(les_gpu* c)
     HIPC(hipMalloc(&at, n * sizeof(int)));
     c->allocs.push_back(at);
     r.attr = (int*)at;
+    void* il = nullptr;
+    HIPC(hipMalloc(&il, plane * sizeof(int)));
+    c->allocs.push_back(il);
+    r.inlet_idx = (int*)il;
     for (int a = 0; a < 10; ++a) if (!(r.stat[a] = dalloc(c, n))) return set_err(c, LES_ERR_HIP, "device allocation failed");
     {
         void* pi = nullptr;
@@ -270,7 +291,7 @@ static int create_ref(les_gpu* c)
     c->noise_h.assign(3 * plane, 0.0);
     attribute_map(c, c->attr_h);
     HIPC(hipMemcpy(r.attr, c->attr_h.data(), n * sizeof(int), hipMemcpyHostToDevice));
-    return LES_OK;
+    return upload_inlet_list(c);
 }
 
 extern "C" int les_gpu_create(const les_params* p, const les_mixture* mix, les_gpu** out)
@@ -441,9 +462,9 @@ static int ref_step(les_gpu* c, les_step_report* rep, bool last)
     size_t plane = (size_t)c->p.ny * c->p.nz;
     HIPC(hipMemcpyAsync(r.vort, c->vort.data(), c->vort.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPC(hipMemcpyAsync(r.noise, c->noise_h.data(), 3 * plane * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_ref_sem_plane, dim3((g.nz + 63) / 64, g.ny), dim3(64), 0, c->stream, g, r.attr, r.vort,
+    hipLaunchKernelGGL(k_ref_sem_plane, dim3((unsigned)((plane + 3) / 4)), dim3(256), 0, c->stream, g, r.attr, r.vort,
                        c->p.num_vortices, r.inlet);
-    hipLaunchKernelGGL(k_ref_sem_normalise, dim3(1), dim3(SEM_NORM_THREADS), 0, c->stream, g, r.attr, r.inlet,
+    hipLaunchKernelGGL(k_ref_sem_normalise, dim3(1), dim3(SEM_NORM_THREADS), 0, c->stream, g, r.inlet_idx, c->n_inlet, r.inlet,
                        c->p.inlet_velocity * c->p.turbulence_intensity);
     LAUNCH_CHECK();
 
@@ -549,7 +570,7 @@ static int set_attribute_map(les_gpu* c, int kind)
     attribute_map(c, c->attr_h, kind);
     HIPC(hipMemcpy(c->r.attr, c->attr_h.data(), c->attr_h.size() * sizeof(int), hipMemcpyHostToDevice));
     c->attr_map = kind;
-    return LES_OK;
+    return upload_inlet_list(c);
 }
 
 extern "C" int les_gpu_init_state(les_gpu* c)

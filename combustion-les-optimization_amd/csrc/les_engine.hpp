@@ -30,6 +30,7 @@ struct R {
     double *nu, *mu_mix, *cp, *lam, *alpha, *nu_t, *S_mag, *ST, *gradG;
     double* Dk[LES_NSP];
     double *inlet, *noise, *vort, *partial, *rho_avg;
+    int* inlet_idx;          // plane indices of the inlet cells (attribute 1), plane order
     unsigned long long* maxbits;
     double* probe_rows;
     int* probe_flags;
@@ -55,6 +56,7 @@ struct les_gpu {
     std::vector<double> vort;      // num_vortices x 7
     std::vector<double> noise_h;   // 3 per inlet-plane cell
     std::vector<int> attr_h;
+    int n_inlet = 0;               // inlet cells of the reference scheme's plane
     int attr_map = 0;              // 0: setting()'s map (:1152-1181); 1: main()'s, kept by a binary resume (:4123-4136)
     std::vector<les_probe_row> probe;
     int stat_count = 0;

@@ -823,100 +823,114 @@ __global__ void __launch_bounds__(256) k_ref_dt_max(GridDims g, const int* attr,
 // ---------------------------------------------------------------------------
 // SEM inflow plane (:964-1003) and its normalisation (:1005-1074)
 // ---------------------------------------------------------------------------
+// One wave per inlet-plane cell, 64 eddies at a time, one per lane.  The
+// reference sums every eddy's strength x influence into the cell in eddy
+// order (:965-1000); an eddy whose distance is not below its radius adds
+// +0.0 or -0.0, which leaves the sum (never -0.0: it starts at +0.0 and a
+// rounded sum of nonzero terms is never -0.0) unchanged, so only the eddies
+// within their radius are added, in eddy order (lowest lane of each chunk
+// first): bit for bit the reference's sums with one distance test per lane.
+// Blocks of 256 threads: four cells per block.
 __global__ void __launch_bounds__(256) k_ref_sem_plane(GridDims g, const int* attr, const double* vort, int nvort,
                                                        double* inlet)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y;
-    if (k >= g.nz) return;
-    long long pl = (long long)j * g.nz + k;
-    long long P = (long long)g.ny * g.nz;
+    const int lane = (int)(threadIdx.x & 63);
+    const long long P = (long long)g.ny * g.nz;
+    const long long pl = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pl >= P) return;
+    const int j = (int)(pl / g.nz), k = (int)(pl - (long long)j * g.nz);
     double su = 0.0, sv = 0.0, sw = 0.0;
     if (attr[pl] == 1) {
-        double yp = j * g.dy, zp = k * g.dz;
-        for (int e = 0; e < nvort; ++e) {
-            const double* V = vort + 7 * e;
-            double ddx = 0.0 - V[0], ddy = yp - V[1], ddz = zp - V[2];
-            double dist = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-            double inf = 0.0;
-            if (dist < V[3]) inf = 1.0 - (dist / V[3]);
-            su += V[4] * inf;
-            sv += V[5] * inf;
-            sw += V[6] * inf;
+        const double yp = j * g.dy, zp = k * g.dz;
+        for (int e0 = 0; e0 < nvort; e0 += 64) {
+            const int e = e0 + lane;
+            double cu = 0.0, cv = 0.0, cw = 0.0;
+            bool hit = false;
+            if (e < nvort) {
+                const double* V = vort + 7 * e;
+                double ddx = 0.0 - V[0], ddy = yp - V[1], ddz = zp - V[2];
+                double dist = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                if (dist < V[3]) {
+                    double inf = 1.0 - (dist / V[3]);
+                    cu = V[4] * inf; cv = V[5] * inf; cw = V[6] * inf;
+                    hit = true;
+                }
+            }
+            unsigned long long m = __ballot(hit);
+            while (m) {
+                const int l = __ffsll((long long)m) - 1;
+                m &= m - 1;
+                su += __shfl(cu, l);
+                sv += __shfl(cv, l);
+                sw += __shfl(cw, l);
+            }
         }
     }
-    inlet[pl] = su; inlet[P + pl] = sv; inlet[2 * P + pl] = sw;
+    if (lane == 0) { inlet[pl] = su; inlet[P + pl] = sv; inlet[2 * P + pl] = sw; }
 }
 
-// Serial (j, k)-order sums, as in the reference, by one thread: the plane has
-// ny*nz cells (800 on the reference grid), so order fidelity is cheap here.
-// The reference's serial sums over the inlet cells, kept bit for bit (each
-// component's sum runs in plane-index order, as :1017-1062 do): the block
-// stages the plane in LDS chunks with all its threads and threads 0..2 each
-// run one component's sum over the chunk; the mean subtraction and the rms
+// The reference's serial sums over the inlet cells (:1005-1062), bit for bit:
+// each component's sum runs in plane order over the inlet cells only (idx:
+// their plane indices in plane order, n of them; the other cells add nothing
+// and are skipped), the block staging the values in LDS chunks and threads
+// 0..2 each running one component's sum; the mean subtraction and the rms
 // scaling, elementwise, are spread over the block.  One block of
 // SEM_NORM_THREADS threads.
 #define SEM_NORM_THREADS 256
 #define SEM_NORM_CHUNK 2048
-__global__ void __launch_bounds__(SEM_NORM_THREADS) k_ref_sem_normalise(GridDims g, const int* attr, double* inlet,
-                                                                       double target)
+__global__ void __launch_bounds__(SEM_NORM_THREADS) k_ref_sem_normalise(GridDims g, const int* idx, int n,
+                                                                       double* inlet, double target)
 {
     __shared__ double sv[3][SEM_NORM_CHUNK];
-    __shared__ int sa[SEM_NORM_CHUNK];
     __shared__ double tot[3];
-    __shared__ int nin_s;
     const long long P = (long long)g.ny * g.nz;
     double* const C[3] = {inlet, inlet + P, inlet + 2 * P};
     const int t = (int)threadIdx.x;
+    if (n <= 0) return;
     for (int pass = 0; pass < 2; ++pass) {
-        // pass 0: sums of u', v', w' and the inlet-cell count; pass 1: sums of squares
+        // pass 0: sums of u', v', w'; pass 1: sums of squares of the centred values
         double acc = 0.0;
-        int nin = 0;
-        for (long long c0 = 0; c0 < P; c0 += SEM_NORM_CHUNK) {
-            const int n = (int)(P - c0 < SEM_NORM_CHUNK ? P - c0 : SEM_NORM_CHUNK);
-            for (int e = t; e < n; e += SEM_NORM_THREADS) {
-                sa[e] = attr[c0 + e];
-                sv[0][e] = C[0][c0 + e];
-                sv[1][e] = C[1][c0 + e];
-                sv[2][e] = C[2][c0 + e];
+        for (int c0 = 0; c0 < n; c0 += SEM_NORM_CHUNK) {
+            const int m = n - c0 < SEM_NORM_CHUNK ? n - c0 : SEM_NORM_CHUNK;
+            for (int e = t; e < m; e += SEM_NORM_THREADS) {
+                const int pl = idx[c0 + e];
+                sv[0][e] = C[0][pl];
+                sv[1][e] = C[1][pl];
+                sv[2][e] = C[2][pl];
             }
             __syncthreads();
             if (t < 3) {
                 const double* x = sv[t];
                 if (pass == 0) {
-                    for (int e = 0; e < n; ++e)
-                        if (sa[e] == 1) { acc += x[e]; nin++; }
+#pragma unroll 8
+                    for (int e = 0; e < m; ++e) acc += x[e];
                 } else {
-                    for (int e = 0; e < n; ++e)
-                        if (sa[e] == 1) acc += x[e] * x[e];
+#pragma unroll 8
+                    for (int e = 0; e < m; ++e) acc += x[e] * x[e];
                 }
             }
             __syncthreads();
         }
         if (t < 3) tot[t] = acc;
-        if (t == 0 && pass == 0) nin_s = nin;
         __syncthreads();
-        const int nn = nin_s;
-        if (nn > 0) {
-            double f[3];
-            if (pass == 0) {
+        double f[3];
+        if (pass == 0) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) f[c] = tot[c] / nn;                 // means
-            } else {
+            for (int c = 0; c < 3; ++c) f[c] = tot[c] / n;                      // means
+        } else {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const double r = sqrt(tot[c] / nn);
-                    f[c] = (r > LES_SMALL) ? target / r : 0.0;              // rms scale factors
-                }
+            for (int c = 0; c < 3; ++c) {
+                const double r = sqrt(tot[c] / n);
+                f[c] = (r > LES_SMALL) ? target / r : 0.0;                   // rms scale factors
             }
-            for (long long pl = t; pl < P; pl += SEM_NORM_THREADS)
-                if (attr[pl] == 1) {
+        }
+        for (int e = t; e < n; e += SEM_NORM_THREADS) {
+            const int pl = idx[e];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        if (pass == 0) C[c][pl] -= f[c];
-                        else C[c][pl] *= f[c];
-                    }
-                }
+            for (int c = 0; c < 3; ++c) {
+                if (pass == 0) C[c][pl] -= f[c];
+                else C[c][pl] *= f[c];
+            }
         }
         __syncthreads();
     }
