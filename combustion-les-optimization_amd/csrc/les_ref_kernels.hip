@@ -9,52 +9,8 @@
 #include "les_ref_kernels.hpp"
 
 // ---------------------------------------------------------------------------
-// property evaluators (:385-447)
+// box filter stencil (:1307-1470)
 // ---------------------------------------------------------------------------
-__device__ static double prop_fit(double t, const double* co)               // :385-391
-{
-    if (t <= 0) return 0.0;
-    if (co[0] == 0.0 && co[1] == 0.0 && co[2] == 0.0 && co[3] == 0.0) return 0.0;
-    double lt = log(t);
-    double lp = co[0] + co[1] * lt + co[2] * pow(lt, 2.0) + co[3] * pow(lt, 3.0);
-    return exp(lp);
-}
-
-__device__ static double mu_sp(const les_mixture* m, int s, double t) { return prop_fit(t, m->sp[s].eta_fit) * 0.1; }
-__device__ static double lam_sp(const les_mixture* m, int s, double t) { return prop_fit(t, m->sp[s].lambda_fit) * 1e-5; }
-__device__ static double dkj(const les_mixture* m, int a, int b, double t) { return prop_fit(t, m->diff_fit[a][b]) * 1e-4; }
-
-__device__ static double cp_sp(const les_mixture* m, int s, double t)         // :401-428
-{
-    const les_species* sp = &m->sp[s];
-    double tt = t;
-    if (tt <= 0.0) tt = 1.0;
-    const double* a = (tt > sp->t_mid) ? sp->nasa_high : sp->nasa_low;
-    double cpr = a[0] + a[1] * tt + a[2] * pow(tt, 2.0) + a[3] * pow(tt, 3.0) + a[4] * pow(tt, 4.0);
-    double cpm = cpr * LES_R_UNIV;
-    double mwk = sp->mw / 1000.0;
-    if (mwk < 1e-9) return 1000.0;
-    return cpm / mwk;
-}
-
-__device__ static double wilke(const les_mixture* m, const double* X, const double* mu_k, const double* val_k)
-{   // :1530-1546, :1842-1863
-    double acc = 0.0;
-    for (int a = 0; a < LES_NSP; ++a) {
-        if (X[a] < 1e-12) continue;
-        double mu_i = mu_k[a], M_i = m->sp[a].mw, den = 0.0;
-        for (int b = 0; b < LES_NSP; ++b) {
-            if (X[b] < 1e-12) continue;
-            double mu_j = mu_k[b], M_j = m->sp[b].mw;
-            double phi = pow(1.0 + pow(mu_i / mu_j, 0.5) * pow(M_j / M_i, 0.25), 2.0) /
-                         (sqrt(8.0) * sqrt(1.0 + M_i / M_j));
-            den += X[b] * phi;
-        }
-        if (den > 1e-9) acc += (X[a] * val_k[a]) / den;
-    }
-    return acc;
-}
-
 __device__ __forceinline__ double box7(const double* f, long long q, long long sx, long long sy, double wc)
 {   // :1329-1331 neighbour order
     return (f[q - sx] + f[q - sy] + f[q - 1] + wc * f[q] + f[q + 1] + f[q + sy] + f[q + sx]) / (6.0 + wc);
@@ -62,9 +18,51 @@ __device__ __forceinline__ double box7(const double* f, long long q, long long s
 
 // ---------------------------------------------------------------------------
 // K1: box filter (:1307-1470) + mixture properties (:1489-1881), fused per cell
+//
+// The reference evaluates every property fit from scratch: log T, (ln T)^2 and
+// (ln T)^3 in each of the 30 fits (5 viscosities, 5 conductivities, 20
+// binary diffusivities, :385-391), T^2..T^4 in each of the 5 NASA
+// polynomials (:401-416), and the 25 Wilke factors phi_ab with their row sums
+// twice, once for the viscosity and once for the conductivity mixing
+// (:1530-1546, :1842-1863), whose inputs (X, mu_k, M) are the same.  Each of
+// these is computed once per cell here with the same operations, so every
+// value is bit for bit what the repeated evaluation gives; the molar-mass
+// factors of phi_ab, constants of the mixture, once per block.
 // ---------------------------------------------------------------------------
+__device__ static double prop_fit_lt(double t, double lt, double lt2, double lt3, const double* co)
+{   // prop_fit with ln t, (ln t)^2, (ln t)^3 given
+    if (t <= 0) return 0.0;
+    if (co[0] == 0.0 && co[1] == 0.0 && co[2] == 0.0 && co[3] == 0.0) return 0.0;
+    double lp = co[0] + co[1] * lt + co[2] * lt2 + co[3] * lt3;
+    return exp(lp);
+}
+
+__device__ static double cp_sp_pow(const les_mixture* m, int s, double tt, double t2, double t3, double t4)
+{   // cp_sp with tt (t clamped as cp_sp does) and its powers given
+    const les_species* sp = &m->sp[s];
+    const double* a = (tt > sp->t_mid) ? sp->nasa_high : sp->nasa_low;
+    double cpr = a[0] + a[1] * tt + a[2] * t2 + a[3] * t3 + a[4] * t4;
+    double cpm = cpr * LES_R_UNIV;
+    double mwk = sp->mw / 1000.0;
+    if (mwk < 1e-9) return 1000.0;
+    return cpm / mwk;
+}
+
 __global__ void __launch_bounds__(256) k_ref_filter_props(GridDims g, RefConsts rc, RefPtrs p)
 {
+    // per-pair factors of phi_ab that depend on the molar masses only
+    __shared__ double s_m4[LES_NSP][LES_NSP], s_den[LES_NSP][LES_NSP];
+    const les_mixture* m = p.mix;
+    {
+        const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+        if (tid < LES_NSP * LES_NSP) {
+            const int a = tid / LES_NSP, b = tid % LES_NSP;
+            const double M_i = m->sp[a].mw, M_j = m->sp[b].mw;
+            s_m4[a][b] = pow(M_j / M_i, 0.25);
+            s_den[a][b] = sqrt(8.0) * sqrt(1.0 + M_i / M_j);
+        }
+        __syncthreads();
+    }
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y * blockDim.y + threadIdx.y;
     int i = blockIdx.z;
@@ -91,7 +89,6 @@ __global__ void __launch_bounds__(256) k_ref_filter_props(GridDims g, RefConsts 
     p.Tf[q] = Tf; p.rhof[q] = rf;
     for (int s = 0; s < LES_NSP; ++s) p.Yf[s][q] = Yf[s];
 
-    const les_mixture* m = p.mix;
     if (a != 3) {
         p.nu[q] = 1.5e-5; p.mu_mix[q] = 1.8e-5;
         for (int s = 0; s < LES_NSP; ++s) p.Dk[s][q] = 0.0;
@@ -108,15 +105,43 @@ __global__ void __launch_bounds__(256) k_ref_filter_props(GridDims g, RefConsts 
             for (int s = 0; s < LES_NSP; ++s) X[s] = (Yf[s] / (m->sp[s].mw / 1000.0)) / sm;
         }
     }
-    double mu_k[LES_NSP];
+    // ln T and its powers, shared by the 30 property fits
+    double lt = 0.0, lt2 = 0.0, lt3 = 0.0;
+    if (Tf > 0) {
+        lt = log(Tf);
+        lt2 = pow(lt, 2.0);
+        lt3 = pow(lt, 3.0);
+    }
+    double mu_k[LES_NSP], lk[LES_NSP];
     if (have_x)
-        for (int s = 0; s < LES_NSP; ++s) mu_k[s] = mu_sp(m, s, Tf);
+        for (int s = 0; s < LES_NSP; ++s) {
+            mu_k[s] = prop_fit_lt(Tf, lt, lt2, lt3, m->sp[s].eta_fit) * 0.1;
+            lk[s] = prop_fit_lt(Tf, lt, lt2, lt3, m->sp[s].lambda_fit) * 1e-5;
+        }
+    // Wilke mixing of mu and lambda (:1530-1546, :1842-1863): one set of
+    // phi_ab and row sums, both accumulations in the reference's order
+    double mu_acc = 0.0, lam_acc = 0.0;
+    if (have_x) {
+        for (int aa = 0; aa < LES_NSP; ++aa) {
+            if (X[aa] < 1e-12) continue;
+            double mu_i = mu_k[aa], den = 0.0;
+            for (int bb = 0; bb < LES_NSP; ++bb) {
+                if (X[bb] < 1e-12) continue;
+                double mu_j = mu_k[bb];
+                double phi = pow(1.0 + pow(mu_i / mu_j, 0.5) * s_m4[aa][bb], 2.0) / s_den[aa][bb];
+                den += X[bb] * phi;
+            }
+            if (den > 1e-9) {
+                mu_acc += (X[aa] * mu_k[aa]) / den;
+                lam_acc += (X[aa] * lk[aa]) / den;
+            }
+        }
+    }
     // update_nu_mix (:1501-1557)
     if (!have_x) {
         p.nu[q] = 1.5e-5; p.mu_mix[q] = 1.8e-5;
     } else {
-        double mu = wilke(m, X, mu_k, mu_k);
-        double mm = fmax(1e-7, mu);
+        double mm = fmax(1e-7, mu_acc);
         p.mu_mix[q] = mm;
         double nu = (rf > 1e-3) ? mm / rf : 1.5e-5;
         p.nu[q] = fmax(1e-7, nu);
@@ -130,12 +155,12 @@ __global__ void __launch_bounds__(256) k_ref_filter_props(GridDims g, RefConsts 
             for (int b = 0; b < LES_NSP; ++b) {
                 if (s == b) continue;
                 if (X[b] > 1e-12) {
-                    double d = dkj(m, s, b, Tf);
+                    double d = prop_fit_lt(Tf, lt, lt2, lt3, m->diff_fit[s][b]) * 1e-4;
                     if (d > 1e-12) sum += X[b] / d;
                 }
             }
             double D;
-            if (fabs(1.0 - X[s]) < 1e-9) D = dkj(m, s, (s != 2) ? 2 : 1, Tf);
+            if (fabs(1.0 - X[s]) < 1e-9) D = prop_fit_lt(Tf, lt, lt2, lt3, m->diff_fit[s][(s != 2) ? 2 : 1]) * 1e-4;
             else if (sum > 1e-12) D = (1.0 - X[s]) / sum;
             else D = 1e-5;
             double ptot = LES_P_CONST + 0.0;        // P_filtered is identically zero
@@ -147,8 +172,9 @@ __global__ void __launch_bounds__(256) k_ref_filter_props(GridDims g, RefConsts 
     double cpm;
     if (Tf <= 0) cpm = 1000.0;
     else {
+        const double t2 = pow(Tf, 2.0), t3 = pow(Tf, 3.0), t4 = pow(Tf, 4.0);
         double cp = 0.0;
-        for (int s = 0; s < LES_NSP; ++s) cp += Yf[s] * cp_sp(m, s, Tf);
+        for (int s = 0; s < LES_NSP; ++s) cp += Yf[s] * cp_sp_pow(m, s, Tf, t2, t3, t4);
         cpm = fmax(100.0, cp);
     }
     p.cp[q] = cpm;
@@ -156,9 +182,7 @@ __global__ void __launch_bounds__(256) k_ref_filter_props(GridDims g, RefConsts 
     if (!have_x) {
         p.lam[q] = 0.025; p.alpha[q] = 1.8e-5;
     } else {
-        double lk[LES_NSP];
-        for (int s = 0; s < LES_NSP; ++s) lk[s] = lam_sp(m, s, Tf);
-        double lam = fmax(1e-3, wilke(m, X, mu_k, lk));
+        double lam = fmax(1e-3, lam_acc);
         p.lam[q] = lam;
         double al = (rf > 1e-6 && cpm > 1e-6) ? lam / (rf * cpm) : 1.8e-5;
         p.alpha[q] = fmax(1e-7, al);
