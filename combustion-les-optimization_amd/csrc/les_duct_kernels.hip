@@ -593,6 +593,51 @@ __global__ void __launch_bounds__(1024) k_duct_rho_final(Pool P)
 // ---------------------------------------------------------------------------
 // K2: properties, nu_t (:1569-1633), S_T (:1913-1965); per-tile DT maxima
 // ---------------------------------------------------------------------------
+// Properties (tables), nu_t (:1569-1633) and the SSR flame speed
+// (:1913-1965) of one interior cell from its filtered T and phi and the
+// central differences of the filtered velocities; mn/ma receive the cell's
+// nu and alpha for the diffusive DT limit.
+template <typename real>
+__device__ __forceinline__ void sgs_cell(const Pool& P, const DuctConsts& k, const GridDims& g, real T, real pfq,
+                                         real du_dx, real dv_dy, real dw_dz, real du_dy, real du_dz, real dv_dx,
+                                         real dv_dz, real dw_dx, real dw_dy, real& nut_out, real& st_out, double& mn,
+                                         double& ma)
+{
+    const real SL = (real)k.s_l;
+    const real DX = (real)g.dx;
+    real rf = M<real>::div((real)k.rho_u_Tu, T);
+    real mu = table_lookup<real>(P.mu_tab, k, T);
+    real cp = table_lookup<real>(P.cp_tab, k, T);
+    real nu = M<real>::div(mu, rf);
+    real lam = mu * cp * rcp_const<real>(k, DUCT_RC_IPRAN);
+    real al = M<real>::div(lam, rf * cp);
+    mn = (double)nu; ma = (double)al;
+    real S12 = (real)0.5 * (du_dy + dv_dx), S13 = (real)0.5 * (du_dz + dw_dx), S23 = (real)0.5 * (dv_dz + dw_dy);
+    real ss = (real)2.0 * (S12 * S12 + S13 * S13 + S23 * S23) + du_dx * du_dx + dv_dy * dv_dy + dw_dz * dw_dz;
+    real smag = sqrt((real)2.0 * ss);
+    real div = du_dx + dv_dy + dw_dz;
+    real gg = progress<real>(pfq, rcp_const<real>(k, DUCT_RC_HID));
+    real cs = (real)k.cs_u + ((real)k.cs_b - (real)k.cs_u) * gg;
+    cs += (real)k.cs_f * (real)4.0 * gg * ((real)1.0 - gg);
+    nut_out = (cs * DX) * (cs * DX) * smag;
+    // SSR-SGS flame speed
+    real num = ss - (div * div) * ((real)1.0 / (real)3.0);
+    real X = M<real>::div((real)k.dx4, nu * nu) * num;
+    real d_eta;
+    if (X < (real)LES_SMALL) d_eta = (real)1.0;
+    else {
+        real px = M<real>::pow34(X);
+        real alg = (real)0.2187 * px;
+        d_eta = (real)0.2340 * px * (alg > (real)1.0 ? M<real>::log(alg) : (real)0.0);
+    }
+    real eta = M<real>::div(DX, d_eta + (real)LES_SMALL);
+    const real dF = rcp_const<real>(k, DUCT_RC_DF);
+    real asf = (real)8.0 * M<real>::exp(M<real>::div((real)10.0 * dF, (real)8.0 * eta + (real)LES_SMALL));
+    real wr = M<real>::pow(asf * eta * rcp_const<real>(k, DUCT_RC_ID), (real)2.0 - (real)k.d3);
+    real te = DX * fmax((real)0.0, div) * rcp_const<real>(k, DUCT_RC_ITSL);
+    st_out = fmax(SL, SL * fmax((real)0.0, wr + te));
+}
+
 template <typename real, int TK>
 __global__ void __launch_bounds__(256) k_duct_sgs(Pool P, int stage, int xcd_map)
 {
@@ -619,51 +664,17 @@ __global__ void __launch_bounds__(256) k_duct_sgs(Pool P, int stage, int xcd_map
             if (i == 0 || i == g.nx - 1) {
                 nut[q] = (real)0.0; stv[q] = SL;
             } else {
-                const real DX = (real)g.dx;
-                real T = Tf[q], rf = M<real>::div((real)k.rho_u_Tu, T);
-                real mu = table_lookup<real>(P.mu_tab, k, T);
-                real cp = table_lookup<real>(P.cp_tab, k, T);
-                real nu = M<real>::div(mu, rf);
-                real lam = mu * cp * rcp_const<real>(k, DUCT_RC_IPRAN);
-                real al = M<real>::div(lam, rf * cp);
-                mn = (double)nu; ma = (double)al;
                 const int xp = q + (int)g.sx, xm = q - (int)g.sx;
                 const int yp = (int)lidx(g, i, wrapi(j + 1, g.ny), kk), ym = (int)lidx(g, i, wrapi(j - 1, g.ny), kk);
                 const int zp = (int)lidx(g, i, j, wrapi(kk + 1, g.nz)), zm = (int)lidx(g, i, j, wrapi(kk - 1, g.nz));
                 const real hx = rcp_const<real>(D.k, DUCT_RC_HID), hy = rcp_const<real>(D.k, DUCT_RC_HID + 1), hz = rcp_const<real>(D.k, DUCT_RC_HID + 2);
-                real du_dx = (uf[xp] - uf[xm]) * hx;
-                real dv_dy = (vf[yp] - vf[ym]) * hy;
-                real dw_dz = (wf[zp] - wf[zm]) * hz;
-                real du_dy = (uf[yp] - uf[ym]) * hy;
-                real du_dz = (uf[zp] - uf[zm]) * hz;
-                real dv_dx = (vf[xp] - vf[xm]) * hx;
-                real dv_dz = (vf[zp] - vf[zm]) * hz;
-                real dw_dx = (wf[xp] - wf[xm]) * hx;
-                real dw_dy = (wf[yp] - wf[ym]) * hy;
-                real S12 = (real)0.5 * (du_dy + dv_dx), S13 = (real)0.5 * (du_dz + dw_dx), S23 = (real)0.5 * (dv_dz + dw_dy);
-                real ss = (real)2.0 * (S12 * S12 + S13 * S13 + S23 * S23) + du_dx * du_dx + dv_dy * dv_dy + dw_dz * dw_dz;
-                real smag = sqrt((real)2.0 * ss);
-                real div = du_dx + dv_dy + dw_dz;
-                real gg = progress<real>(pf[q], rcp_const<real>(D.k, DUCT_RC_HID));
-                real cs = (real)k.cs_u + ((real)k.cs_b - (real)k.cs_u) * gg;
-                cs += (real)k.cs_f * (real)4.0 * gg * ((real)1.0 - gg);
-                nut[q] = (cs * DX) * (cs * DX) * smag;
-                // SSR-SGS flame speed
-                real num = ss - (div * div) * ((real)1.0 / (real)3.0);
-                real X = M<real>::div((real)k.dx4, nu * nu) * num;
-                real d_eta;
-                if (X < (real)LES_SMALL) d_eta = (real)1.0;
-                else {
-                    real px = M<real>::pow34(X);
-                    real alg = (real)0.2187 * px;
-                    d_eta = (real)0.2340 * px * (alg > (real)1.0 ? M<real>::log(alg) : (real)0.0);
-                }
-                real eta = M<real>::div(DX, d_eta + (real)LES_SMALL);
-                const real dF = rcp_const<real>(k, DUCT_RC_DF);
-                real asf = (real)8.0 * M<real>::exp(M<real>::div((real)10.0 * dF, (real)8.0 * eta + (real)LES_SMALL));
-                real wr = M<real>::pow(asf * eta * rcp_const<real>(k, DUCT_RC_ID), (real)2.0 - (real)k.d3);
-                real te = DX * fmax((real)0.0, div) * rcp_const<real>(k, DUCT_RC_ITSL);
-                stv[q] = fmax(SL, SL * fmax((real)0.0, wr + te));
+                real nt, stq;
+                sgs_cell<real>(P, k, g, Tf[q], pf[q],
+                               (uf[xp] - uf[xm]) * hx, (vf[yp] - vf[ym]) * hy, (wf[zp] - wf[zm]) * hz,
+                               (uf[yp] - uf[ym]) * hy, (uf[zp] - uf[zm]) * hz, (vf[xp] - vf[xm]) * hx,
+                               (vf[zp] - vf[zm]) * hz, (wf[xp] - wf[xm]) * hx, (wf[yp] - wf[ym]) * hy, nt, stq, mn, ma);
+                nut[q] = nt;
+                stv[q] = stq;
             }
         }
         block_max2_256(mn, ma, red);
