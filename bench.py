@@ -15,7 +15,7 @@ Workloads (one JSON line each, rank 0):
                           and the served stream (arrivals compressed 4x:
                           runs/hour, p50/p99), each with the profile error of
                           every run against its FP64 twin.
-  duct128-fp32, duct128-muscl-fp64, ref128: single-run variants.
+  duct128-fp32, duct128-muscl-fp64, ref60, ref128: single-run variants.
   ensemble64-fp32         64 x 64x48x48 FP32 members in one pool: LES runs/hour.
   served96-fp32           96 requests, Poisson arrivals (mean gap 2 s), uneven
                           grids, admitted by HBM footprint: runs/hour, p50/p99.
@@ -43,6 +43,9 @@ PEAK_HBM_GBS = 8000.0
 # and writes q1 (10 field passes), stages 2 and 3 read q_s and q^n and write
 # (15 each): 13.33 passes x 8 B = 106.7 B (FP64), x 4 B = 53.3 B (FP32).
 WORKLOADS = {
+    "ref60": dict(preset=0, nx=60, ny=20, nz=40, stages=4, bytes_per_cell_stage=340.0, over={},
+                  dtype="f64", desc="reference scheme at the reference's own 60x20x40 grid (RK4, TENO5/MUSCL-superbee, "
+                                    "5-species mixing), FP64"),
     "ref128": dict(preset=0, nx=128, ny=128, nz=128, stages=4, bytes_per_cell_stage=340.0, over={},
                    dtype="f64",
                    desc="reference scheme (RK4, TENO5/MUSCL-superbee, 5-species mixing), FP64, 128^3"),
@@ -91,13 +94,13 @@ def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
-# MI355X VALU issue: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles
-PEAK_VALU_GWIS = 256 * 4 * 2.4 / 4
-
-
 def valu_from_profiles(workload):
-    """VALU wave-instructions per stage of the RHS pair (rocprofv3 --pmc
-    SQ_INSTS_VALU over the same build), where recorded."""
+    """VALU issue of the RHS pair per stage from a rocprofv3 --pmc run of the
+    same build, where recorded: VALU wave-instructions (SQ_INSTS_VALU), the
+    share of SIMD quad-cycles that issued at least one VALU instruction
+    ((SQ_ACTIVE_INST_VALU - SQ_ACTIVE_INST_VALU2) over SQ_BUSY_CYCLES / 4 x 32
+    SIMDs per shader engine), the dual-issue share, the shader clock and the
+    waves' wait fraction."""
     path = os.path.join(ROOT, "profiles", "rhs_valu.json")
     if not os.path.exists(path):
         return None
@@ -219,7 +222,8 @@ def single_line(a, lesgpu, name):
     k_ms = r["rhs_ms"] if r["rhs_ms"] > 0 else stage_ms
     achieved = cells * B / (k_ms * 1e-3) / 1e9
     out = {
-        "metric": METRIC,
+        "metric": METRIC if (wl["nx"], wl["ny"], wl["nz"]) == (128, 128, 128)
+        else "cell-updates/s per RK stage at %dx%dx%d" % (wl["nx"], wl["ny"], wl["nz"]),
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -242,11 +246,17 @@ def single_line(a, lesgpu, name):
     }
     vi = valu_from_profiles(name)
     if vi:
-        # the pair's VALU instruction count (measured) over its live duration:
-        # the bound these kernels actually run against
-        g = vi / (k_ms * 1e-3) / 1e9
-        out["roofline"]["valu_issue"] = {"achieved": g, "peak": PEAK_VALU_GWIS, "unit": "G wave-instr/s",
-                                         "frac": g / PEAK_VALU_GWIS, "insts_per_stage": vi}
+        # the pair's VALU issue from its counters: busy share of the SIMDs'
+        # issue cycles (at most 1 by construction), and the measured VALU
+        # instruction count over the live duration
+        out["roofline"]["valu_issue"] = {
+            "frac": vi["busy_frac"], "frac_of": "SIMD quad-cycles issuing at least one VALU instruction",
+            "achieved": vi["insts_per_stage"] / (k_ms * 1e-3) / 1e9, "unit": "G VALU wave-instr/s",
+            "insts_per_stage": vi["insts_per_stage"], "insts_per_simd_quad": vi["insts_per_simd_quad"],
+            "dual_issue_share": vi["dual_issue_share"], "clock_ghz": vi["clock_ghz"],
+            "wait_any_frac": vi["wait_any_frac"], "l2_hit": vi["l2_hit"],
+            "source": "profiles/rhs_valu.json (rocprofv3 --pmc SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_VALU2 "
+                      "SQ_BUSY_CYCLES SQ_INSTS_VALU SQ_WAIT_ANY SQ_WAVE_CYCLES; TCC_HIT_sum TCC_MISS_sum)"}
     return out, p
 
 
