@@ -401,7 +401,28 @@ static int ref_rhs(les_gpu* c, bool temp_state, int stage, int t_loop)
     RefConsts rc = make_ref_consts(c);
     RefPtrs p = ref_ptrs(c, temp_state, stage);
     Launch3 L = launch3(g.nx, g.ny, g.nz);
-    hipLaunchKernelGGL(k_ref_filter_props, L.grid, L.block, 0, c->stream, g, rc, p);
+    // a grid of fewer than two line waves per SIMD (1024 SIMDs) runs the
+    // property and RHS launches split into output groups: more waves, each
+    // with a shorter dependent instruction stream (same values either way);
+    // the heaviest group first, as blocks are dispatched in slab order
+    const long long line_waves = (long long)L.grid.x * g.ny * g.nx;
+    const bool split = line_waves < 2048 && (long long)g.nx * REF_GROUPS_MAX <= 65535;
+    RefGroups gp{}, gr{};
+    gp.n = gr.n = 1;
+    if (split) {
+        // transport; fields and cp; each diffusivity
+        const unsigned pm[] = {REF_PROPS_TRANSPORT, REF_PROPS_FIELDS, 1u, 2u, 4u, 8u, 16u};
+        // momentum; G; T and Y0; Y1 and Y2; Y3 and Y4
+        const unsigned rm[] = {7u, 16u, 8u | 32u, 64u | 128u, 256u | 512u};
+        gp.n = (int)(sizeof pm / sizeof pm[0]);
+        gr.n = (int)(sizeof rm / sizeof rm[0]);
+        for (int b = 0; b < gp.n; ++b) gp.mask[b] = pm[b];
+        for (int b = 0; b < gr.n; ++b) gr.mask[b] = rm[b];
+    }
+    dim3 gridp = L.grid, gridr = L.grid;
+    gridp.z = (unsigned)(g.nx * gp.n);
+    gridr.z = (unsigned)(g.nx * gr.n);
+    hipLaunchKernelGGL(k_ref_filter_props, gridp, L.block, 0, c->stream, g, rc, p, gp);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_ref_nut_st, L.grid, L.block, 0, c->stream, g, rc, p);
     LAUNCH_CHECK();
@@ -411,7 +432,7 @@ static int ref_rhs(les_gpu* c, bool temp_state, int stage, int t_loop)
     hipLaunchKernelGGL(k_ref_rho_final, dim3(1), dim3(256), 0, c->stream, nb, c->r.partial, c->r.rho_avg);
     LAUNCH_CHECK();
     if (c->timing) HIPC(hipEventRecord(c->ev[0], c->stream));
-    hipLaunchKernelGGL(k_ref_rhs, L.grid, L.block, 0, c->stream, g, rc, p, t_loop);
+    hipLaunchKernelGGL(k_ref_rhs, gridr, L.block, 0, c->stream, g, rc, p, t_loop, gr);
     LAUNCH_CHECK();
     if (c->timing) {
         HIPC(hipEventRecord(c->ev[1], c->stream));

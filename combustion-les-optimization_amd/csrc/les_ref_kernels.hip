@@ -48,7 +48,35 @@ __device__ static double cp_sp_pow(const les_mixture* m, int s, double tt, doubl
     return cpm / mwk;
 }
 
-__global__ void __launch_bounds__(256) k_ref_filter_props(GridDims g, RefConsts rc, RefPtrs p)
+// update_Dk (:1695-1749) for species s: mixture-averaged diffusivity from the
+// binary fits, with X, T and ln T powers given
+__device__ static double dk_species(const les_mixture* m, int s, const double* X, double Tf, double lt,
+                                    double lt2, double lt3)
+{
+    double sum = 0.0;
+    for (int b = 0; b < LES_NSP; ++b) {
+        if (s == b) continue;
+        if (X[b] > 1e-12) {
+            double d = prop_fit_lt(Tf, lt, lt2, lt3, m->diff_fit[s][b]) * 1e-4;
+            if (d > 1e-12) sum += X[b] / d;
+        }
+    }
+    double D;
+    if (fabs(1.0 - X[s]) < 1e-9) D = prop_fit_lt(Tf, lt, lt2, lt3, m->diff_fit[s][(s != 2) ? 2 : 1]) * 1e-4;
+    else if (sum > 1e-12) D = (1.0 - X[s]) / sum;
+    else D = 1e-5;
+    double ptot = LES_P_CONST + 0.0;        // P_filtered is identically zero
+    if (ptot > 1.0) D *= (LES_PATM / ptot);
+    return fmax(1e-9, fmin(1e-2, D));
+}
+
+// gs.n == 1: one thread per cell writes every filtered field and property.
+// gs.n > 1: gridDim.z = nx * gs.n; slab b writes what gs.mask[b] names: bit s
+// (s < LES_NSP) the diffusivity of species s, REF_PROPS_FIELDS the filtered
+// fields and cp, REF_PROPS_TRANSPORT nu, mu, lambda and alpha.  A slab filters
+// what it needs again and derives X, ln T and cp with the same operations, so
+// every value has the same bits as in the one-slab form.
+__global__ void __launch_bounds__(256) k_ref_filter_props(GridDims g, RefConsts rc, RefPtrs p, RefGroups gs)
 {
     // per-pair factors of phi_ab that depend on the molar masses only
     __shared__ double s_m4[LES_NSP][LES_NSP], s_den[LES_NSP][LES_NSP];
@@ -65,34 +93,81 @@ __global__ void __launch_bounds__(256) k_ref_filter_props(GridDims g, RefConsts 
     }
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
+    const int b = gs.n > 1 ? (int)blockIdx.z / g.nx : 0;
+    int i = (int)blockIdx.z - b * g.nx;
+    const unsigned want = gs.n > 1 ? gs.mask[b] : REF_PROPS_ALL;
     if (k >= g.nz || j >= g.ny) return;
     long long q = lidx(g, i, j, k);
     int a = p.attr[q];
     bool interior = (i > 0 && i < g.nx - 1 && j > 0 && j < g.ny - 1 && k > 0 && k < g.nz - 1);
     double Yf[LES_NSP];
-    double Tf, rf;
+    double Tf, rf = 0.0;
+    const bool w_fields = want & REF_PROPS_FIELDS, w_trans = want & REF_PROPS_TRANSPORT;
+    const unsigned w_dk = want & ((1u << LES_NSP) - 1u);
+    if (!w_fields && !w_trans) {           // diffusivities only: Y and T filtered again
+        if (a != 3) {
+            for (int sd = 0; sd < LES_NSP; ++sd)
+                if (want & (1u << sd)) p.Dk[sd][q] = 0.0;
+            return;
+        }
+        const double wc = rc.filter_wc;
+        if (interior) {
+            Tf = box7(p.T, q, g.sx, g.sy, wc);
+            for (int s = 0; s < LES_NSP; ++s) Yf[s] = box7(p.Y[s], q, g.sx, g.sy, wc);
+        } else {
+            Tf = p.T[q];
+            for (int s = 0; s < LES_NSP; ++s) Yf[s] = p.Y[s][q];
+        }
+        double X[LES_NSP];
+        bool have_x = false;
+        if (Tf > 0) {
+            double sm = 0.0;
+            for (int s = 0; s < LES_NSP; ++s) sm += Yf[s] / (m->sp[s].mw / 1000.0);
+            if (!(sm < 1e-9)) {
+                have_x = true;
+                for (int s = 0; s < LES_NSP; ++s) X[s] = (Yf[s] / (m->sp[s].mw / 1000.0)) / sm;
+            }
+        }
+        if (!have_x) {
+            for (int sd = 0; sd < LES_NSP; ++sd)
+                if (want & (1u << sd)) p.Dk[sd][q] = 1e-5;
+            return;
+        }
+        const double lt = log(Tf), lt2 = pow(lt, 2.0), lt3 = pow(lt, 3.0);
+        for (int sd = 0; sd < LES_NSP; ++sd)
+            if (want & (1u << sd)) p.Dk[sd][q] = dk_species(m, sd, X, Tf, lt, lt2, lt3);
+        return;
+    }
     if (interior && a == 3) {
         const double wc = rc.filter_wc;
-        p.uf[q] = box7(p.u, q, g.sx, g.sy, wc);
-        p.vf[q] = box7(p.v, q, g.sx, g.sy, wc);
-        p.wf[q] = box7(p.w, q, g.sx, g.sy, wc);
+        if (w_fields) {
+            p.uf[q] = box7(p.u, q, g.sx, g.sy, wc);
+            p.vf[q] = box7(p.v, q, g.sx, g.sy, wc);
+            p.wf[q] = box7(p.w, q, g.sx, g.sy, wc);
+        }
         Tf = box7(p.T, q, g.sx, g.sy, wc);
         rf = box7(p.rho, q, g.sx, g.sy, wc);
         for (int s = 0; s < LES_NSP; ++s) Yf[s] = box7(p.Y[s], q, g.sx, g.sy, wc);
-        p.Gf[q] = box7(p.G, q, g.sx, g.sy, wc);
+        if (w_fields) p.Gf[q] = box7(p.G, q, g.sx, g.sy, wc);
     } else {
-        p.uf[q] = p.u[q]; p.vf[q] = p.v[q]; p.wf[q] = p.w[q];
-        Tf = p.T[q]; rf = p.rho[q]; p.Gf[q] = p.G[q];
+        if (w_fields) {
+            p.uf[q] = p.u[q]; p.vf[q] = p.v[q]; p.wf[q] = p.w[q]; p.Gf[q] = p.G[q];
+        }
+        Tf = p.T[q]; rf = p.rho[q];
         for (int s = 0; s < LES_NSP; ++s) Yf[s] = p.Y[s][q];
     }
-    p.Tf[q] = Tf; p.rhof[q] = rf;
-    for (int s = 0; s < LES_NSP; ++s) p.Yf[s][q] = Yf[s];
+    if (w_fields) {
+        p.Tf[q] = Tf; p.rhof[q] = rf;
+        for (int s = 0; s < LES_NSP; ++s) p.Yf[s][q] = Yf[s];
+    }
 
     if (a != 3) {
-        p.nu[q] = 1.5e-5; p.mu_mix[q] = 1.8e-5;
-        for (int s = 0; s < LES_NSP; ++s) p.Dk[s][q] = 0.0;
-        p.cp[q] = 1000.0; p.lam[q] = 0.0; p.alpha[q] = 0.0;
+        if (w_trans) {
+            p.nu[q] = 1.5e-5; p.mu_mix[q] = 1.8e-5; p.lam[q] = 0.0; p.alpha[q] = 0.0;
+        }
+        for (int s = 0; s < LES_NSP; ++s)
+            if (w_dk & (1u << s)) p.Dk[s][q] = 0.0;
+        if (w_fields) p.cp[q] = 1000.0;
         return;
     }
     double X[LES_NSP];
@@ -107,13 +182,13 @@ __global__ void __launch_bounds__(256) k_ref_filter_props(GridDims g, RefConsts 
     }
     // ln T and its powers, shared by the 30 property fits
     double lt = 0.0, lt2 = 0.0, lt3 = 0.0;
-    if (Tf > 0) {
+    if (Tf > 0 && (w_trans || w_dk)) {
         lt = log(Tf);
         lt2 = pow(lt, 2.0);
         lt3 = pow(lt, 3.0);
     }
     double mu_k[LES_NSP], lk[LES_NSP];
-    if (have_x)
+    if (have_x && w_trans)
         for (int s = 0; s < LES_NSP; ++s) {
             mu_k[s] = prop_fit_lt(Tf, lt, lt2, lt3, m->sp[s].eta_fit) * 0.1;
             lk[s] = prop_fit_lt(Tf, lt, lt2, lt3, m->sp[s].lambda_fit) * 1e-5;
@@ -121,7 +196,7 @@ __global__ void __launch_bounds__(256) k_ref_filter_props(GridDims g, RefConsts 
     // Wilke mixing of mu and lambda (:1530-1546, :1842-1863): one set of
     // phi_ab and row sums, both accumulations in the reference's order
     double mu_acc = 0.0, lam_acc = 0.0;
-    if (have_x) {
+    if (have_x && w_trans) {
         for (int aa = 0; aa < LES_NSP; ++aa) {
             if (X[aa] < 1e-12) continue;
             double mu_i = mu_k[aa], den = 0.0;
@@ -138,7 +213,8 @@ __global__ void __launch_bounds__(256) k_ref_filter_props(GridDims g, RefConsts 
         }
     }
     // update_nu_mix (:1501-1557)
-    if (!have_x) {
+    if (!w_trans) {
+    } else if (!have_x) {
         p.nu[q] = 1.5e-5; p.mu_mix[q] = 1.8e-5;
     } else {
         double mm = fmax(1e-7, mu_acc);
@@ -147,27 +223,9 @@ __global__ void __launch_bounds__(256) k_ref_filter_props(GridDims g, RefConsts 
         p.nu[q] = fmax(1e-7, nu);
     }
     // update_Dk (:1695-1749)
-    if (!have_x) {
-        for (int s = 0; s < LES_NSP; ++s) p.Dk[s][q] = 1e-5;
-    } else {
-        for (int s = 0; s < LES_NSP; ++s) {
-            double sum = 0.0;
-            for (int b = 0; b < LES_NSP; ++b) {
-                if (s == b) continue;
-                if (X[b] > 1e-12) {
-                    double d = prop_fit_lt(Tf, lt, lt2, lt3, m->diff_fit[s][b]) * 1e-4;
-                    if (d > 1e-12) sum += X[b] / d;
-                }
-            }
-            double D;
-            if (fabs(1.0 - X[s]) < 1e-9) D = prop_fit_lt(Tf, lt, lt2, lt3, m->diff_fit[s][(s != 2) ? 2 : 1]) * 1e-4;
-            else if (sum > 1e-12) D = (1.0 - X[s]) / sum;
-            else D = 1e-5;
-            double ptot = LES_P_CONST + 0.0;        // P_filtered is identically zero
-            if (ptot > 1.0) D *= (LES_PATM / ptot);
-            p.Dk[s][q] = fmax(1e-9, fmin(1e-2, D));
-        }
-    }
+    for (int s = 0; s < LES_NSP; ++s)
+        if (w_dk & (1u << s)) p.Dk[s][q] = have_x ? dk_species(m, s, X, Tf, lt, lt2, lt3) : 1e-5;
+    if (!w_fields && !w_trans) return;
     // update_Cp_mix (:1772-1786)
     double cpm;
     if (Tf <= 0) cpm = 1000.0;
@@ -177,9 +235,10 @@ __global__ void __launch_bounds__(256) k_ref_filter_props(GridDims g, RefConsts 
         for (int s = 0; s < LES_NSP; ++s) cp += Yf[s] * cp_sp_pow(m, s, Tf, t2, t3, t4);
         cpm = fmax(100.0, cp);
     }
-    p.cp[q] = cpm;
+    if (w_fields) p.cp[q] = cpm;
     // update_lambda_mix_alpha (:1811-1872)
-    if (!have_x) {
+    if (!w_trans) {
+    } else if (!have_x) {
         p.lam[q] = 0.025; p.alpha[q] = 1.8e-5;
     } else {
         double lam = fmax(1e-3, lam_acc);
@@ -462,16 +521,26 @@ __device__ __forceinline__ double fluxdiv(double gp, double gm, double fp, doubl
 // ---------------------------------------------------------------------------
 // K4: RHS of momentum, G, species and T (:2552-2962)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_ref_rhs(GridDims g, RefConsts rc, RefPtrs p, int t_loop)
+// gs.n == 1: one thread computes all ten outputs of its cell.  gs.n > 1:
+// gridDim.z = nx * gs.n and slab b of nx blocks computes the outputs in
+// gs.mask[b] (bit f: drhou, drhov, drhow, dT, dG with |grad G|, dY0..dY4), so
+// a small grid runs several times the waves, each with a part of the
+// instruction stream.  Every output is the same arithmetic either way (no FMA
+// contraction in this file), so every grouping gives identical bits.
+__global__ void __launch_bounds__(256) k_ref_rhs(GridDims g, RefConsts rc, RefPtrs p, int t_loop, RefGroups gs)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
+    const int b = gs.n > 1 ? (int)blockIdx.z / g.nx : 0;
+    int i = (int)blockIdx.z - b * g.nx;
+    const unsigned want = gs.n > 1 ? gs.mask[b] : REF_RHS_ALL;
+    const bool do_T = want & (1u << 3), do_G = want & (1u << 4);
     if (k < 1 || k >= g.nz - 1 || j < 1 || j >= g.ny - 1 || i < 1 || i >= g.nx - 1) return;
     long long q = lidx(g, i, j, k);
     if (p.attr[q] != 3) {
-        for (int f = 0; f < LES_NRHS; ++f) p.out[f][q] = 0.0;
-        p.gradG[q] = 0.0;
+        for (int f = 0; f < LES_NRHS; ++f)
+            if (want & (1u << f)) p.out[f][q] = 0.0;
+        if (do_G) p.gradG[q] = 0.0;
         return;
     }
     const double DX = g.dx, DY = g.dy, DZ = g.dz;
@@ -485,86 +554,94 @@ __global__ void __launch_bounds__(256) k_ref_rhs(GridDims g, RefConsts rc, RefPt
     const double* U[3] = {uf, vf, wf};
     const double rho_avg = p.rho_avg[0];
 
-    // (a) momentum (:2556-2691)
-    double conv[3], diff[3];
-    for (int a = 0; a < 3; ++a) conv[a] = conv_conservative(U[a], q, near2, S, U, rf, d);
-    double mu_eff = p.mu_mix[q] + rf[q] * nut[q];
-    for (int a = 0; a < 3; ++a) {
-        const double* V = U[a];
-        double lx = (V[q + X] - 2.0 * V[q] + V[q - X]) / (DX * DX);
-        double ly = (V[q + Y] - 2.0 * V[q] + V[q - Y]) / (DY * DY);
-        double lz = (V[q + 1] - 2.0 * V[q] + V[q - 1]) / (DZ * DZ);
-        diff[a] = mu_eff * (lx + ly + lz);
-    }
-    double grav = (rf[q] - rho_avg) * LES_G_GRAV;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
     const double Gl = Gf[q];
     const double wdt = rc.accel_width, ctr = rc.accel_center;
-    if (Gl > 0.01 && Gl < 0.99) {
-        double gx = (Gf[q + X] - Gf[q - X]) / (2.0 * DX);
-        double gy = (Gf[q + Y] - Gf[q - Y]) / (2.0 * DY);
-        double gz = (Gf[q + 1] - Gf[q - 1]) / (2.0 * DZ);
-        if (gx < 0.0) gx = 0.0;
-        if (gy < 0.0) gy = 0.0;
-        double zrel = (k * DZ) - rc.lz_half;
-        if (zrel > 0.0) { if (gz < 0.0) gz = 0.0; }
-        else { if (gz > 0.0) gz = 0.0; }
-        double df = Gl - ctr;
-        double win = exp(-(df * df) / (2.0 * wdt * wdt));
-        double common = rc.accel_strength * rf[q] * win * 1.0;
-        fx = common * gx * rc.weight_x;
-        fy = common * gy * 1.0;
-        fz = common * gz * 1.0;
-        double zp = k * DZ, ramp = 1.0;
-        if (zp < rc.ramp_distance) ramp = zp / rc.ramp_distance;
-        fx *= ramp; fy *= ramp; fz *= ramp;
+    // (a) momentum (:2556-2691)
+    if (want & 7u) {
+        double conv[3] = {0.0, 0.0, 0.0}, diff[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 3; ++a)
+            if (want & (1u << a)) conv[a] = conv_conservative(U[a], q, near2, S, U, rf, d);
+        double mu_eff = p.mu_mix[q] + rf[q] * nut[q];
+        for (int a = 0; a < 3; ++a) {
+            if (!(want & (1u << a))) continue;
+            const double* V = U[a];
+            double lx = (V[q + X] - 2.0 * V[q] + V[q - X]) / (DX * DX);
+            double ly = (V[q + Y] - 2.0 * V[q] + V[q - Y]) / (DY * DY);
+            double lz = (V[q + 1] - 2.0 * V[q] + V[q - 1]) / (DZ * DZ);
+            diff[a] = mu_eff * (lx + ly + lz);
+        }
+        double grav = (rf[q] - rho_avg) * LES_G_GRAV;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        if (Gl > 0.01 && Gl < 0.99) {
+            double gx = (Gf[q + X] - Gf[q - X]) / (2.0 * DX);
+            double gy = (Gf[q + Y] - Gf[q - Y]) / (2.0 * DY);
+            double gz = (Gf[q + 1] - Gf[q - 1]) / (2.0 * DZ);
+            if (gx < 0.0) gx = 0.0;
+            if (gy < 0.0) gy = 0.0;
+            double zrel = (k * DZ) - rc.lz_half;
+            if (zrel > 0.0) { if (gz < 0.0) gz = 0.0; }
+            else { if (gz > 0.0) gz = 0.0; }
+            double df = Gl - ctr;
+            double win = exp(-(df * df) / (2.0 * wdt * wdt));
+            double common = rc.accel_strength * rf[q] * win * 1.0;
+            fx = common * gx * rc.weight_x;
+            fy = common * gy * 1.0;
+            fz = common * gz * 1.0;
+            double zp = k * DZ, ramp = 1.0;
+            if (zp < rc.ramp_distance) ramp = zp / rc.ramp_distance;
+            fx *= ramp; fy *= ramp; fz *= ramp;
+        }
+        const double Fm = rc.max_force;
+        if (fx > Fm) fx = Fm;
+        if (fx < -Fm) fx = -Fm;
+        if (fy > Fm) fy = Fm;
+        if (fy < -Fm) fy = -Fm;
+        if (fz > Fm) fz = Fm;
+        if (fz < -Fm) fz = -Fm;
+        if (want & 1u) p.out[0][q] = -conv[0] + diff[0] + grav + fx;
+        if (want & 2u) p.out[1][q] = -conv[1] + diff[1] + fy;
+        if (want & 4u) p.out[2][q] = -conv[2] + diff[2] + fz;
     }
-    const double Fm = rc.max_force;
-    if (fx > Fm) fx = Fm;
-    if (fx < -Fm) fx = -Fm;
-    if (fy > Fm) fy = Fm;
-    if (fy < -Fm) fy = -Fm;
-    if (fz > Fm) fz = Fm;
-    if (fz < -Fm) fz = -Fm;
-    p.out[0][q] = -conv[0] + diff[0] + grav + fx;
-    p.out[1][q] = -conv[1] + diff[1] + fy;
-    p.out[2][q] = -conv[2] + diff[2] + fz;
+    if (!(want & ~7u)) return;
 
-    // (b) G (:2710-2818)
-    double cg = conv_teno(Gf, q, near2, near3, S, vel, d);
     const double sct = rc.sc_t;
     double nu0 = nut[q];
     double nxp = nut[q + X], nxm = nut[q - X], nyp = nut[q + Y], nym = nut[q - Y], nzp = nut[q + 1], nzm = nut[q - 1];
-    double sgs = fluxdiv((nxp + nu0) / (2.0 * sct), (nu0 + nxm) / (2.0 * sct), Gf[q + X], Gf[q], Gf[q - X], DX) +
-                 fluxdiv((nyp + nu0) / (2.0 * sct), (nu0 + nym) / (2.0 * sct), Gf[q + Y], Gf[q], Gf[q - Y], DY) +
-                 fluxdiv((nzp + nu0) / (2.0 * sct), (nu0 + nzm) / (2.0 * sct), Gf[q + 1], Gf[q], Gf[q - 1], DZ);
-    double gx, gy, gz;
-    if (t_loop >= rc.transition_step && !near3) {
-        gx = weno5_grad(Gf, q, X, DX);
-        gy = weno5_grad(Gf, q, Y, DY);
-        gz = weno5_grad(Gf, q, 1, DZ);
-    } else {
-        gx = (Gf[q + X] - Gf[q - X]) / (2.0 * DX);
-        gy = (Gf[q + Y] - Gf[q - Y]) / (2.0 * DY);
-        gz = (Gf[q + 1] - Gf[q - 1]) / (2.0 * DZ);
+    // (b) G (:2710-2818)
+    if (do_G) {
+        double cg = conv_teno(Gf, q, near2, near3, S, vel, d);
+        double sgs = fluxdiv((nxp + nu0) / (2.0 * sct), (nu0 + nxm) / (2.0 * sct), Gf[q + X], Gf[q], Gf[q - X], DX) +
+                     fluxdiv((nyp + nu0) / (2.0 * sct), (nu0 + nym) / (2.0 * sct), Gf[q + Y], Gf[q], Gf[q - Y], DY) +
+                     fluxdiv((nzp + nu0) / (2.0 * sct), (nu0 + nzm) / (2.0 * sct), Gf[q + 1], Gf[q], Gf[q - 1], DZ);
+        double gx, gy, gz;
+        if (t_loop >= rc.transition_step && !near3) {
+            gx = weno5_grad(Gf, q, X, DX);
+            gy = weno5_grad(Gf, q, Y, DY);
+            gz = weno5_grad(Gf, q, 1, DZ);
+        } else {
+            gx = (Gf[q + X] - Gf[q - X]) / (2.0 * DX);
+            gy = (Gf[q + Y] - Gf[q - Y]) / (2.0 * DY);
+            gz = (Gf[q + 1] - Gf[q - 1]) / (2.0 * DZ);
+        }
+        double gmag = sqrt(gx * gx + gy * gy + gz * gz);
+        p.gradG[q] = gmag;
+        double prop = p.ST[q] * gmag;
+        double art = 0.0;
+        if (Gl > 0.1 && Gl < 0.99) {
+            double df = Gl - ctr;
+            double win = exp(-(df * df) / (2.0 * wdt * wdt));
+            double lx = (Gf[q + X] - 2.0 * Gl + Gf[q - X]) / (DX * DX);
+            double ly = (Gf[q + Y] - 2.0 * Gl + Gf[q - Y]) / (DY * DY);
+            double lz = (Gf[q + 1] - 2.0 * Gl + Gf[q - 1]) / (DZ * DZ);
+            double rp = fabs(j * DY);
+            double tr = rc.tip_radius;
+            double tip = 1.0 - exp(-(rp * rp) / (tr * tr));
+            double coef = 5.0 * (nu0 + 1.0e-5) * win * tip;
+            art = coef * (lx + ly + lz);
+        }
+        p.out[4][q] = -cg + prop + sgs + art;
     }
-    double gmag = sqrt(gx * gx + gy * gy + gz * gz);
-    p.gradG[q] = gmag;
-    double prop = p.ST[q] * gmag;
-    double art = 0.0;
-    if (Gl > 0.1 && Gl < 0.99) {
-        double df = Gl - ctr;
-        double win = exp(-(df * df) / (2.0 * wdt * wdt));
-        double lx = (Gf[q + X] - 2.0 * Gl + Gf[q - X]) / (DX * DX);
-        double ly = (Gf[q + Y] - 2.0 * Gl + Gf[q - Y]) / (DY * DY);
-        double lz = (Gf[q + 1] - 2.0 * Gl + Gf[q - 1]) / (DZ * DZ);
-        double rp = fabs(j * DY);
-        double tr = rc.tip_radius;
-        double tip = 1.0 - exp(-(rp * rp) / (tr * tr));
-        double coef = 5.0 * (nu0 + 1.0e-5) * win * tip;
-        art = coef * (lx + ly + lz);
-    }
-    p.out[4][q] = -cg + prop + sgs + art;
+    if (!(want & ~31u) && !do_T) return;
 
     // (c) species (:2842-2896)
     double gv = Gl;
@@ -574,6 +651,7 @@ __global__ void __launch_bounds__(256) k_ref_rhs(GridDims g, RefConsts rc, RefPt
     double dyp = (nyp + nu0) / (2.0 * sct), dym = (nu0 + nym) / (2.0 * sct);
     double dzp = (nzp + nu0) / (2.0 * sct), dzm = (nu0 + nzm) / (2.0 * sct);
     for (int s = 0; s < LES_NSP; ++s) {
+        if (!(want & (1u << (5 + s)))) continue;
         const double* Yf = p.Yf[s];
         double cy = conv_muscl(Yf, q, near2, S, vel, d);
         double lx = (Yf[q + X] - 2.0 * Yf[q] + Yf[q - X]) / (DX * DX);
@@ -591,6 +669,8 @@ __global__ void __launch_bounds__(256) k_ref_rhs(GridDims g, RefConsts rc, RefPt
         else tgt = rc.inletY[2];
         p.out[5 + s][q] = -cy + mol + sg + rc.relaxation * (tgt - Yf[q]);
     }
+
+    if (!do_T) return;
 
     // (d) temperature (:2916-2955)
     double ct = conv_muscl(Tf, q, near2, S, vel, d);

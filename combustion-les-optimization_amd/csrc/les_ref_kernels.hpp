@@ -33,6 +33,21 @@ struct RefStagePtrs {
     double* k[4][LES_NRHS];
 };
 
+// Output groups of a split RHS or property launch (small grids): n slabs of
+// nx blocks, slab b computing the outputs in mask[b].  RHS bits: the ten
+// outputs drhou, drhov, drhow, dT, dG, dY0..dY4.  Property bits: bit s the
+// diffusivity of species s, then the filtered fields with cp, then the
+// transport properties nu, mu, lambda, alpha.
+#define REF_GROUPS_MAX 12
+#define REF_RHS_ALL ((1u << LES_NRHS) - 1u)
+#define REF_PROPS_FIELDS (1u << LES_NSP)
+#define REF_PROPS_TRANSPORT (1u << (LES_NSP + 1))
+#define REF_PROPS_ALL ((1u << (LES_NSP + 2)) - 1u)
+struct RefGroups {
+    int n;
+    unsigned mask[REF_GROUPS_MAX];
+};
+
 struct ProbeArgs {
     int n_station;
     int i_probe[LES_NPROBE_MAX];

@@ -55,6 +55,38 @@ def test_rhs_synthetic_state(pair):
             assert e <= TOL, (t_loop, nm, e)
 
 
+@pytest.mark.parametrize("grid", [(60, 20, 40), (36, 40, 72)], ids=["grouped_launches", "one_launch"])
+def test_rhs_both_launch_forms(lesgpu, oracle, grid):
+    """The property and RHS kernels run as output groups (one slab of blocks
+    per group) on grids of fewer than 2048 line waves, 60x20x40 having 1200,
+    and as one slab otherwise, 36x40x72 having 2880; every output of both
+    forms against the oracle at 1e-10, on a state that takes the TENO, MUSCL
+    and upwind branches."""
+    nx, ny, nz = grid
+    print("case: reference-scheme rhs %dx%dx%d vs oracle" % grid, flush=True)
+    g = lesgpu.LesGpu(lesgpu.params(0, nx=nx, ny=ny, nz=nz), lesgpu.mixture())
+    o = oracle.Oracle(oracle.params(0, nx=nx, ny=ny, nz=nz), oracle.mixture())
+    try:
+        g.init_state()
+        o.init_state()
+        st = synthetic_reference_state(o, seed=5)
+        g.upload_state(st)
+        o.set_state(st)
+        names = ["drhou", "drhov", "drhow", "dT", "dG", "dY0", "dY1", "dY2", "dY3", "dY4"]
+        for t_loop in (0, 600):
+            for nm, a, b in zip(names, g.rhs(t_loop), o.rhs(t_loop)):
+                assert np.abs(b).max() > 0, nm
+                e = rel_err(a, b)
+                assert e <= TOL, (t_loop, nm, e)
+            for nm in ["nu", "alpha", "mu_mix", "cp_mix", "lambda_mix", "u_f", "w_f", "T_f", "G_f", "rho_f",
+                       "D0", "D2", "D4", "gradG"]:
+                e = rel_err(g.field(nm), o.field(nm))
+                assert e <= TOL, (t_loop, nm, e)
+    finally:
+        g.close()
+        o.close()
+
+
 @pytest.mark.parametrize("start", ["init", "synthetic"])
 def test_steps_match_oracle(pair, start):
     g, o = pair
