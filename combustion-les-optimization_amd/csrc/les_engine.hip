@@ -245,8 +245,10 @@ static int create_ref(les_gpu* c)
             if (!(r.k[st4][f] = dalloc(c, n))) return set_err(c, LES_ERR_HIP, "device allocation failed");
     size_t plane = (size_t)c->p.ny * c->p.nz;
     r.inlet = dalloc(c, 3 * plane);
-    r.noise = dalloc(c, 3 * plane);
-    r.vort = dalloc(c, 7 * (size_t)(c->p.num_vortices > 0 ? c->p.num_vortices : 1));
+    // eddies and inlet noise in one block, sent to the device in one copy a step
+    const size_t nvort = 7 * (size_t)(c->p.num_vortices > 0 ? c->p.num_vortices : 1);
+    r.vort = dalloc(c, nvort + 3 * plane);
+    r.noise = r.vort ? r.vort + nvort : nullptr;
     r.partial = dalloc(c, 2 * 1024);
     c->mon_partial = dalloc(c, 7 * 512);
     r.rho_avg = dalloc(c, 1);
@@ -286,6 +288,20 @@ static int create_ref(les_gpu* c)
         }
         HIPC(hipMemcpy(r.pt_idx, cells, np * sizeof(int), hipMemcpyHostToDevice));
         c->npts_r = np;
+    }
+    {
+        void* hp[4] = {nullptr, nullptr, nullptr, nullptr};
+        const size_t hb[4] = {4 * sizeof(unsigned long long), (nvort + 3 * plane) * sizeof(double),
+                              64 * sizeof(int), (size_t)LES_NPROBE_MAX * c->p.ny * 8 * sizeof(double)};
+        for (int a = 0; a < 4; ++a) {
+            HIPC(hipHostMalloc(&hp[a], hb[a], hipHostMallocDefault));
+            c->host_allocs.push_back(hp[a]);
+        }
+        c->h_maxbits = (unsigned long long*)hp[0];
+        c->h_vort = (double*)hp[1];
+        c->h_noise = c->h_vort + nvort;
+        c->h_probe_flags = (int*)hp[2];
+        c->h_probe_rows = (double*)hp[3];
     }
     c->vort.assign(7 * (size_t)(c->p.num_vortices > 0 ? c->p.num_vortices : 1), 0.0);
     c->noise_h.assign(3 * plane, 0.0);
@@ -350,6 +366,7 @@ extern "C" void les_gpu_destroy(les_gpu* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     duct_destroy(c);
     for (void* p : c->allocs) (void)hipFree(p);
+    for (void* p : c->host_allocs) (void)hipHostFree(p);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -454,6 +471,30 @@ __global__ void k_ref_points(const int* idx, int npts, const double* u, const do
     out[4 * p] = u[q]; out[4 * p + 1] = v[q]; out[4 * p + 2] = w[q]; out[4 * p + 3] = G[q];
 }
 
+// The probe rows of step c->probe_pending, copied into pinned memory by that
+// step and complete once the stream has been synchronised since, become rows
+// (record_flame_region_data :3403-3471, SAMPLING_RANGE :280).
+static void drain_probes(les_gpu* c)
+{
+    if (c->probe_pending < 0) return;
+    const int t = c->probe_pending;
+    const GridDims& g = c->g;
+    for (int n = 0; n < c->p.num_probe_x; ++n) {
+        if (!c->h_probe_flags[n]) continue;
+        for (int j = 0; j < g.ny; ++j) {
+            const double* q = &c->h_probe_rows[((size_t)n * g.ny + j) * 8];
+            double yp = j * g.dy, yr = yp - q[0];
+            if (fabs(yr) > 1.0) continue;
+            les_probe_row pr;
+            pr.timestep = t; pr.station = n; pr.x_location = c->p.probe_x[n];
+            pr.y_flame = q[0]; pr.y_physical = yp; pr.y_relative = yr;
+            pr.u = q[1]; pr.v = q[2]; pr.w = q[3]; pr.G = q[4]; pr.nu_t = q[5]; pr.S_mag = q[6];
+            c->probe.push_back(pr);
+        }
+    }
+    c->probe_pending = -1;
+}
+
 static int ref_step(les_gpu* c, les_step_report* rep, bool last)
 {
     const GridDims& g = c->g;
@@ -463,15 +504,15 @@ static int ref_step(les_gpu* c, les_step_report* rep, bool last)
     // DT (:4399-4411): exact device maxima, DT on the host
     HIPC(hipMemsetAsync(r.maxbits, 0, 4 * sizeof(unsigned long long), c->stream));
     int nb = (int)((g.n + 255) / 256);
-    if (nb > 1024) nb = 1024;
+    if (nb > 256) nb = 256;                 // one block per CU: 3 atomics each
     hipLaunchKernelGGL(k_ref_dt_max, dim3(nb), dim3(256), 0, c->stream, g, r.attr, r.u, r.v, r.w, r.nu, r.alpha,
                        r.maxbits);
     LAUNCH_CHECK();
-    unsigned long long mb[3];
-    HIPC(hipMemcpyAsync(mb, r.maxbits, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(hipMemcpyAsync(c->h_maxbits, r.maxbits, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));   // the only wait of a step: the host needs DT for the eddies
+    drain_probes(c);
     double mv, mn, ma;
-    memcpy(&mv, &mb[0], 8); memcpy(&mn, &mb[1], 8); memcpy(&ma, &mb[2], 8);
+    memcpy(&mv, &c->h_maxbits[0], 8); memcpy(&mn, &c->h_maxbits[1], 8); memcpy(&ma, &c->h_maxbits[2], 8);
     double maxvel = mv > LES_SMALL ? mv : c->p.inlet_velocity;
     double dtc = c->p.courant * g.dx / (maxvel + LES_SMALL);
     double dtn = c->p.courant_diffusion * (g.dx * g.dx) / (mn + LES_SMALL);
@@ -481,8 +522,12 @@ static int ref_step(les_gpu* c, les_step_report* rep, bool last)
     // SEM eddies + inlet noise on the host (glibc rand() stream), plane on the device
     host_sem_and_noise(c, DT);
     size_t plane = (size_t)c->p.ny * c->p.nz;
-    HIPC(hipMemcpyAsync(r.vort, c->vort.data(), c->vort.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPC(hipMemcpyAsync(r.noise, c->noise_h.data(), 3 * plane * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // staged in pinned memory, which the previous step's copies have finished
+    // reading (the stream was synchronised above)
+    memcpy(c->h_vort, c->vort.data(), c->vort.size() * sizeof(double));
+    memcpy(c->h_noise, c->noise_h.data(), 3 * plane * sizeof(double));
+    HIPC(hipMemcpyAsync(r.vort, c->h_vort, (c->vort.size() + 3 * plane) * sizeof(double), hipMemcpyHostToDevice,
+                        c->stream));
     hipLaunchKernelGGL(k_ref_sem_plane, dim3((unsigned)((plane + 3) / 4)), dim3(256), 0, c->stream, g, r.attr, r.vort,
                        c->p.num_vortices, r.inlet);
     hipLaunchKernelGGL(k_ref_sem_normalise, dim3(1), dim3(SEM_NORM_THREADS), 0, c->stream, g, r.inlet_idx, c->n_inlet, r.inlet,
@@ -514,7 +559,9 @@ static int ref_step(les_gpu* c, les_step_report* rep, bool last)
                            r.pt_ring + (size_t)(t % c->pt_cap_r) * LES_NPOINT_MAX * 4);
         LAUNCH_CHECK();
     }
-    // probes (:4572-4574) into the host row buffer
+    // probes (:4572-4574): rows copied into pinned memory, turned into rows
+    // at the next synchronisation (the next step's DT wait or the end of
+    // les_gpu_step)
     if (c->p.num_probe_x > 0) {
         ProbeArgs pa;
         memset(&pa, 0, sizeof(pa));
@@ -523,24 +570,11 @@ static int ref_step(les_gpu* c, les_step_report* rep, bool last)
         hipLaunchKernelGGL(k_ref_probe, dim3(1), dim3(64), 0, c->stream, g, pa, r.u, r.v, r.w, r.G, r.nu_t, r.S_mag,
                            r.probe_rows, r.probe_flags);
         LAUNCH_CHECK();
-        int flags[LES_NPROBE_MAX];
-        std::vector<double> rows((size_t)pa.n_station * g.ny * 8);
-        HIPC(hipMemcpyAsync(flags, r.probe_flags, pa.n_station * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPC(hipMemcpyAsync(rows.data(), r.probe_rows, rows.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPC(hipStreamSynchronize(c->stream));
-        for (int n = 0; n < pa.n_station; ++n) {
-            if (!flags[n]) continue;
-            for (int j = 0; j < g.ny; ++j) {
-                const double* q = &rows[((size_t)n * g.ny + j) * 8];
-                double yp = j * g.dy, yr = yp - q[0];
-                if (fabs(yr) > 1.0) continue;                 // SAMPLING_RANGE (:280)
-                les_probe_row pr;
-                pr.timestep = t; pr.station = n; pr.x_location = c->p.probe_x[n];
-                pr.y_flame = q[0]; pr.y_physical = yp; pr.y_relative = yr;
-                pr.u = q[1]; pr.v = q[2]; pr.w = q[3]; pr.G = q[4]; pr.nu_t = q[5]; pr.S_mag = q[6];
-                c->probe.push_back(pr);
-            }
-        }
+        HIPC(hipMemcpyAsync(c->h_probe_flags, r.probe_flags, pa.n_station * sizeof(int), hipMemcpyDeviceToHost,
+                            c->stream));
+        HIPC(hipMemcpyAsync(c->h_probe_rows, r.probe_rows, (size_t)pa.n_station * g.ny * 8 * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+        c->probe_pending = t;
     }
     if (c->p.stats_start_step >= 0 && t >= c->p.stats_start_step) {
         StatPtrs st;
@@ -659,6 +693,7 @@ extern "C" int les_gpu_init_state(les_gpu* c)
         c->step = 0;
         c->stat_count = 0;
         c->probe.clear();
+        c->probe_pending = -1;
         return LES_OK;
     });
 }
@@ -757,6 +792,7 @@ extern "C" int les_gpu_step(les_gpu* c, int n_steps, les_step_report* rep)
         }
         HIPC(hipEventRecord(c->ev[3], c->stream));
         HIPC(hipEventSynchronize(c->ev[3]));
+        drain_probes(c);
         float ms = 0.f;
         HIPC(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
         c->step_ms = n_steps > 0 ? ms / n_steps : 0.0;
@@ -1028,6 +1064,7 @@ extern "C" int les_gpu_import_aux(les_gpu* c, const void* buf, size_t bytes)
         c->stat_count = a.stat_count;
         c->pt_base_r = a.step;
         c->probe.clear();
+        c->probe_pending = -1;
         return LES_OK;
     });
 }

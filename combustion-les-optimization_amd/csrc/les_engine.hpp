@@ -72,6 +72,16 @@ struct les_gpu {
     int npts_r = 0;
     int pt_cap_r = 4096;
     int pt_base_r = 0;
+    // pinned host staging of the reference scheme's step (hipHostMalloc): the
+    // DT maxima read back, the eddies and inlet noise sent, and one step's
+    // probe rows, which are turned into rows at the next synchronisation
+    unsigned long long* h_maxbits = nullptr;
+    double* h_vort = nullptr;
+    double* h_noise = nullptr;
+    int* h_probe_flags = nullptr;
+    double* h_probe_rows = nullptr;
+    int probe_pending = -1;        // step whose rows are in h_probe_rows, not yet turned into rows
+    std::vector<void*> host_allocs;
     R r;
     DuctState* d = nullptr;
 };

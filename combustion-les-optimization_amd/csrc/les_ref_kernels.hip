@@ -911,16 +911,24 @@ __global__ void __launch_bounds__(256) k_ref_dt_max(GridDims g, const int* attr,
             if (alpha[q] > ma) ma = alpha[q];
         }
     }
-    // wavefront max via shuffles, then one atomic per wavefront
+    // wavefront max via shuffles, the block's four wave maxima through LDS,
+    // then one atomic per block and value: the atomics on the three words are
+    // serialised, so their count (not the cell loop) sets this kernel's time
+    // when every wave issues its own
+    __shared__ double red[3][4];
     for (int o = 32; o > 0; o >>= 1) {
         mv = fmax(mv, __shfl_xor(mv, o));
         mn = fmax(mn, __shfl_xor(mn, o));
         ma = fmax(ma, __shfl_xor(ma, o));
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomic_max_pos(&out3[0], mv);
-        atomic_max_pos(&out3[1], mn);
-        atomic_max_pos(&out3[2], ma);
+    const int wv = (int)threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][wv] = mv; red[1][wv] = mn; red[2][wv] = ma; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int c3 = (int)threadIdx.x;
+        double m = red[c3][0];
+        for (int b = 1; b < (int)(blockDim.x >> 6); ++b) m = fmax(m, red[c3][b]);
+        atomic_max_pos(&out3[c3], m);
     }
 }
 
