@@ -253,12 +253,14 @@ static int create_ref(les_gpu* c)
     c->mon_partial = dalloc(c, 7 * 512);
     r.rho_avg = dalloc(c, 1);
     r.maxbits = (unsigned long long*)dalloc(c, 4);
+    r.dt = dalloc(c, 1);
     r.probe_rows = dalloc(c, (size_t)LES_NPROBE_MAX * c->p.ny * 8);
     void* fl = nullptr;
     HIPC(hipMalloc(&fl, 64 * sizeof(int)));
     c->allocs.push_back(fl);
     r.probe_flags = (int*)fl;
-    if (!r.inlet || !r.noise || !r.vort || !r.partial || !r.rho_avg || !r.maxbits || !r.probe_rows || !c->mon_partial)
+    if (!r.inlet || !r.noise || !r.vort || !r.partial || !r.rho_avg || !r.maxbits || !r.dt || !r.probe_rows ||
+        !c->mon_partial)
         return set_err(c, LES_ERR_HIP, "device allocation failed");
     void* at = nullptr;
     HIPC(hipMalloc(&at, n * sizeof(int)));
@@ -300,6 +302,7 @@ static int create_ref(les_gpu* c)
         c->h_maxbits = (unsigned long long*)hp[0];
         c->h_vort = (double*)hp[1];
         c->h_noise = c->h_vort + nvort;
+        HIPC(hipHostGetDevicePointer(&c->h_vort_dev, c->h_vort, 0));
         c->h_probe_flags = (int*)hp[2];
         c->h_probe_rows = (double*)hp[3];
     }
@@ -346,7 +349,8 @@ extern "C" int les_gpu_create(const les_params* p, const les_mixture* mix, les_g
         }
         if (rc == LES_OK) {
             if (hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess ||
-                hipEventCreate(&c->ev[2]) != hipSuccess || hipEventCreate(&c->ev[3]) != hipSuccess)
+                hipEventCreate(&c->ev[2]) != hipSuccess || hipEventCreate(&c->ev[3]) != hipSuccess ||
+                hipEventCreateWithFlags(&c->ev_dt, hipEventDisableTiming) != hipSuccess)
                 rc = LES_ERR_HIP;
         }
         if (rc == LES_OK) rc = (p->mode == LES_MODE_REFERENCE) ? create_ref(c) : duct_create(c);
@@ -368,6 +372,7 @@ extern "C" void les_gpu_destroy(les_gpu* c)
     for (void* p : c->allocs) (void)hipFree(p);
     for (void* p : c->host_allocs) (void)hipHostFree(p);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->ev_dt) (void)hipEventDestroy(c->ev_dt);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -495,57 +500,76 @@ static void drain_probes(les_gpu* c)
     c->probe_pending = -1;
 }
 
+// n doubles from device-visible pinned host memory into device memory
+__global__ void __launch_bounds__(256) k_pull_doubles(const double* src, double* dst, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 static int ref_step(les_gpu* c, les_step_report* rep, bool last)
 {
     const GridDims& g = c->g;
     R& r = c->r;
     RefConsts rc = make_ref_consts(c);
     int t = c->step;
-    // DT (:4399-4411): exact device maxima, DT on the host
+    // DT (:4399-4411): exact device maxima, DT computed on the device for the
+    // RK stages and, from the same maxima read back, on the host for the
+    // eddies; the RK stages are queued before the host waits, so the host's
+    // eddy update overlaps them
     HIPC(hipMemsetAsync(r.maxbits, 0, 4 * sizeof(unsigned long long), c->stream));
     int nb = (int)((g.n + 255) / 256);
     if (nb > 256) nb = 256;                 // one block per CU: 3 atomics each
     hipLaunchKernelGGL(k_ref_dt_max, dim3(nb), dim3(256), 0, c->stream, g, r.attr, r.u, r.v, r.w, r.nu, r.alpha,
                        r.maxbits);
+    hipLaunchKernelGGL(k_ref_dt_final, dim3(1), dim3(64), 0, c->stream, r.maxbits, c->p.inlet_velocity, c->p.courant,
+                       c->p.courant_diffusion, g.dx, r.dt);
     LAUNCH_CHECK();
     HIPC(hipMemcpyAsync(c->h_maxbits, r.maxbits, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));   // the only wait of a step: the host needs DT for the eddies
+    HIPC(hipEventRecord(c->ev_dt, c->stream));
+
+    // RK4 (:4420-4544)
+    RefStagePtrs sp = stage_ptrs(c);
+    int nbc = (int)((g.n + 255) / 256);
+    hipLaunchKernelGGL(k_ref_save, dim3(nbc), dim3(256), 0, c->stream, g, sp);
+    for (int s = 0; s < 4; ++s) {
+        int e = ref_rhs(c, s > 0, s, t);
+        if (e != LES_OK) return e;
+        if (s < 3) hipLaunchKernelGGL(k_ref_stage, dim3(nbc), dim3(256), 0, c->stream, g, rc, sp, s, r.dt);
+        else hipLaunchKernelGGL(k_ref_final, dim3(nbc), dim3(256), 0, c->stream, g, rc, sp, r.dt);
+        LAUNCH_CHECK();
+    }
+
+    // the step's one host wait: DT's maxima (and everything queued before
+    // them, the previous step's probe rows and copies included)
+    HIPC(hipEventSynchronize(c->ev_dt));
     drain_probes(c);
     double mv, mn, ma;
     memcpy(&mv, &c->h_maxbits[0], 8); memcpy(&mn, &c->h_maxbits[1], 8); memcpy(&ma, &c->h_maxbits[2], 8);
-    double maxvel = mv > LES_SMALL ? mv : c->p.inlet_velocity;
-    double dtc = c->p.courant * g.dx / (maxvel + LES_SMALL);
-    double dtn = c->p.courant_diffusion * (g.dx * g.dx) / (mn + LES_SMALL);
-    double dta = c->p.courant_diffusion * (g.dx * g.dx) / (ma + LES_SMALL);
-    double DT = fmin(dtc, fmin(dtn, dta));
+    const double maxvel = mv > LES_SMALL ? mv : c->p.inlet_velocity;
+    const double DT = ref_dt_rule(mv, mn, ma, c->p.inlet_velocity, c->p.courant, c->p.courant_diffusion, g.dx);
 
-    // SEM eddies + inlet noise on the host (glibc rand() stream), plane on the device
+    // SEM eddies + inlet noise on the host (glibc rand() stream), plane on the
+    // device: the plane is read only by the inflow boundary condition, so it
+    // is built after the RK stages are queued
     host_sem_and_noise(c, DT);
     size_t plane = (size_t)c->p.ny * c->p.nz;
-    // staged in pinned memory, which the previous step's copies have finished
-    // reading (the stream was synchronised above)
+    // staged in pinned memory, which the previous step's copy has finished
+    // reading (it was queued before the event waited on above)
     memcpy(c->h_vort, c->vort.data(), c->vort.size() * sizeof(double));
     memcpy(c->h_noise, c->noise_h.data(), 3 * plane * sizeof(double));
-    HIPC(hipMemcpyAsync(r.vort, c->h_vort, (c->vort.size() + 3 * plane) * sizeof(double), hipMemcpyHostToDevice,
-                        c->stream));
+    {   // pulled by a kernel from the pinned block: a DMA copy here would wait
+        // for the queue hand-over between the copy engine and the kernels
+        const long long nn = (long long)(c->vort.size() + 3 * plane);
+        hipLaunchKernelGGL(k_pull_doubles, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream,
+                           (const double*)c->h_vort_dev, r.vort, nn);
+    }
     hipLaunchKernelGGL(k_ref_sem_plane, dim3((unsigned)((plane + 3) / 4)), dim3(256), 0, c->stream, g, r.attr, r.vort,
                        c->p.num_vortices, r.inlet);
     hipLaunchKernelGGL(k_ref_sem_normalise, dim3(1), dim3(SEM_NORM_THREADS), 0, c->stream, g, r.inlet_idx, c->n_inlet, r.inlet,
                        c->p.inlet_velocity * c->p.turbulence_intensity);
     LAUNCH_CHECK();
 
-    // RK4 (:4420-4544)
-    RefStagePtrs sp = stage_ptrs(c);
-    int nbc = (int)((g.n + 255) / 256);
-    hipLaunchKernelGGL(k_ref_save, dim3(nbc), dim3(256), 0, c->stream, g, sp);
-    const double cf[3] = {0.5 * DT, 0.5 * DT, 1.0 * DT};
-    for (int s = 0; s < 4; ++s) {
-        int e = ref_rhs(c, s > 0, s, t);
-        if (e != LES_OK) return e;
-        if (s < 3) hipLaunchKernelGGL(k_ref_stage, dim3(nbc), dim3(256), 0, c->stream, g, rc, sp, s, cf[s]);
-        else hipLaunchKernelGGL(k_ref_final, dim3(nbc), dim3(256), 0, c->stream, g, rc, sp, DT / 6.0);
-        LAUNCH_CHECK();
-    }
     // boundary conditions (:2967-3161)
     hipLaunchKernelGGL(k_ref_bc_x, dim3((g.nz + 63) / 64, g.ny), dim3(64), 0, c->stream, g, rc, sp, r.inlet, r.noise);
     hipLaunchKernelGGL(k_ref_bc_y, dim3((g.nz + 63) / 64, g.nx - 2), dim3(64), 0, c->stream, g, rc, sp);

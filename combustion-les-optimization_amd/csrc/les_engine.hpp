@@ -32,6 +32,7 @@ struct R {
     double *inlet, *noise, *vort, *partial, *rho_avg;
     int* inlet_idx;          // plane indices of the inlet cells (attribute 1), plane order
     unsigned long long* maxbits;
+    double* dt;              // the step's DT (k_ref_dt_final)
     double* probe_rows;
     int* probe_flags;
     double* stat[10];
@@ -62,6 +63,7 @@ struct les_gpu {
     int stat_count = 0;
     // timing (HIP events on the engine's stream)
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev_dt = nullptr;    // a reference-scheme step's DT maxima are on the host
     int timing = 0;
     double rhs_ms_sum = 0.0;
     int rhs_launches = 0;
@@ -77,6 +79,7 @@ struct les_gpu {
     // probe rows, which are turned into rows at the next synchronisation
     unsigned long long* h_maxbits = nullptr;
     double* h_vort = nullptr;
+    void* h_vort_dev = nullptr;    // h_vort's device address (read by k_pull_doubles)
     double* h_noise = nullptr;
     int* h_probe_flags = nullptr;
     double* h_probe_rows = nullptr;

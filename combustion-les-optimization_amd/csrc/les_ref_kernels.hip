@@ -704,10 +704,26 @@ __device__ __forceinline__ void eos_cell(const RefConsts& rc, const double* Yc, 
     }
 }
 
-__global__ void __launch_bounds__(256) k_ref_stage(GridDims g, RefConsts rc, RefStagePtrs s, int stage, double cdt)
+// DT of the step into device memory, so the RK stages need not wait for the
+// host (which reads the same maxima back for the eddies)
+__global__ void k_ref_dt_final(const unsigned long long* maxbits, double inlet_velocity, double courant,
+                               double courant_diffusion, double dx, double* dt)
+{
+    if (threadIdx.x != 0) return;
+    const double mv = __longlong_as_double((long long)maxbits[0]);
+    const double mn = __longlong_as_double((long long)maxbits[1]);
+    const double ma = __longlong_as_double((long long)maxbits[2]);
+    dt[0] = ref_dt_rule(mv, mn, ma, inlet_velocity, courant, courant_diffusion, dx);
+}
+
+// stage < 3: the RK4 stage update with coefficient 0.5 DT, 0.5 DT, DT (:4432-4455)
+__global__ void __launch_bounds__(256) k_ref_stage(GridDims g, RefConsts rc, RefStagePtrs s, int stage,
+                                                   const double* dtp)
 {
     long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= g.n) return;
+    const double DT = dtp[0];
+    const double cdt = stage < 2 ? 0.5 * DT : 1.0 * DT;
     int a = s.attr[q];
     double Tn, Gn, Yn[LES_NSP];
     if (a == 3) {
@@ -738,10 +754,11 @@ __global__ void __launch_bounds__(256) k_ref_stage(GridDims g, RefConsts rc, Ref
     }
 }
 
-__global__ void __launch_bounds__(256) k_ref_final(GridDims g, RefConsts rc, RefStagePtrs s, double h)
+__global__ void __launch_bounds__(256) k_ref_final(GridDims g, RefConsts rc, RefStagePtrs s, const double* dtp)
 {
     long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= g.n) return;
+    const double h = dtp[0] / 6.0;
     int a = s.attr[q];
     if (a != 3 && a != 2) return;
     double* const* k1 = s.k[0];

@@ -48,6 +48,19 @@ struct RefGroups {
     unsigned mask[REF_GROUPS_MAX];
 };
 
+// The DT rule (:4399-4411) from the three exact maxima; the host (for the
+// eddies) and the device (for the RK stages) evaluate this same code, so both
+// hold the same DT.
+__host__ __device__ inline double ref_dt_rule(double mv, double mn, double ma, double inlet_velocity,
+                                              double courant, double courant_diffusion, double dx)
+{
+    double maxvel = mv > LES_SMALL ? mv : inlet_velocity;
+    double dtc = courant * dx / (maxvel + LES_SMALL);
+    double dtn = courant_diffusion * (dx * dx) / (mn + LES_SMALL);
+    double dta = courant_diffusion * (dx * dx) / (ma + LES_SMALL);
+    return fmin(dtc, fmin(dtn, dta));
+}
+
 struct ProbeArgs {
     int n_station;
     int i_probe[LES_NPROBE_MAX];
