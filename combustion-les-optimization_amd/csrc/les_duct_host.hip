@@ -427,7 +427,6 @@ static int stage_launch(les_gpu* c, int stage, real* const* Lout)
     const int G = d->grid;
     const int qin_B = stage > 0, qout_B = stage < 2;
     hipLaunchKernelGGL((k_duct_filter<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, qin_B, d->xcd_map);
-    hipLaunchKernelGGL(k_duct_rho_final, dim3(d->max_slots), dim3(1024), 0, c->stream, d->P);
     hipLaunchKernelGGL((k_duct_sgs<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, stage, d->xcd_map);
     DLAUNCH_CHECK();
     RhsArgs<real> a;

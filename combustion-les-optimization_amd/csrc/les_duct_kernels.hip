@@ -18,8 +18,8 @@
 //
 // Per RK stage:
 //   k_duct_filter      state -> filtered primitives u, v, w, T, phi (+ per-tile sums of rho)
-//   k_duct_rho_final   per-slot domain-mean density (fixed-order reduction)
-//   k_duct_sgs         nu_t, S_T (+ per-tile max nu / alpha for the next DT)
+//   k_duct_sgs         nu_t, S_T (+ per-tile max nu / alpha for the next DT; the
+//                      run's first tile reduces its domain-mean density)
 //   k_duct_rhs_scalar  phi and T: RHS fused with the SSP-RK3 combine
 //   k_duct_rhs_momentum rho*u, rho*v, rho*w: RHS fused with the combine
 // Per step: DT, eddy update, inflow plane, its normalisation (on a side
@@ -572,24 +572,6 @@ __global__ void __launch_bounds__(256) k_duct_filter(Pool P, int qin_is_B, int x
     }
 }
 
-// per-slot domain-mean density: fixed-order reduction of the slot's tile sums
-__global__ void __launch_bounds__(1024) k_duct_rho_final(Pool P)
-{
-    __shared__ double red[1024];
-    const int s = blockIdx.x;
-    const SlotDesc& D = P.desc[s];
-    if (!D.active || P.dyn[s].done) return;
-    double v = 0.0;
-    for (int b = threadIdx.x; b < D.ntiles; b += 1024) v += P.rho_part[D.tile0 + b];
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) P.dyn[s].rho_avg = red[0] / (double)D.g.n;
-}
-
 // ---------------------------------------------------------------------------
 // K2: properties, nu_t (:1569-1633), S_T (:1913-1965); per-tile DT maxima
 // ---------------------------------------------------------------------------
@@ -682,6 +664,17 @@ __global__ void __launch_bounds__(256) k_duct_sgs(Pool P, int stage, int xcd_map
             double* dp = P.dpart + 2 * ((long long)stage * P.tile_cap + t);
             dp[0] = mn;
             dp[1] = ma;
+        }
+        // the run's first tile also reduces the run's per-tile sums of rho
+        // (written by the filter) into its domain-mean density, in a fixed
+        // order (strided per thread, then the block's fixed-order sum); the
+        // momentum kernel of this stage reads it
+        if (t == ldc(&P.desc[tp.s].tile0)) {
+            const int nt = ldc(&P.desc[tp.s].ntiles);
+            double v = 0.0;
+            for (int b = threadIdx.x; b < nt; b += 256) v += P.rho_part[t + b];
+            const double tot = block_sum256(v, red);
+            if (threadIdx.x == 0) P.dyn[tp.s].rho_avg = tot / (double)g.n;
         }
     }
 }
