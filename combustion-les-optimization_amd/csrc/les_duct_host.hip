@@ -126,11 +126,10 @@ struct DuctState {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   // [0] plain step, [1] with reinitialisation
     bool graph_ok[2] = {false, false};
     int use_graph = 1;
-    // side stream for the launches that run beside the main chain (inflow
-    // turbulence beside the RK stages, point/probe output beside the
-    // statistics), forked from and joined back into the main stream
+    // side stream for the inflow turbulence, which runs beside the RK stages,
+    // forked from and joined back into the main stream
     hipStream_t aux = nullptr;
-    hipEvent_t evf[2] = {nullptr, nullptr}, evj[2] = {nullptr, nullptr};
+    hipEvent_t evf[1] = {nullptr}, evj[1] = {nullptr};   // fork / join of the inflow branch
 };
 
 static void grid_dims(const les_params& p, GridDims& g)
@@ -212,7 +211,7 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
     if (!d) return LES_ERR_HIP;
     c->d = d;
     if (hipStreamCreateWithFlags(&d->aux, hipStreamNonBlocking) != hipSuccess) return LES_ERR_HIP;
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 1; ++a)
         if (hipEventCreateWithFlags(&d->evf[a], hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&d->evj[a], hipEventDisableTiming) != hipSuccess)
             return LES_ERR_HIP;
@@ -514,18 +513,18 @@ static int step_launches(les_gpu* c, bool reinit)
         }
         DLAUNCH_CHECK();
     }
-    // outputs of the new state: point samples and probe rows beside the statistics
-    if ((e = side_fork(c, 1)) != LES_OK) return e;
-    hipLaunchKernelGGL((k_duct_points<real>), dim3(S), dim3(64), 0, side, d->P);
+    // outputs of the new state: point samples, probe rows, statistics, in
+    // line (a side branch beside the statistics measured slower: the join
+    // costs more than the short point and probe launches)
+    hipLaunchKernelGGL((k_duct_points<real>), dim3(S), dim3(64), 0, c->stream, d->P);
     if (d->probe_cap > 0)
-        hipLaunchKernelGGL((k_duct_probe<real>), dim3(LES_NPROBE_MAX, S), dim3(256), 0, side, d->P);
+        hipLaunchKernelGGL((k_duct_probe<real>), dim3(LES_NPROBE_MAX, S), dim3(256), 0, c->stream, d->P);
     {
         const long long plm = (long long)d->ny_max * d->nz_max;
         const int sch = (int)std::min<long long>(STAT_CHUNKS, std::max<long long>(1, (plm + STAT_CHUNK_CELLS - 1) / STAT_CHUNK_CELLS));
         hipLaunchKernelGGL((k_duct_stats<real>), dim3(sch, d->nx_max, S), dim3(256), 0, c->stream, d->P);
     }
     hipLaunchKernelGGL(k_duct_stats_final, dim3((d->nx_max * 10 + 255) / 256, S), dim3(256), 0, c->stream, d->P);
-    if ((e = side_join(c, 1)) != LES_OK) return e;
     hipLaunchKernelGGL(k_duct_step_end, dim3((S + 255) / 256), dim3(256), 0, c->stream, d->P, S);
     DLAUNCH_CHECK();
     return LES_OK;
@@ -969,7 +968,7 @@ void duct_destroy(les_gpu* c)
         (void)hipStreamSynchronize(c->d->aux);
         (void)hipStreamDestroy(c->d->aux);
     }
-    for (int a = 0; a < 2; ++a) {
+    for (int a = 0; a < 1; ++a) {
         if (c->d->evf[a]) (void)hipEventDestroy(c->d->evf[a]);
         if (c->d->evj[a]) (void)hipEventDestroy(c->d->evj[a]);
     }

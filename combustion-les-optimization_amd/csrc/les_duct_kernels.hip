@@ -25,7 +25,7 @@
 // Per step: DT, eddy update, inflow plane, its normalisation (on a side
 // stream, beside the stages), the 3 stages, projection relaxation (runs with
 // proj_iters > 0), inflow/outflow planes, reinitialisation (runs due),
-// statistics beside point series and probe rows, step end.  The host side
+// point series, probe rows, statistics, step end.  The host side
 // (pool, launch sequence, graphs, ABI) is in les_duct_host.hip.
 #include "les_engine.hpp"
 
