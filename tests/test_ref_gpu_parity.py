@@ -116,6 +116,33 @@ def test_steps_match_oracle(pair, start):
             assert abs(getattr(a, f) - getattr(b, f)) <= TOL * max(1.0, abs(getattr(b, f))), f
 
 
+def test_probe_rows_of_a_multi_step_call(pair):
+    """One les_gpu_step call of five steps: each step's probe rows are copied
+    to pinned memory and turned into rows at the next step's wait, the last
+    step's at the end of the call; the rows of all its steps equal the
+    oracle's, in order."""
+    g, o = pair
+    print("case: probe rows of one 5-step call", flush=True)
+    g.init_state()
+    o.init_state()
+    st = synthetic_reference_state(o, seed=11)      # a flame band at every station from the start
+    g.upload_state(st)
+    o.set_state(st)
+    g.probe_rows()
+    o.probe_rows()
+    g.step(5)
+    for _ in range(5):
+        o.step(1)
+    pg, po = g.probe_rows(), o.probe_rows()
+    assert len(pg) == len(po) and len(po) > 0
+    assert sorted({a.timestep for a in pg}) == sorted({b.timestep for b in po})
+    assert len({a.timestep for a in pg}) >= 2            # rows of several steps of the one call
+    for a, b in zip(pg, po):
+        assert (a.timestep, a.station) == (b.timestep, b.station)
+        for f in ["y_flame", "u", "v", "w", "G", "nu_t", "S_mag"]:
+            assert abs(getattr(a, f) - getattr(b, f)) <= TOL * max(1.0, abs(getattr(b, f))), f
+
+
 def test_step_report_monitor(pair):
     g, o = pair
     print("case: monitor reductions", flush=True)
