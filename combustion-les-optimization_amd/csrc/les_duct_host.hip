@@ -734,6 +734,10 @@ static int pool_autotune(les_gpu* c, const les_params& probe, bool xcd_only = fa
     if (xcd_only) tks[0] = d->tk;
     double best = 1e30;
     int btk = d->tk, bx = d->xcd_map;
+    // two passes over the candidates, each kept at its fastest sample: the
+    // first candidates of a single pass ran on a GPU not yet at its working
+    // clocks and lost to the later ones by more than the candidates differ
+    for (int pass = 0; pass < 2; ++pass)
     for (int a = 0; a < (xcd_only ? 1 : 4); ++a)
         for (int x = 0; x < 2; ++x) {
             const long long pc = (long long)probe.nx * probe.ny * probe.nz;
@@ -950,12 +954,23 @@ int duct_create(les_gpu* c)
     // which XCD mapping is faster depends on the grid and precision (FP64 at
     // 128^3 prefers round-robin tiles, smaller grids and FP32 contiguous
     // ranges per XCD); results do not depend on it, so each run tunes it (a
-    // failed trial, e.g. no memory for the scratch pool, keeps the default)
-    if (pool_autotune(c, p, true) != LES_OK) {
+    // failed trial, e.g. no memory for the scratch pool, keeps the default),
+    // unless LES_XCD_MAP=0 or 1 fixes it
+    const char* fixed = getenv("LES_XCD_MAP");
+    if (fixed && (fixed[0] == '0' || fixed[0] == '1') && fixed[1] == 0) {
+        c->d->xcd_map = fixed[0] - '0';
+    } else if (pool_autotune(c, p, true) != LES_OK) {
         c->d->xcd_map = 1;
         c->err[0] = 0;
     }
     c->attr_h.assign((size_t)c->g.n, 3);
+    return LES_OK;
+}
+
+int duct_launch_config(const les_gpu* c, int* tile_k, int* xcd_map)
+{
+    if (tile_k) *tile_k = c->d->tk;
+    if (xcd_map) *xcd_map = c->d->xcd_map;
     return LES_OK;
 }
 

@@ -906,6 +906,12 @@ extern "C" int les_gpu_kernel_timing(const les_gpu* c, double* rhs_ms, int* laun
     });
 }
 
+extern "C" int les_gpu_launch_config(const les_gpu* c, int* tile_k, int* xcd_map)
+{
+    if (!c || !c->d || c->p.mode != LES_MODE_DUCT) return LES_ERR_PARAM;
+    return duct_launch_config(c, tile_k, xcd_map);
+}
+
 extern "C" int les_gpu_set_timing(les_gpu* c, int on)
 {
     return abi_guard(c, [&]() -> int {

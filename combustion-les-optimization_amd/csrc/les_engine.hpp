@@ -131,4 +131,5 @@ int  duct_export_aux(les_gpu* c, void* buf, size_t* bytes);
 int  duct_import_aux(les_gpu* c, const void* buf, size_t bytes);
 int  duct_set_step(les_gpu* c, int step);
 int  duct_init_sem(les_gpu* c);
+int  duct_launch_config(const les_gpu* c, int* tile_k, int* xcd_map);
 int  duct_npoints(const les_gpu* c);

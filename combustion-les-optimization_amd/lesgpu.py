@@ -20,6 +20,7 @@ EXPORTED = [
     "les_gpu_init_state", "les_gpu_upload_state", "les_gpu_download_state", "les_gpu_download_field",
     "les_gpu_step", "les_gpu_rhs", "les_gpu_probe_fetch", "les_gpu_set_params", "les_gpu_get_step",
     "les_gpu_kernel_timing",
+    "les_gpu_launch_config",
 ]
 
 _lib = None
@@ -54,6 +55,7 @@ def lib():
         L.les_gpu_get_step.argtypes = [vp]
         L.les_gpu_kernel_timing.argtypes = [vp, P(C.c_double), P(C.c_int), P(C.c_double)]
         L.les_gpu_set_timing.argtypes = [vp, C.c_int]
+        L.les_gpu_launch_config.argtypes = [vp, P(C.c_int), P(C.c_int)]
         L.les_gpu_stats_count.argtypes = [vp]
         L.les_gpu_point_fetch.argtypes = [vp, P(C.c_double), C.c_int, P(C.c_int), P(C.c_int)]
         L.les_batch_config_default.argtypes = [P(abi.BatchConfig)]
@@ -203,6 +205,14 @@ class LesGpu:
 
     def set_timing(self, on=True):
         self._check(lib().les_gpu_set_timing(self.h, 1 if on else 0), "les_gpu_set_timing")
+
+    def launch_config(self):
+        """Duct mode: the run's tile width and XCD mapping (les_gpu_launch_config);
+        None in mode R."""
+        tk, xm = C.c_int(), C.c_int()
+        if lib().les_gpu_launch_config(self.h, C.byref(tk), C.byref(xm)) != 0:
+            return None
+        return {"tile_k": tk.value, "xcd_map": xm.value}
 
     def kernel_timing(self):
         ms, n, st = C.c_double(), C.c_int(), C.c_double()

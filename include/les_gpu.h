@@ -274,6 +274,14 @@ int les_gpu_get_step(const les_gpu* ctx);
 int les_gpu_kernel_timing(const les_gpu* ctx, double* rhs_ms, int* launches, double* step_ms);
 /* Turn per-launch HIP-event timing of the RHS kernel on (1) or off (0). */
 int les_gpu_set_timing(les_gpu* ctx, int on);
+/* Launch shape a duct-mode run uses (no counterpart in the reference: the
+ * reference has no device launches): tile_k (LES_TILE_FLAT or the tile row
+ * width) and xcd_map (1: the blocks of one XCD take one contiguous tile range,
+ * 0: tiles round-robin over the XCDs).  A run times both mappings at creation
+ * and keeps the faster one; the environment variable LES_XCD_MAP=0 or 1, set
+ * before les_gpu_create, fixes the mapping instead (results do not depend on
+ * it).  LES_ERR_PARAM in mode R, whose kernels launch one thread per cell. */
+int les_gpu_launch_config(const les_gpu* ctx, int* tile_k, int* xcd_map);
 /* Number of steps accumulated in the FP64 statistics (:3191 time_avg_steps). */
 int les_gpu_stats_count(const les_gpu* ctx);
 /* Achievable HBM bandwidth: a plain device copy of `bytes` (read + write),

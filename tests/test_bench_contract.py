@@ -45,11 +45,14 @@ def test_profile_inputs_are_present_and_consistent():
     with open(os.path.join(ROOT, "profiles", "rhs_valu.json")) as f:
         valu = json.load(f)
     for wl in ("duct128-fp64", "duct128-fp32"):
-        assert bench.traffic_from_profiles(wl) == traffic[wl] > 0
+        # HBM bytes per stage of the RHS pair for each XCD mapping a run may pick
+        for xm in (0, 1):
+            assert bench.traffic_from_profiles(wl, xm) == traffic[wl][str(xm)] > 0
         v = bench.valu_from_profiles(wl)
         assert v == valu[wl] and v["insts_per_stage"] > 0
         # physical shares: a quad-cycle issues at most once without dual issue
         assert 0 < v["busy_frac"] <= 1 and 0 <= v["dual_issue_share"] <= 1 and 0 < v["wait_any_frac"] < 1
         assert abs(v["insts_per_simd_quad"] - v["busy_frac"] * (1 + v["dual_issue_share"])) < 0.05
     # FP32 moves fewer bytes than FP64 for the same stage
-    assert traffic["duct128-fp32"] < traffic["duct128-fp64"]
+    for xm in ("0", "1"):
+        assert traffic["duct128-fp32"][xm] < traffic["duct128-fp64"][xm]

@@ -128,6 +128,33 @@ def test_graph_replay_bitwise(lesgpu):
         b.close()
 
 
+def test_xcd_map_override_and_launch_config(lesgpu, monkeypatch):
+    """LES_XCD_MAP fixes a single run's XCD mapping (les_gpu_launch_config
+    reports it) and the mapping does not change results: both runs equal
+    bit for bit after 4 steps.  Mode R reports no launch shape."""
+    print("case: LES_XCD_MAP=0/1, 64x32x32 duct, 4 steps", flush=True)
+    states = []
+    for xm in (0, 1):
+        monkeypatch.setenv("LES_XCD_MAP", str(xm))
+        g = lesgpu.LesGpu(lesgpu.params(1))
+        try:
+            lc = g.launch_config()
+            assert lc["xcd_map"] == xm and lc["tile_k"] in (0, 32, 64, 128)
+            g.init_state()
+            g.step(4)
+            states.append(g.download_state(5))
+        finally:
+            g.close()
+    for x, y in zip(*states):
+        assert np.array_equal(x, y)
+    monkeypatch.delenv("LES_XCD_MAP")
+    r = lesgpu.LesGpu(lesgpu.params(0))
+    try:
+        assert r.launch_config() is None
+    finally:
+        r.close()
+
+
 def test_point_series(lesgpu):
     print("case: fixed-point series", flush=True)
     import ctypes as C
