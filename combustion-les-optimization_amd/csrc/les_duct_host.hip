@@ -314,6 +314,15 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
     return LES_OK;
 }
 
+// The stencil grid: the tile high-water mark rounded up to whole rounds of
+// the XCD mapping (8 blocks, or 8 chunks of tiles).
+static void pool_set_grid(DuctState* d)
+{
+    const int r = 8 * xcd_chunk(d->xcd_map);
+    const int grid = d->ntile_hwm > 0 ? (d->ntile_hwm + r - 1) / r * r : r;
+    if (grid != d->grid) { d->grid = grid; d->graph_ok[0] = d->graph_ok[1] = false; }
+}
+
 // Uploads the descriptor table, tile list and tile high-water mark.
 static int pool_sync_tables(les_gpu* c)
 {
@@ -322,8 +331,7 @@ static int pool_sync_tables(les_gpu* c)
     int hwm = 0;
     for (int t = 0; t < d->tile_cap; ++t) if (d->tile_slot[t] >= 0) hwm = t + 1;
     d->ntile_hwm = hwm;
-    const int grid = hwm > 0 ? (hwm + 7) / 8 * 8 : 8;
-    if (grid != d->grid) { d->grid = grid; d->graph_ok[0] = d->graph_ok[1] = false; }
+    pool_set_grid(d);
     DHIPC(hipMemcpyAsync(d->d_desc, d->desc.data(), sizeof(SlotDesc) * d->max_slots, hipMemcpyHostToDevice, c->stream));
     DHIPC(hipMemcpyAsync(d->d_tile_slot, d->tile_slot.data(), 4 * (size_t)d->tile_cap, hipMemcpyHostToDevice, c->stream));
     DHIPC(hipMemcpyAsync(d->d_ntile, &d->ntile_hwm, 4, hipMemcpyHostToDevice, c->stream));
@@ -739,7 +747,7 @@ static int pool_autotune(les_gpu* c, const les_params& probe, bool xcd_only = fa
     // clocks and lost to the later ones by more than the candidates differ
     for (int pass = 0; pass < 2; ++pass)
     for (int a = 0; a < (xcd_only ? 1 : 4); ++a)
-        for (int x = 0; x < 2; ++x) {
+        for (int x = 0; x < 4; ++x) {
             const long long pc = (long long)probe.nx * probe.ny * probe.nz;
             long long reps = d->pool_cells / pc;
             if (reps > d->max_slots) reps = d->max_slots;
@@ -802,6 +810,7 @@ static int pool_autotune(les_gpu* c, const les_params& probe, bool xcd_only = fa
         }
     d->tk = btk;
     d->xcd_map = bx;
+    pool_set_grid(d);
     d->tuned_ms = best / 3.0;
     d->graph_ok[0] = d->graph_ok[1] = false;
     return LES_OK;
@@ -957,12 +966,13 @@ int duct_create(les_gpu* c)
     // failed trial, e.g. no memory for the scratch pool, keeps the default),
     // unless LES_XCD_MAP=0 or 1 fixes it
     const char* fixed = getenv("LES_XCD_MAP");
-    if (fixed && (fixed[0] == '0' || fixed[0] == '1') && fixed[1] == 0) {
+    if (fixed && fixed[0] >= '0' && fixed[0] <= '3' && fixed[1] == 0) {
         c->d->xcd_map = fixed[0] - '0';
     } else if (pool_autotune(c, p, true) != LES_OK) {
         c->d->xcd_map = 1;
         c->err[0] = 0;
     }
+    pool_set_grid(c->d);
     c->attr_h.assign((size_t)c->g.n, 3);
     return LES_OK;
 }

@@ -327,6 +327,9 @@ __device__ __forceinline__ real face_minus(int scheme, const real* v, bool posit
 // ---------------------------------------------------------------------------
 #define DUCT_CAS __attribute__((address_space(4)))
 
+// tiles per chunk of the chunked XCD mappings (xcd_map 2 and 3); 1 otherwise
+__host__ __device__ inline int xcd_chunk(int xcd_map) { return xcd_map == 2 ? 16 : (xcd_map == 3 ? 64 : 1); }
+
 // One tile per block.  Blocks are dispatched round-robin over the 8 XCDs
 // (block b runs on XCD b % 8); with xcd_map the blocks of one XCD take one
 // contiguous range of the tile list, so tiles that share stencil halos
@@ -335,7 +338,15 @@ __device__ __forceinline__ real face_minus(int scheme, const real* v, bool posit
 __device__ __forceinline__ int tile_of_block(const Pool& P, int xcd_map)
 {
     int b = blockIdx.x;
-    if (xcd_map && (gridDim.x & 7) == 0) b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    if (xcd_map == 1 && (gridDim.x & 7) == 0) {
+        b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    } else if (xcd_map >= 2) {
+        // chunks of C consecutive tiles dealt round-robin over the XCDs (the
+        // grid is a multiple of 8 C): the XCDs sweep the tile list together,
+        // each keeping the y/z neighbours within its chunks in its own L2
+        const int C = xcd_chunk(xcd_map), m = (int)blockIdx.x >> 3;
+        b = (((m / C) << 3) + ((int)blockIdx.x & 7)) * C + m % C;
+    }
     return b < *(const DUCT_CAS int*)P.ntile ? b : -1;
 }
 

@@ -276,11 +276,12 @@ int les_gpu_kernel_timing(const les_gpu* ctx, double* rhs_ms, int* launches, dou
 int les_gpu_set_timing(les_gpu* ctx, int on);
 /* Launch shape a duct-mode run uses (no counterpart in the reference: the
  * reference has no device launches): tile_k (LES_TILE_FLAT or the tile row
- * width) and xcd_map (1: the blocks of one XCD take one contiguous tile range,
- * 0: tiles round-robin over the XCDs).  A run times both mappings at creation
- * and keeps the faster one; the environment variable LES_XCD_MAP=0 or 1, set
- * before les_gpu_create, fixes the mapping instead (results do not depend on
- * it).  LES_ERR_PARAM in mode R, whose kernels launch one thread per cell. */
+ * width) and xcd_map (how tiles are dealt to the GPU's 8 XCDs: 0 round-robin,
+ * 1 one contiguous range per XCD, 2 and 3 chunks of 16 and 64 consecutive
+ * tiles round-robin).  A run times the mappings at creation and keeps the
+ * fastest; the environment variable LES_XCD_MAP=0..3, set before
+ * les_gpu_create, fixes the mapping instead (results do not depend on it).
+ * LES_ERR_PARAM in mode R, whose kernels launch one thread per cell. */
 int les_gpu_launch_config(const les_gpu* ctx, int* tile_k, int* xcd_map);
 /* Number of steps accumulated in the FP64 statistics (:3191 time_avg_steps). */
 int les_gpu_stats_count(const les_gpu* ctx);
@@ -313,7 +314,8 @@ typedef struct les_batch_config {
     int       max_nx, max_ny, max_nz, max_vortices;   /* per-run limits */
     int       precision;       /* LES_PREC_*: one storage type per pool */
     int       tile_k;          /* tile width in k: LES_TILE_FLAT, 32, 64 or 128 */
-    int       xcd_map;         /* 1: the blocks of one XCD take one contiguous tile range */
+    int       xcd_map;         /* tiles to XCDs: 0 round-robin, 1 one contiguous range per XCD,
+                                  2 / 3 chunks of 16 / 64 tiles round-robin */
     int       autotune;        /* 1: time tile-width and XCD-mapping candidates at create
                                   on the probe grid and keep the fastest */
     int       point_cap;       /* steps kept in each run's point-series ring */

@@ -194,19 +194,21 @@ def test_xface_copy_fills_ghost_planes(lesgpu, precision):
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
 def test_xcd_mapping_does_not_change_results(lesgpu, precision):
     """The block-to-tile mapping only changes which XCD runs a tile: the state
-    and profiles are bitwise the same under both mappings (single runs tune
-    the mapping per run, so their results must not depend on it)."""
+    and profiles are bitwise the same under all four mappings (round-robin,
+    contiguous ranges, chunks of 16 and of 64 tiles; single runs tune the
+    mapping per run, so their results must not depend on it)."""
     print("case: XCD mapping neutrality, precision=%d" % precision, flush=True)
     p = run_params(lesgpu, precision, nx=64, ny=32, nz=64, seed=9)
     out = []
-    for xm in (0, 1):
+    for xm in (0, 1, 2, 3):
         with pool(lesgpu, precision, slots=1, cells=p.nx * p.ny * p.nz, xcd_map=xm) as b:
             s = b.admit(p, n_steps=6)
             b.step(6)
             out.append((b.download(s), b.field(s, "profiles", 10 * p.nx)))
-    for f in range(5):
-        assert np.array_equal(out[0][0][f], out[1][0][f]), f
-    assert np.array_equal(out[0][1], out[1][1])
+    for o in out[1:]:
+        for f in range(5):
+            assert np.array_equal(out[0][0][f], o[0][f]), f
+        assert np.array_equal(out[0][1], o[1])
 
 
 def wrinkled_front_state(p, amp=3e-3):

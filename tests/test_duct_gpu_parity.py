@@ -130,11 +130,12 @@ def test_graph_replay_bitwise(lesgpu):
 
 def test_xcd_map_override_and_launch_config(lesgpu, monkeypatch):
     """LES_XCD_MAP fixes a single run's XCD mapping (les_gpu_launch_config
-    reports it) and the mapping does not change results: both runs equal
-    bit for bit after 4 steps.  Mode R reports no launch shape."""
-    print("case: LES_XCD_MAP=0/1, 64x32x32 duct, 4 steps", flush=True)
+    reports it) and the mapping does not change results: the runs under all
+    four mappings equal bit for bit after 4 steps.  Mode R reports no launch
+    shape."""
+    print("case: LES_XCD_MAP=0..3, 64x32x32 duct, 4 steps", flush=True)
     states = []
-    for xm in (0, 1):
+    for xm in (0, 1, 2, 3):
         monkeypatch.setenv("LES_XCD_MAP", str(xm))
         g = lesgpu.LesGpu(lesgpu.params(1))
         try:
@@ -145,8 +146,9 @@ def test_xcd_map_override_and_launch_config(lesgpu, monkeypatch):
             states.append(g.download_state(5))
         finally:
             g.close()
-    for x, y in zip(*states):
-        assert np.array_equal(x, y)
+    for st in states[1:]:
+        for x, y in zip(states[0], st):
+            assert np.array_equal(x, y)
     monkeypatch.delenv("LES_XCD_MAP")
     r = lesgpu.LesGpu(lesgpu.params(0))
     try:
