@@ -103,6 +103,7 @@ struct DuctState {
     int tk = 64;
     int grid = 8;                  // tile high-water mark rounded up to a multiple of 8
     int xcd_map = 1;
+    int zigzag = 0;                // stencil launches alternate their sweep direction (XCD_REVERSE)
     double tuned_ms = 0.0;
     Pool P;
     SlotDesc* d_desc = nullptr;
@@ -230,7 +231,8 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
         d->cus = prop.multiProcessorCount;
-    d->xcd_map = cfg.xcd_map;
+    d->xcd_map = cfg.xcd_map & 3;
+    d->zigzag = (cfg.xcd_map & XCD_REVERSE) ? 1 : 0;
     // tiles of all resident runs, for any tile width: per run at most
     // n/256 + nx*ny/2 + nx*nz/32 + nx (ceil bounds of each tile shape)
     {
@@ -433,8 +435,13 @@ static int stage_launch(les_gpu* c, int stage, real* const* Lout)
     DuctState* d = c->d;
     const int G = d->grid;
     const int qin_B = stage > 0, qout_B = stage < 2;
+    // zigzag: the four launches alternate the direction in which each XCD
+    // walks its tiles, so each starts on the tiles whose outputs the previous
+    // one wrote last (still in that XCD's L2); a tuned choice (it pays for a
+    // pool of many small runs, not for one large run)
     hipLaunchKernelGGL((k_duct_filter<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, qin_B, d->xcd_map);
-    hipLaunchKernelGGL((k_duct_sgs<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, stage, d->xcd_map);
+    const int zz = d->zigzag ? XCD_REVERSE : 0;
+    hipLaunchKernelGGL((k_duct_sgs<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, stage, d->xcd_map | zz);
     DLAUNCH_CHECK();
     RhsArgs<real> a;
     a.qin_is_B = qin_B; a.qout_is_B = qout_B; a.stage = stage;
@@ -445,7 +452,9 @@ static int stage_launch(les_gpu* c, int stage, real* const* Lout)
     for (int f = 0; f < DNF; ++f) a.Lout[f] = Lout ? Lout[f] : nullptr;
     if (c->timing) DHIPC(hipEventRecord(c->ev[0], c->stream));
     hipLaunchKernelGGL((k_duct_rhs_scalar<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, a);
-    hipLaunchKernelGGL((k_duct_rhs_momentum<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, a);
+    RhsArgs<real> am = a;
+    am.xcd_map |= zz;
+    hipLaunchKernelGGL((k_duct_rhs_momentum<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, am);
     DLAUNCH_CHECK();
     if (c->timing) {
         DHIPC(hipEventRecord(c->ev[1], c->stream));
@@ -731,87 +740,116 @@ static int slot_monitor(les_gpu* c, int s, les_step_report* rep)
 // ---------------------------------------------------------------------------
 // autotuner: times each (tile width, XCD mapping) candidate on a scratch pool
 // filled with replicas of the probe run (as many as the pool's slots and cells
-// hold, so the timed launches have the pool's occupancy) and keeps the fastest.
+// hold, so the timed launches have the pool's occupancy) and keeps the fastest,
+// then times the winner with and without alternating sweep directions.
 // xcd_only: the pool's tile width stays and only the XCD mapping is tuned (it
 // does not change results: per-tile partial sums are reduced in tile order).
 // ---------------------------------------------------------------------------
 static int pool_autotune(les_gpu* c, const les_params& probe, bool xcd_only = false)
 {
     DuctState* d = c->d;
+    // one candidate's fastest of three 5-step samples (the candidates differ
+    // by a few per cent, about the spread of one short sample) on a scratch
+    // pool of replicas of the probe run; rc LES_OK with ms 0 when the scratch
+    // pool does not fit beside the real one
+    auto time_candidate = [&](int tk, int xmap, float* ms_out) -> int {
+        *ms_out = 0.f;
+        const long long pc = (long long)probe.nx * probe.ny * probe.nz;
+        long long reps = d->pool_cells / pc;
+        if (reps > d->max_slots) reps = d->max_slots;
+        if (reps < 1) reps = 1;
+        les_batch_config cfg;
+        memset(&cfg, 0, sizeof(cfg));
+        cfg.max_slots = (int)reps;
+        cfg.pool_cells = reps * pc;
+        cfg.max_nx = probe.nx; cfg.max_ny = probe.ny; cfg.max_nz = probe.nz;
+        cfg.max_vortices = probe.num_vortices > 0 ? probe.num_vortices : 1;
+        cfg.precision = d->precision;
+        cfg.tile_k = tk;
+        cfg.xcd_map = xmap;
+        cfg.table_t0 = probe.table_t0; cfg.table_dt = probe.table_dt; cfg.table_n = probe.table_n;
+        cfg.point_cap = 16;
+        les_gpu* t = new (std::nothrow) les_gpu();
+        if (!t) return LES_ERR_HIP;
+        t->p = probe;
+        t->stream = c->stream;
+        int e = pool_create(t, cfg);
+        if (e != LES_OK) {
+            // skipped (the pool keeps its configured launch shape if no
+            // candidate fits)
+            t->stream = nullptr;
+            les_gpu_destroy(t);
+            (void)hipGetLastError();
+            return LES_OK;
+        }
+        for (long long r = 0; r < reps && e == LES_OK; ++r) {
+            int slot = -1;
+            e = pool_admit(t, probe, 0, &slot);
+            if (e == LES_OK) e = slot_init(t, slot, probe);
+        }
+        if (e == LES_OK) e = pool_steps(t, 3);
+        float ms = 0.f;
+        for (int rep = 0; rep < 3 && e == LES_OK; ++rep) {
+            float m = 0.f;
+            if (hipEventRecord(c->ev[2], c->stream) != hipSuccess) break;
+            e = pool_steps(t, 5);
+            (void)hipEventRecord(c->ev[3], c->stream);
+            (void)hipEventSynchronize(c->ev[3]);
+            (void)hipEventElapsedTime(&m, c->ev[2], c->ev[3]);
+            if (m > 0 && (ms == 0.f || m < ms)) ms = m;
+        }
+        (void)hipStreamSynchronize(c->stream);
+        t->stream = nullptr;
+        les_gpu_destroy(t);
+        if (e != LES_OK) {
+            snprintf(c->err, sizeof(c->err), "autotune candidate tile_k=%d xcd_map=%d failed (%d)", tk, xmap, e);
+            return e;
+        }
+        *ms_out = ms;
+        return LES_OK;
+    };
     int tks[4] = {0, 32, 64, 128};
     if (xcd_only) tks[0] = d->tk;
+    const int zz = d->zigzag ? XCD_REVERSE : 0;
     double best = 1e30;
     int btk = d->tk, bx = d->xcd_map;
     // two passes over the candidates, each kept at its fastest sample: the
     // first candidates of a single pass ran on a GPU not yet at its working
     // clocks and lost to the later ones by more than the candidates differ
     for (int pass = 0; pass < 2; ++pass)
-    for (int a = 0; a < (xcd_only ? 1 : 4); ++a)
-        for (int x = 0; x < 4; ++x) {
-            const long long pc = (long long)probe.nx * probe.ny * probe.nz;
-            long long reps = d->pool_cells / pc;
-            if (reps > d->max_slots) reps = d->max_slots;
-            if (reps < 1) reps = 1;
-            les_batch_config cfg;
-            memset(&cfg, 0, sizeof(cfg));
-            cfg.max_slots = (int)reps;
-            cfg.pool_cells = reps * pc;
-            cfg.max_nx = probe.nx; cfg.max_ny = probe.ny; cfg.max_nz = probe.nz;
-            cfg.max_vortices = probe.num_vortices > 0 ? probe.num_vortices : 1;
-            cfg.precision = d->precision;
-            cfg.tile_k = tks[a];
-            cfg.xcd_map = x;
-            cfg.table_t0 = probe.table_t0; cfg.table_dt = probe.table_dt; cfg.table_n = probe.table_n;
-            cfg.point_cap = 16;
-            les_gpu* t = new (std::nothrow) les_gpu();
-            if (!t) return LES_ERR_HIP;
-            t->p = probe;
-            t->stream = c->stream;
-            int e = pool_create(t, cfg);
-            if (e != LES_OK) {
-                // the candidate's scratch pool does not fit beside the real
-                // one: skip it (the pool keeps its configured launch shape
-                // if no candidate fits)
-                t->stream = nullptr;
-                les_gpu_destroy(t);
-                (void)hipGetLastError();
-                continue;
+        for (int a = 0; a < (xcd_only ? 1 : 4); ++a)
+            for (int x = 0; x < 4; ++x) {
+                float ms = 0.f;
+                const int e = time_candidate(tks[a], x | zz, &ms);
+                if (e != LES_OK) return e;
+                if (ms > 0 && ms < best) {
+                    best = ms;
+                    btk = tks[a];
+                    bx = x;
+                }
             }
-            for (long long r = 0; r < reps && e == LES_OK; ++r) {
-                int slot = -1;
-                e = pool_admit(t, probe, 0, &slot);
-                if (e == LES_OK) e = slot_init(t, slot, probe);
+    int bz = d->zigzag;
+    if (!xcd_only && best < 1e29) {
+        // the winner with and without alternating sweep directions, each
+        // timed twice in turn
+        double bs[2] = {1e30, 1e30};
+        for (int pass = 0; pass < 2; ++pass)
+            for (int z = 0; z < 2; ++z) {
+                float ms = 0.f;
+                const int e = time_candidate(btk, bx | (z ? XCD_REVERSE : 0), &ms);
+                if (e != LES_OK) return e;
+                if (ms > 0 && ms < bs[z]) bs[z] = ms;
             }
-            if (e == LES_OK) e = pool_steps(t, 2);
-            // the fastest of three 3-step samples (the candidates differ by a
-            // few per cent, about the spread of one sample)
-            float ms = 0.f;
-            for (int rep = 0; rep < 3 && e == LES_OK; ++rep) {
-                float m = 0.f;
-                if (hipEventRecord(c->ev[2], c->stream) != hipSuccess) break;
-                e = pool_steps(t, 3);
-                (void)hipEventRecord(c->ev[3], c->stream);
-                (void)hipEventSynchronize(c->ev[3]);
-                (void)hipEventElapsedTime(&m, c->ev[2], c->ev[3]);
-                if (m > 0 && (ms == 0.f || m < ms)) ms = m;
-            }
-            if (e == LES_OK && ms > 0 && ms < best) {
-                best = ms;
-                btk = tks[a];
-                bx = x;
-            }
-            (void)hipStreamSynchronize(c->stream);
-            t->stream = nullptr;
-            les_gpu_destroy(t);
-            if (e != LES_OK) {
-                snprintf(c->err, sizeof(c->err), "autotune candidate tile_k=%d xcd_map=%d failed (%d)", tks[a], x, e);
-                return e;
-            }
+        if (bs[0] < 1e29 || bs[1] < 1e29) {
+            bz = bs[1] < bs[0] ? 1 : 0;
+            best = bs[bz];
         }
+    }
     d->tk = btk;
     d->xcd_map = bx;
+    d->zigzag = bz;
     pool_set_grid(d);
-    d->tuned_ms = best / 3.0;
+    d->tuned_ms = best / 5.0;
     d->graph_ok[0] = d->graph_ok[1] = false;
     return LES_OK;
 }
@@ -964,7 +1002,7 @@ int duct_create(les_gpu* c)
     // 128^3 prefers round-robin tiles, smaller grids and FP32 contiguous
     // ranges per XCD); results do not depend on it, so each run tunes it (a
     // failed trial, e.g. no memory for the scratch pool, keeps the default),
-    // unless LES_XCD_MAP=0 or 1 fixes it
+    // unless LES_XCD_MAP (0..3) fixes it
     const char* fixed = getenv("LES_XCD_MAP");
     if (fixed && fixed[0] >= '0' && fixed[0] <= '3' && fixed[1] == 0) {
         c->d->xcd_map = fixed[0] - '0';
@@ -1289,7 +1327,7 @@ extern "C" int les_batch_tuning(const les_batch* b, int* tile_k, int* grid_block
         const DuctState* d = b->c->d;
         if (tile_k) *tile_k = d->tk;
         if (grid_blocks) *grid_blocks = d->grid;
-        if (xcd_map) *xcd_map = d->xcd_map;
+        if (xcd_map) *xcd_map = d->xcd_map | (d->zigzag ? XCD_REVERSE : 0);
         if (tuned_ms) *tuned_ms = d->tuned_ms;
         return LES_OK;
     });

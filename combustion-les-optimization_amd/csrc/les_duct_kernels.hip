@@ -327,25 +327,33 @@ __device__ __forceinline__ real face_minus(int scheme, const real* v, bool posit
 // ---------------------------------------------------------------------------
 #define DUCT_CAS __attribute__((address_space(4)))
 
+#define XCD_REVERSE 4      // flag on a launch's xcd_map: its blocks walk their XCD's tiles backwards
+
 // tiles per chunk of the chunked XCD mappings (xcd_map 2 and 3); 1 otherwise
-__host__ __device__ inline int xcd_chunk(int xcd_map) { return xcd_map == 2 ? 16 : (xcd_map == 3 ? 64 : 1); }
+__host__ __device__ inline int xcd_chunk(int xcd_map) { return (xcd_map & 3) == 2 ? 16 : ((xcd_map & 3) == 3 ? 64 : 1); }
 
 // One tile per block.  Blocks are dispatched round-robin over the 8 XCDs
 // (block b runs on XCD b % 8); with xcd_map the blocks of one XCD take one
 // contiguous range of the tile list, so tiles that share stencil halos
 // (neighbours in j, k and i) are read through the same L2.  The grid is the
-// tile high-water mark rounded up to a multiple of 8.
+// tile high-water mark rounded up to whole rounds (8 blocks, or 8 chunks).
 __device__ __forceinline__ int tile_of_block(const Pool& P, int xcd_map)
 {
     int b = blockIdx.x;
+    if ((xcd_map & XCD_REVERSE) && (gridDim.x & 7) == 0) {
+        // each XCD takes the same tiles in reverse order: a launch that
+        // follows one in forward order starts on the tiles the other wrote last
+        b = (((int)(gridDim.x >> 3) - 1 - (b >> 3)) << 3) | (b & 7);
+    }
+    xcd_map &= 3;
     if (xcd_map == 1 && (gridDim.x & 7) == 0) {
-        b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+        b = (b & 7) * (int)(gridDim.x >> 3) + (b >> 3);
     } else if (xcd_map >= 2) {
         // chunks of C consecutive tiles dealt round-robin over the XCDs (the
         // grid is a multiple of 8 C): the XCDs sweep the tile list together,
         // each keeping the y/z neighbours within its chunks in its own L2
-        const int C = xcd_chunk(xcd_map), m = (int)blockIdx.x >> 3;
-        b = (((m / C) << 3) + ((int)blockIdx.x & 7)) * C + m % C;
+        const int C = xcd_chunk(xcd_map), m = b >> 3;
+        b = (((m / C) << 3) + (b & 7)) * C + m % C;
     }
     return b < *(const DUCT_CAS int*)P.ntile ? b : -1;
 }

@@ -315,7 +315,8 @@ typedef struct les_batch_config {
     int       precision;       /* LES_PREC_*: one storage type per pool */
     int       tile_k;          /* tile width in k: LES_TILE_FLAT, 32, 64 or 128 */
     int       xcd_map;         /* tiles to XCDs: 0 round-robin, 1 one contiguous range per XCD,
-                                  2 / 3 chunks of 16 / 64 tiles round-robin */
+                                  2 / 3 chunks of 16 / 64 tiles round-robin; + 4: the stencil
+                                  launches alternate the order in which each XCD walks its tiles */
     int       autotune;        /* 1: time tile-width and XCD-mapping candidates at create
                                   on the probe grid and keep the fastest */
     int       point_cap;       /* steps kept in each run's point-series ring */

@@ -148,6 +148,7 @@ def test_batch_point_series_and_autotune(lesgpu):
         t = b.tuning()
         print("tuned:", t, flush=True)
         assert t["tile_k"] in (0, 32, 64, 128) and t["grid_blocks"] > 0 and t["tuned_ms"] > 0
+        assert t["xcd_map"] in range(8)                          # mapping 0..3, +4 alternated sweeps
         s = b.admit(probe, 12)
         b.step(12)
         ser = b.point_series(s, 64)
@@ -195,12 +196,13 @@ def test_xface_copy_fills_ghost_planes(lesgpu, precision):
 def test_xcd_mapping_does_not_change_results(lesgpu, precision):
     """The block-to-tile mapping only changes which XCD runs a tile: the state
     and profiles are bitwise the same under all four mappings (round-robin,
-    contiguous ranges, chunks of 16 and of 64 tiles; single runs tune the
-    mapping per run, so their results must not depend on it)."""
+    contiguous ranges, chunks of 16 and of 64 tiles) and with the stencil
+    launches' sweep direction alternated (+4) (single runs tune the mapping
+    per run and pools the sweep, so results must not depend on either)."""
     print("case: XCD mapping neutrality, precision=%d" % precision, flush=True)
     p = run_params(lesgpu, precision, nx=64, ny=32, nz=64, seed=9)
     out = []
-    for xm in (0, 1, 2, 3):
+    for xm in (0, 1, 2, 3, 4, 6):
         with pool(lesgpu, precision, slots=1, cells=p.nx * p.ny * p.nz, xcd_map=xm) as b:
             s = b.admit(p, n_steps=6)
             b.step(6)
