@@ -435,26 +435,25 @@ static int stage_launch(les_gpu* c, int stage, real* const* Lout)
     DuctState* d = c->d;
     const int G = d->grid;
     const int qin_B = stage > 0, qout_B = stage < 2;
-    // zigzag: the four launches alternate the direction in which each XCD
-    // walks its tiles, so each starts on the tiles whose outputs the previous
-    // one wrote last (still in that XCD's L2); a tuned choice (it pays for a
-    // pool of many small runs, not for one large run)
-    hipLaunchKernelGGL((k_duct_filter<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, qin_B, d->xcd_map);
+    // zigzag: successive launches alternate the direction in which each XCD
+    // walks its tiles (three per stage, so the pattern flips from stage to
+    // stage), so each starts on the tiles whose outputs the previous one wrote
+    // last (still in that XCD's L2); a tuned choice (it pays for a pool of many
+    // small runs, not for one large run)
     const int zz = d->zigzag ? XCD_REVERSE : 0;
-    hipLaunchKernelGGL((k_duct_sgs<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, stage, d->xcd_map | zz);
+    const int zf = (stage & 1) ? zz : 0;
+    hipLaunchKernelGGL((k_duct_filter<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, qin_B, d->xcd_map | zf);
+    hipLaunchKernelGGL((k_duct_sgs<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, stage, d->xcd_map | (zz ^ zf));
     DLAUNCH_CHECK();
     RhsArgs<real> a;
     a.qin_is_B = qin_B; a.qout_is_B = qout_B; a.stage = stage;
     a.ca = (stage == 1) ? 0.75 : (1.0 / 3.0);
     a.cb = (stage == 1) ? 0.25 : (2.0 / 3.0);
     a.track_vel = (stage == 2);
-    a.xcd_map = d->xcd_map;
+    a.xcd_map = d->xcd_map | zf;
     for (int f = 0; f < DNF; ++f) a.Lout[f] = Lout ? Lout[f] : nullptr;
     if (c->timing) DHIPC(hipEventRecord(c->ev[0], c->stream));
-    hipLaunchKernelGGL((k_duct_rhs_scalar<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, a);
-    RhsArgs<real> am = a;
-    am.xcd_map |= zz;
-    hipLaunchKernelGGL((k_duct_rhs_momentum<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, am);
+    hipLaunchKernelGGL((k_duct_rhs_all<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, a);
     DLAUNCH_CHECK();
     if (c->timing) {
         DHIPC(hipEventRecord(c->ev[1], c->stream));

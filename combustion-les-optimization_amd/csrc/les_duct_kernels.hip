@@ -20,8 +20,8 @@
 //   k_duct_filter      state -> filtered primitives u, v, w, T, phi (+ per-tile sums of rho)
 //   k_duct_sgs         nu_t, S_T (+ per-tile max nu / alpha for the next DT; the
 //                      run's first tile reduces its domain-mean density)
-//   k_duct_rhs_scalar  phi and T: RHS fused with the SSP-RK3 combine
-//   k_duct_rhs_momentum rho*u, rho*v, rho*w: RHS fused with the combine
+//   k_duct_rhs_all     rho*u, rho*v, rho*w, phi, T: RHS fused with the SSP-RK3
+//                      combine (+ per-tile max |u| in the last stage)
 // Per step: DT, eddy update, inflow plane, its normalisation (on a side
 // stream, beside the stages), the 3 stages, projection relaxation (runs with
 // proj_iters > 0), inflow/outflow planes, reinitialisation (runs due),
@@ -204,12 +204,15 @@ __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >
 // Arithmetic of the two storage types.  double (the FP64 path, compared with
 // the oracle to 1e-10): the device libm's exp and log; x^y as exp(y log x);
 // x^(3/4) as two square roots; divisions by Newton-Raphson from the hardware
-// reciprocal (v_rcp_f64, two refinement steps e = 1 - b r, r += r e, then one
-// residual correction of the quotient q += r (a - b q), Markstein's scheme),
-// within about 1 ulp of
-// the IEEE quotient for the finite, normal, moderate operands these kernels
-// divide (temperatures, positive sums of squares, eddy viscosities) and
-// without the scaling and fix-up steps the IEEE sequence adds.  float (the
+// reciprocal (v_rcp_f64, one refinement step e = 1 - b r, r += r e, then one
+// residual correction of the quotient q += r (a - b q), Markstein's scheme:
+// the correction squares the refined reciprocal's relative error, so the
+// quotient is within about 1 ulp of the IEEE one; measured on 4.2 million
+// random operand pairs of magnitudes 1e-30..1e30 it equalled the IEEE
+// quotient every time, as it did with two refinement steps, while with none
+// it was up to 18 ulp off) for the finite, normal, moderate operands these
+// kernels divide (temperatures, positive sums of squares, eddy viscosities)
+// and without the scaling and fix-up steps the IEEE sequence adds.  float (the
 // FP32 form): the hardware's native reciprocal, exp2 and log2 (v_rcp_f32,
 // v_exp_f32, v_log_f32; about 1 ulp), which the FP32 tolerances cover.
 template <typename real>
@@ -217,9 +220,7 @@ struct M {
     static __device__ __forceinline__ real div(real a, real b)
     {
         real r = __builtin_amdgcn_rcp(b);
-        real e = fma(-b, r, (real)1.0);
-        r = fma(r, e, r);
-        e = fma(-b, r, (real)1.0);
+        const real e = fma(-b, r, (real)1.0);
         r = fma(r, e, r);
         const real q = a * r;
         return fma(r, fma(-b, q, a), q);
@@ -271,7 +272,7 @@ __device__ __forceinline__ real table_lookup(const double* tab, const DuctConsts
 // candidate values and the denominator.  Algebraically the textbook formula;
 // only the rounding differs from it.
 template <typename real>
-__device__ __forceinline__ real weno5_face(real v0, real v1, real v2, real v3, real v4)
+__device__ __forceinline__ void weno5_nd(real v0, real v1, real v2, real v3, real v4, real& num_out, real& den_out)
 {
     const real k = (real)1.0408329997330663, eps = (real)1.0e-6;    // k = sqrt(13/12)
     const real a0 = k * (v0 - (real)2.0 * v1 + v2), h0 = (real)0.5 * v0 - (real)2.0 * v1 + (real)1.5 * v2;
@@ -287,7 +288,16 @@ __device__ __forceinline__ real weno5_face(real v0, real v1, real v2, real v3, r
     const real q2 = (real)0.6 * v2 + (real)1.5 * v3 - (real)0.3 * v4;       // 0.3 x (2, 5, -1)
     const real num = t12 * q0 + t02 * q1 + t01 * q2;
     const real den = (real)0.6 * t12 + (real)3.6 * t02 + (real)1.8 * t01;  // 6 x (0.1, 0.6, 0.3)
-    return M<real>::div(num, den);
+    num_out = num;
+    den_out = den;
+}
+
+template <typename real>
+__device__ __forceinline__ real weno5_face(real v0, real v1, real v2, real v3, real v4)
+{
+    real n, d;
+    weno5_nd<real>(v0, v1, v2, v3, v4, n, d);
+    return M<real>::div(n, d);
 }
 
 template <typename real>
@@ -319,6 +329,28 @@ __device__ __forceinline__ real face_minus(int scheme, const real* v, bool posit
 {
     return face_left<real>(scheme, positive ? v[0] : v[5], positive ? v[1] : v[4], positive ? v[2] : v[3],
                            positive ? v[3] : v[2], positive ? v[4] : v[1]);
+}
+
+// p f(i+1/2) - m f(i-1/2) of the cell's two upwind faces (signs pos_p, pos_m).
+// FP64 WENO5: the two quotients over one division, (p n+ d- - m n- d+) / (d+ d-);
+// each denominator is a sum of products of two (eps + IS_k)^2 terms, between
+// about 1e-24 (eps = 1e-6) and 1e35 (temperature jumps of 2800 K, the clip
+// range), so their product stays far inside the FP64 range (not the FP32 one,
+// which keeps one division per face).
+template <typename real>
+__device__ __forceinline__ real face_pair(int scheme, const real* v, bool pos_p, bool pos_m, real p, real m)
+{
+    if constexpr (sizeof(real) == 8) {
+        if (scheme == LES_ADV_WENO5) {
+            real np, dp, nm, dm;
+            weno5_nd<real>(pos_p ? v[1] : v[6], pos_p ? v[2] : v[5], pos_p ? v[3] : v[4], pos_p ? v[4] : v[3],
+                           pos_p ? v[5] : v[2], np, dp);
+            weno5_nd<real>(pos_m ? v[0] : v[5], pos_m ? v[1] : v[4], pos_m ? v[2] : v[3], pos_m ? v[3] : v[2],
+                           pos_m ? v[4] : v[1], nm, dm);
+            return M<real>::div(p * np * dm - m * nm * dp, dp * dm);
+        }
+    }
+    return p * face_plus<real>(scheme, v, pos_p) - m * face_minus<real>(scheme, v, pos_m);
 }
 
 
@@ -712,131 +744,25 @@ struct RhsArgs {
     real* Lout[DNF];     // optional: the RHS itself, slot-local indices (testing hook)
 };
 
-// scalar RHS (phi: G equation :2706-2818; T :2912-2955) + SSP-RK3 combine
+// The RHS of all five fields of one RK stage fused with the SSP-RK3 combine,
+// one launch:
+//   rho*u, rho*v, rho*w (:2552-2691 structure): conservative WENO5/minmod face
+//     fluxes with face-averaged rho and velocity, mu_eff Laplacian, buoyancy
+//     (rho - rho_avg) g, flame force in the progress band;
+//   phi (G equation, :2706-2818) and T (:2912-2955): upwind convection by the
+//     cell's velocity, SGS fluxes with face-averaged nu_t, S_T |grad phi|,
+//     artificial diffusion in the flame band, conduction, nudging to the
+//     G-weighted temperature;
+// and, in the last stage, the per-tile max |u| of the new state.  Per cell the
+// quantities the equations share are formed once: each direction's line
+// offsets, the cell's velocity along it, the neighbours' T, rho and nu_t, the
+// progress variable, the property tables; the direction loop stays rolled (one
+// copy of its code and registers) and inside it the seven-point lines of u, v,
+// w, phi and T are loaded one ahead of the reconstruction that uses them.
+// FP64: 4 waves per SIMD (125 VGPRs); FP32: 6 waves (80 VGPRs; 7 spill).
 template <typename real, int TK>
-// FP64: at least 4 waves per SIMD (116 VGPRs; a 5-wave cap spills);
-// FP32: 8 waves with flat tiles, 7 with row tiles (8 would spill 16 bytes)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 4 : (TK == 0 ? 8 : 7))))
-k_duct_rhs_scalar(Pool P, RhsArgs<real> a)
-{
-    const int t = tile_of_block(P, a.xcd_map);
-    TilePos tp;
-    if (t < 0 || !tile_decode<TK>(P, t, tp)) return;
-    {
-        if (!tp.in) return;
-        const SlotView D = slot_view(P, tp.s);
-        const GridDims g = D.g;
-        const DuctConsts& k = D.k;
-        const int i = tp.i, j = tp.j, kk = tp.kk;
-        const int bi = i * (int)g.sx;
-        const int q = bi + j * g.nz + kk;
-        void* const* QI = a.qin_is_B ? P.B : P.A;
-        void* const* QO = a.qout_is_B ? P.B : P.A;
-        const auto qin3 = run_arr((const real*)QI[3] + D.off);
-        const auto qin4 = run_arr((const real*)QI[4] + D.off);
-        const auto qo3 = run_arr((real*)QO[3] + D.off);
-        const auto qo4 = run_arr((real*)QO[4] + D.off);
-        if (i == 0 || i == g.nx - 1) {
-            qo3[q] = qin3[q];
-            qo4[q] = qin4[q];
-            return;
-        }
-        const auto uf = run_buf((const real*)P.F[0] + D.off, g.n);
-        const auto vf = run_buf((const real*)P.F[1] + D.off, g.n);
-        const auto wf = run_buf((const real*)P.F[2] + D.off, g.n);
-        const auto Tf = run_buf((const real*)P.F[3] + D.off, g.n);
-        const auto pf = run_buf((const real*)P.F[4] + D.off, g.n);
-        const auto nut = run_buf((const real*)P.nut + D.off, g.n);
-        const real rut = (real)k.rho_u_Tu;
-        const bool xband = (i < 3 || i > g.nx - 4);
-        const int scheme = k.scheme;
-        const RunBuf<real> vel[3] = {uf, vf, wf};
-        const real T0 = Tf[q];
-        const real nu0 = nut[q];
-        const int xp = q + (int)g.sx, xm = q - (int)g.sx;
-        const int yp = line_idx(g, bi, j, kk, 1, 1), ym = line_idx(g, bi, j, kk, 1, -1);
-        const int zp = line_idx(g, bi, j, kk, 2, 1), zm = line_idx(g, bi, j, kk, 2, -1);
-        const RunBuf<real> S2[2] = {pf, Tf};
-        real cv[2], sg[2], lap[2], grad[3];
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const RunBuf<real> Fs = S2[s];
-            const real isc2 = rcp_const<real>(k, (s == 0) ? DUCT_RC_ISC : DUCT_RC_IPR);
-            real acc = (real)0, sacc = (real)0, lacc = (real)0;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const real id = rcp_const<real>(k, DUCT_RC_ID + d), id2 = rcp_const<real>(k, DUCT_RC_ID2 + d);
-                const real hid = rcp_const<real>(k, DUCT_RC_HID + d);
-                const int qp = d == 0 ? xp : (d == 1 ? yp : zp);
-                const int qm = d == 0 ? xm : (d == 1 ? ym : zm);
-                real ua = vel[d][q];
-                real vm1 = Fs[qm], v0 = Fs[q], vp1 = Fs[qp];
-                real tt;
-                if (d == 0 && xband) {
-                    tt = (ua >= (real)0.0) ? ua * (v0 - vm1) * id : ua * (vp1 - v0) * id;
-                } else {
-                    real v[7];
-#pragma unroll
-                    for (int o = 0; o < 7; ++o) v[o] = Fs[line_idx(g, bi, j, kk, d, o - 3)];
-                    bool pos = ua >= (real)0.0;
-                    real fp = face_plus<real>(scheme, v, pos);
-                    real fm = face_minus<real>(scheme, v, pos);
-                    tt = ua * (fp - fm) * id;
-                }
-                acc = (d == 0) ? tt : acc + tt;
-                real ntp = nut[qp], ntm = nut[qm];
-                real gp = (ntp + nu0) * isc2, gm = (nu0 + ntm) * isc2;
-                real t2 = (gp * (vp1 - v0) - gm * (v0 - vm1)) * id2;
-                sacc = (d == 0) ? t2 : sacc + t2;
-                real t3 = ((vp1 - v0) - (v0 - vm1)) * id2;
-                lacc = (d == 0) ? t3 : lacc + t3;
-                if (s == 0) grad[d] = (vp1 - vm1) * hid;
-            }
-            cv[s] = acc; sg[s] = sacc; lap[s] = lacc;
-        }
-        const real g0 = progress<real>(pf[q], rcp_const<real>(D.k, DUCT_RC_HID));
-        const real ctr = (real)k.accel_center;
-        real gmag = sqrt(grad[0] * grad[0] + grad[1] * grad[1] + grad[2] * grad[2]);
-        real art = (real)0;
-        if (g0 > (real)0.1 && g0 < (real)0.99) {
-            real df = g0 - ctr;
-            real win = M<real>::exp(-(df * df) * rcp_const<real>(k, DUCT_RC_IW2));
-            art = (real)5.0 * (nu0 + (real)1.0e-5) * win * lap[0];
-        }
-        const auto stv = run_buf((const real*)P.st + D.off, g.n);
-        real L3 = -cv[0] + stv[q] * gmag + sg[0] + art;
-        const real rf0 = M<real>::div(rut, T0);
-        real mu = table_lookup<real>(P.mu_tab, k, T0);
-        real cp = table_lookup<real>(P.cp_tab, k, T0);
-        real lam = mu * cp * rcp_const<real>(k, DUCT_RC_IPRAN);
-        real al = M<real>::div(lam, rf0 * cp);
-        const real Tu = (real)k.Tu, Tb = (real)k.Tb;
-        real Ttg = Tu + g0 * (Tb - Tu);
-        real L4 = -cv[1] + al * lap[1] + sg[1] + (real)k.relaxation * (Ttg - T0);
-        if (a.Lout[0]) { a.Lout[3][q] = L3; a.Lout[4][q] = L4; }
-        const real DT = (real)ldc(&P.dyn[tp.s].dt);
-        real y3 = qin3[q] + DT * L3, y4 = qin4[q] + DT * L4;
-        real v3, v4;
-        if (a.stage == 0) { v3 = y3; v4 = y4; }
-        else {
-            const auto qa3 = run_arr((const real*)P.A[3] + D.off);
-            const auto qa4 = run_arr((const real*)P.A[4] + D.off);
-            v3 = (real)a.ca * qa3[q] + (real)a.cb * y3;
-            v4 = (real)a.ca * qa4[q] + (real)a.cb * y4;
-        }
-        v4 = fmax((real)200.0, fmin((real)3000.0, v4));
-        qo3[q] = v3;
-        qo4[q] = v4;
-    }
-}
-
-// momentum RHS (:2552-2691 structure) + SSP-RK3 combine; in the last stage also
-// the per-tile max |u| of the new state (its T was written by the scalar kernel)
-template <typename real, int TK>
-// FP64: 4 waves per SIMD (103 VGPRs, the line offsets and the prefetched next
-// line included); FP32: 7 waves (71 VGPRs; 8 would spill).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 4 : 7)))
-k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 4 : 6)))
+k_duct_rhs_all(Pool P, RhsArgs<real> a)
 {
     __shared__ double vred[8];
     const int t = tile_of_block(P, a.xcd_map);
@@ -853,13 +779,9 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
             const int q = bi + j * g.nz + kk;
             void* const* QI = a.qin_is_B ? P.B : P.A;
             void* const* QO = a.qout_is_B ? P.B : P.A;
-            const RunArr<const real> qi[3] = {run_arr((const real*)QI[0] + D.off), run_arr((const real*)QI[1] + D.off),
-                                              run_arr((const real*)QI[2] + D.off)};
-            const RunArr<real> qo[3] = {run_arr((real*)QO[0] + D.off), run_arr((real*)QO[1] + D.off),
-                                        run_arr((real*)QO[2] + D.off)};
             if (i == 0 || i == g.nx - 1) {
 #pragma unroll
-                for (int f = 0; f < 3; ++f) qo[f][q] = qi[f][q];
+                for (int f = 0; f < DNF; ++f) run_arr((real*)QO[f] + D.off)[q] = run_arr((const real*)QI[f] + D.off)[q];
             } else {
                 const auto uf = run_buf((const real*)P.F[0] + D.off, g.n);
                 const auto vf = run_buf((const real*)P.F[1] + D.off, g.n);
@@ -873,15 +795,15 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 const RunBuf<real> vel[3] = {uf, vf, wf};
                 const real T0 = Tf[q];
                 const real rf0 = M<real>::div(rut, T0);
-                // the direction loop stays rolled (one copy of the component
-                // loop's code and registers); the component loop is unrolled
+                const real nu0 = nut[q];
+                const real isc = rcp_const<real>(k, DUCT_RC_ISC), ipr = rcp_const<real>(k, DUCT_RC_IPR);
+                // momentum convection (cv0..cv2); phi and T: SGS flux minus
+                // convection (ap, aT) and Laplacian (lp, lT); |grad phi|^2 (gg)
                 real cv0 = (real)0, cv1 = (real)0, cv2 = (real)0;
+                real ap = (real)0, aT = (real)0, lp = (real)0, lT = (real)0, gg = (real)0;
 #pragma unroll 1
                 for (int d = 0; d < 3; ++d) {
                     const real id = rcp_dir<real>(k, DUCT_RC_ID, d);
-                    // byte offsets of the 7-point line along d, selected by d once
-                    // per line for every field read on it (held in registers, so
-                    // the selection is not repeated before each load)
                     int lo[7];
 #pragma unroll
                     for (int o = 0; o < 7; ++o) {
@@ -890,94 +812,117 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                     }
                     const RunBuf<real> U = d == 0 ? vel[0] : (d == 1 ? vel[1] : vel[2]);
                     const real u0 = U.at(lo[3]);
-                    real up = (real)0.5 * (U.at(lo[4]) + u0);
-                    real um = (real)0.5 * (u0 + U.at(lo[2]));
-                    real rp = (real)0.5 * (M<real>::div(rut, Tf.at(lo[4])) + rf0);
-                    real rm = (real)0.5 * (rf0 + M<real>::div(rut, Tf.at(lo[2])));
-                    real mp = rp * up, mm = rm * um;
+                    const real up = (real)0.5 * (U.at(lo[4]) + u0);
+                    const real um = (real)0.5 * (u0 + U.at(lo[2]));
+                    const real rp = (real)0.5 * (M<real>::div(rut, Tf.at(lo[4])) + rf0);
+                    const real rm = (real)0.5 * (rf0 + M<real>::div(rut, Tf.at(lo[2])));
+                    const real mp = rp * up, mm = rm * um;
+                    real ntp = (real)0, ntm = (real)0;     // nu_t at +-1, loaded with the phi line
                     const bool band = (d == 0 && xband);
-                    // the line of component m + 1 is loaded while component m is
-                    // reconstructed, so its loads do not wait behind this one's
-                    // arithmetic; the first-order band reads only offsets -1..1, so there the
-                    // outer offsets repeat them and every load stays inside the run
+                    // the first-order band reads only offsets -1..1: there the outer
+                    // offsets repeat them, so every load stays inside the run
                     if (band) {
                         lo[0] = lo[1] = lo[2];
                         lo[5] = lo[6] = lo[4];
                     }
+                    // line n + 1 (u, v, w, phi, T in turn) is loaded while line n
+                    // is reconstructed
                     real nxt[7];
 #pragma unroll
                     for (int o = 0; o < 7; ++o) nxt[o] = vel[0].at(lo[o]);
 #pragma unroll
-                    for (int m = 0; m < 3; ++m) {
+                    for (int m = 0; m < 5; ++m) {
                         real v[7];
 #pragma unroll
                         for (int o = 0; o < 7; ++o) v[o] = nxt[o];
-                        if (m < 2) {
-                            const RunBuf<real> Vn = m == 0 ? vel[1] : vel[2];
+                        if (m < 4) {
+                            const RunBuf<real> Vn = m == 0 ? vf : (m == 1 ? wf : (m == 2 ? pf : Tf));
 #pragma unroll
                             for (int o = 0; o < 7; ++o) nxt[o] = Vn.at(lo[o]);
                         }
-                        real fp, fm;
-                        if (band) {
-                            fp = (up >= (real)0.0) ? v[3] : v[4];
-                            fm = (um >= (real)0.0) ? v[2] : v[3];
+                        if (m == 2) { ntp = nut.at(lo[4]); ntm = nut.at(lo[2]); }
+                        if (m < 3) {
+                            real term;
+                            if (band) {
+                                const real fp = (up >= (real)0.0) ? v[3] : v[4];
+                                const real fm = (um >= (real)0.0) ? v[2] : v[3];
+                                term = (mp * fp - mm * fm) * id;
+                            } else {
+                                term = face_pair<real>(scheme, v, up >= (real)0.0, um >= (real)0.0, mp, mm) * id;
+                            }
+                            const real c = m == 0 ? cv0 : (m == 1 ? cv1 : cv2);
+                            const real nc = (d == 0) ? term : c + term;
+                            cv0 = m == 0 ? nc : cv0;
+                            cv1 = m == 1 ? nc : cv1;
+                            cv2 = m == 2 ? nc : cv2;
                         } else {
-                            fp = face_plus<real>(scheme, v, up >= (real)0.0);
-                            fm = face_minus<real>(scheme, v, um >= (real)0.0);
+                            // scalar s = m - 3 (phi, T): upwind by the cell's velocity
+                            const real vm1 = v[2], v0 = v[3], vp1 = v[4];
+                            real tt;
+                            if (band) {
+                                tt = (u0 >= (real)0.0) ? u0 * (v0 - vm1) * id : u0 * (vp1 - v0) * id;
+                            } else {
+                                const bool pos = u0 >= (real)0.0;
+                                tt = u0 * face_pair<real>(scheme, v, pos, pos, (real)1.0, (real)1.0) * id;
+                            }
+                            const real id2 = rcp_dir<real>(k, DUCT_RC_ID2, d);
+                            const real isc2 = m == 3 ? isc : ipr;
+                            const real gp = (ntp + nu0) * isc2, gm = (nu0 + ntm) * isc2;
+                            const real t2 = (gp * (vp1 - v0) - gm * (v0 - vm1)) * id2;
+                            const real t3 = ((vp1 - v0) - (v0 - vm1)) * id2;
+                            if (m == 3) {
+                                ap = (d == 0) ? t2 - tt : ap + (t2 - tt);
+                                lp = (d == 0) ? t3 : lp + t3;
+                                const real gd = (vp1 - vm1) * rcp_dir<real>(k, DUCT_RC_HID, d);
+                                gg = (d == 0) ? gd * gd : gg + gd * gd;
+                            } else {
+                                aT = (d == 0) ? t2 - tt : aT + (t2 - tt);
+                                lT = (d == 0) ? t3 : lT + t3;
+                            }
                         }
-                        real term = (mp * fp - mm * fm) * id;
-                        const real c = m == 0 ? cv0 : (m == 1 ? cv1 : cv2);
-                        const real nc = (d == 0) ? term : c + term;
-                        cv0 = m == 0 ? nc : cv0;
-                        cv1 = m == 1 ? nc : cv1;
-                        cv2 = m == 2 ? nc : cv2;
                     }
                 }
                 const real conv[3] = {cv0, cv1, cv2};
-                real mu = table_lookup<real>(P.mu_tab, k, T0);
-                real nu0 = nut[q];
-                real mu_eff = mu + rf0 * nu0;
+                const real mu = table_lookup<real>(P.mu_tab, k, T0);
+                const real cpv = table_lookup<real>(P.cp_tab, k, T0);
+                const real mu_eff = mu + rf0 * nu0;
                 const int xp = q + (int)g.sx, xm = q - (int)g.sx;
                 const int yp = line_idx(g, bi, j, kk, 1, 1), ym = line_idx(g, bi, j, kk, 1, -1);
                 const int zp = line_idx(g, bi, j, kk, 2, 1), zm = line_idx(g, bi, j, kk, 2, -1);
                 const real ix2 = rcp_const<real>(k, DUCT_RC_ID2), iy2 = rcp_const<real>(k, DUCT_RC_ID2 + 1), iz2 = rcp_const<real>(k, DUCT_RC_ID2 + 2);
-                real diff[3];
                 real df0 = (real)0, df1 = (real)0, df2 = (real)0;
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
                     const RunBuf<real> U = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
-                    real uc = U[q];
-                    // second differences as differences of neighbour differences:
-                    // exact subtractions of close values, so a smooth field's
-                    // small Laplacian keeps its relative precision in FP32
-                    real lx = ((U[xp] - uc) - (uc - U[xm])) * ix2;
-                    real ly = ((U[yp] - uc) - (uc - U[ym])) * iy2;
-                    real lz = ((U[zp] - uc) - (uc - U[zm])) * iz2;
+                    const real uc = U[q];
+                    // second differences as differences of neighbour differences
+                    const real lx = ((U[xp] - uc) - (uc - U[xm])) * ix2;
+                    const real ly = ((U[yp] - uc) - (uc - U[ym])) * iy2;
+                    const real lz = ((U[zp] - uc) - (uc - U[zm])) * iz2;
                     const real dm = mu_eff * (lx + ly + lz);
                     df0 = m == 0 ? dm : df0;
                     df1 = m == 1 ? dm : df1;
                     df2 = m == 2 ? dm : df2;
                 }
-                diff[0] = df0; diff[1] = df1; diff[2] = df2;
                 const real rho_avg = (real)ldc(&P.dyn[tp.s].rho_avg);
-                real grav = (rf0 - rho_avg) * (real)LES_G_GRAV;
-                real g0 = progress<real>(pf[q], rcp_const<real>(D.k, DUCT_RC_HID));
-                real fx = (real)0, fy = (real)0, fz = (real)0;
+                const real grav = (rf0 - rho_avg) * (real)LES_G_GRAV;
+                const real g0 = progress<real>(pf[q], rcp_const<real>(k, DUCT_RC_HID));
                 const real ctr = (real)k.accel_center, Fm = (real)k.max_force;
+                real fx = (real)0, fy = (real)0, fz = (real)0;
                 if (g0 > (real)0.01 && g0 < (real)0.99) {
                     real gx = (real)0, gy = (real)0, gz = (real)0;
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        const int a = d == 0 ? xp : (d == 1 ? yp : zp), b = d == 0 ? xm : (d == 1 ? ym : zm);
-                        const real gd = (progress<real>(pf[a], rcp_const<real>(D.k, DUCT_RC_HID)) -
-                                         progress<real>(pf[b], rcp_const<real>(D.k, DUCT_RC_HID))) * rcp_dir<real>(k, DUCT_RC_HID, d);
+                        const int pa = d == 0 ? xp : (d == 1 ? yp : zp), pb = d == 0 ? xm : (d == 1 ? ym : zm);
+                        const real gd = (progress<real>(pf[pa], rcp_const<real>(k, DUCT_RC_HID)) -
+                                         progress<real>(pf[pb], rcp_const<real>(k, DUCT_RC_HID))) * rcp_dir<real>(k, DUCT_RC_HID, d);
                         gx = d == 0 ? gd : gx;
                         gy = d == 1 ? gd : gy;
                         gz = d == 2 ? gd : gz;
                     }
-                    real df = g0 - ctr;
-                    real win = M<real>::exp(-(df * df) * rcp_const<real>(k, DUCT_RC_IW2));
-                    real common = (real)k.accel_strength * rf0 * win;
+                    const real df = g0 - ctr;
+                    const real win = M<real>::exp(-(df * df) * rcp_const<real>(k, DUCT_RC_IW2));
+                    const real common = (real)k.accel_strength * rf0 * win;
                     fx = common * gx * (real)k.weight_x;
                     fy = common * gy;
                     fz = common * gz;
@@ -985,26 +930,40 @@ k_duct_rhs_momentum(Pool P, RhsArgs<real> a)
                 fx = fmax(-Fm, fmin(Fm, fx));
                 fy = fmax(-Fm, fmin(Fm, fy));
                 fz = fmax(-Fm, fmin(Fm, fz));
-                real Lq[3];
-                Lq[0] = -conv[0] + diff[0] + grav + fx;
-                Lq[1] = -conv[1] + diff[1] + fy;
-                Lq[2] = -conv[2] + diff[2] + fz;
+                real L[DNF];
+                L[0] = -conv[0] + df0 + grav + fx;
+                L[1] = -conv[1] + df1 + fy;
+                L[2] = -conv[2] + df2 + fz;
+                // phi: S_T |grad phi|, SGS flux, artificial diffusion in the flame band
+                real art = (real)0;
+                if (g0 > (real)0.1 && g0 < (real)0.99) {
+                    const real df = g0 - ctr;
+                    const real win = M<real>::exp(-(df * df) * rcp_const<real>(k, DUCT_RC_IW2));
+                    art = (real)5.0 * (nu0 + (real)1.0e-5) * win * lp;
+                }
+                const auto stv = run_buf((const real*)P.st + D.off, g.n);
+                L[3] = ap + stv[q] * sqrt(gg) + art;
+                // T: conduction with alpha = mu cp / Pr / (rho cp), SGS flux, nudging
+                const real lam = mu * cpv * rcp_const<real>(k, DUCT_RC_IPRAN);
+                const real al = M<real>::div(lam, rf0 * cpv);
+                const real Tu = (real)k.Tu, Tb = (real)k.Tb;
+                const real Ttg = Tu + g0 * (Tb - Tu);
+                L[4] = aT + al * lT + (real)k.relaxation * (Ttg - T0);
                 if (a.Lout[0]) {
 #pragma unroll
-                    for (int f = 0; f < 3; ++f) a.Lout[f][q] = Lq[f];
+                    for (int f = 0; f < DNF; ++f) a.Lout[f][q] = L[f];
                 }
                 const real DT = (real)ldc(&P.dyn[tp.s].dt);
-                real out[3];
+                real out[DNF];
 #pragma unroll
-                for (int f = 0; f < 3; ++f) {
-                    real y = qi[f][q] + DT * Lq[f];
-                    if (a.stage == 0) out[f] = y;
-                    else out[f] = (real)a.ca * run_arr((const real*)P.A[f] + D.off)[q] + (real)a.cb * y;
-                    qo[f][q] = out[f];
+                for (int f = 0; f < DNF; ++f) {
+                    const real y = run_arr((const real*)QI[f] + D.off)[q] + DT * L[f];
+                    out[f] = a.stage == 0 ? y : (real)a.ca * run_arr((const real*)P.A[f] + D.off)[q] + (real)a.cb * y;
                 }
-                if (a.track_vel) {
-                    vmax = cell_speed<real>(out[0], out[1], out[2], run_arr((const real*)QO[4] + D.off)[q], k);
-                }
+                out[4] = fmax((real)200.0, fmin((real)3000.0, out[4]));
+#pragma unroll
+                for (int f = 0; f < DNF; ++f) run_arr((real*)QO[f] + D.off)[q] = out[f];
+                if (a.track_vel) vmax = cell_speed<real>(out[0], out[1], out[2], out[4], k);
             }
         }
         if (a.track_vel) {
