@@ -71,7 +71,7 @@ WORKLOADS = {
                                           "halo planes (40x192x192), FP32, with its halo-exchange copies"),
 }
 XGMI_LINK_GBS = 153.0   # one xGMI link, one direction (MI355X: 7 links per GPU)
-RHS_KERNELS = "k_duct_rhs_scalar + k_duct_rhs_momentum (one RK stage's RHS and combine, two launches)"
+RHS_KERNELS = "k_duct_rhs_all (one RK stage's RHS of all five fields fused with the SSP-RK3 combine, one launch)"
 
 
 def parse():
@@ -97,7 +97,7 @@ def log(msg):
 
 
 def valu_from_profiles(workload):
-    """VALU issue of the RHS pair per stage from a rocprofv3 --pmc run of the
+    """VALU issue of the RHS launch per stage from a rocprofv3 --pmc run of the
     same build, where recorded: VALU wave-instructions (SQ_INSTS_VALU), the
     share of SIMD quad-cycles that issued at least one VALU instruction
     ((SQ_ACTIVE_INST_VALU - SQ_ACTIVE_INST_VALU2) over SQ_BUSY_CYCLES / 4 x 32
@@ -112,7 +112,7 @@ def valu_from_profiles(workload):
 
 
 def traffic_from_profiles(workload, xcd_map=None):
-    """HBM bytes per stage of the RHS pair as measured with rocprofv3 --pmc
+    """HBM bytes per stage of the RHS launch as measured with rocprofv3 --pmc
     (DRAM read and write requests in 32-byte units), where recorded, for the
     run's XCD mapping (halo re-reads through L2 misses depend on it)."""
     path = os.path.join(ROOT, "profiles", "rhs_traffic.json")
@@ -254,7 +254,7 @@ def single_line(a, lesgpu, name):
     }
     vi = valu_from_profiles(name)
     if vi:
-        # the pair's VALU issue from its counters: busy share of the SIMDs'
+        # the RHS launch's VALU issue from its counters: busy share of the SIMDs'
         # issue cycles (at most 1 by construction), and the measured VALU
         # instruction count over the live duration
         out["roofline"]["valu_issue"] = {

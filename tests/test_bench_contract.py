@@ -45,7 +45,7 @@ def test_profile_inputs_are_present_and_consistent():
     with open(os.path.join(ROOT, "profiles", "rhs_valu.json")) as f:
         valu = json.load(f)
     for wl in ("duct128-fp64", "duct128-fp32"):
-        # HBM bytes per stage of the RHS pair for each XCD mapping a run may pick
+        # HBM bytes per stage of the RHS launch for each XCD mapping a run may pick
         for xm in (0, 1):
             assert bench.traffic_from_profiles(wl, xm) == traffic[wl][str(xm)] > 0
         v = bench.valu_from_profiles(wl)
