@@ -27,13 +27,14 @@ static int tk_dispatch(int tk, F&& f)
 {
     switch (tk) {
     case 0: return f(std::integral_constant<int, 0>());
+    case 16: return f(std::integral_constant<int, 16>());
     case 32: return f(std::integral_constant<int, 32>());
     case 128: return f(std::integral_constant<int, 128>());
     default: return f(std::integral_constant<int, 64>());
     }
 }
 
-static bool tk_valid(int tk) { return tk == 0 || tk == 32 || tk == 64 || tk == 128; }
+static bool tk_valid(int tk) { return tk == 0 || tk == 16 || tk == 32 || tk == 64 || tk == 128; }
 
 // Tiles of one run: TK > 0 covers TK x (256/TK) cells of an x-plane, flat
 // tiles 256 consecutive cells of an x-plane.
@@ -807,7 +808,7 @@ static int pool_autotune(les_gpu* c, const les_params& probe, bool xcd_only = fa
         *ms_out = ms;
         return LES_OK;
     };
-    int tks[4] = {0, 32, 64, 128};
+    int tks[5] = {0, 16, 32, 64, 128};
     if (xcd_only) tks[0] = d->tk;
     const int zz = d->zigzag ? XCD_REVERSE : 0;
     double best = 1e30;
@@ -816,7 +817,7 @@ static int pool_autotune(les_gpu* c, const les_params& probe, bool xcd_only = fa
     // first candidates of a single pass ran on a GPU not yet at its working
     // clocks and lost to the later ones by more than the candidates differ
     for (int pass = 0; pass < 2; ++pass)
-        for (int a = 0; a < (xcd_only ? 1 : 4); ++a)
+        for (int a = 0; a < (xcd_only ? 1 : 5); ++a)
             for (int x = 0; x < 4; ++x) {
                 float ms = 0.f;
                 const int e = time_candidate(tks[a], x | zz, &ms);
@@ -983,7 +984,7 @@ int duct_create(les_gpu* c)
     cfg.max_vortices = p.num_vortices > 0 ? p.num_vortices : 1;
     cfg.precision = p.precision;
     // the widest tile row that divides nz (no idle lanes), else flat tiles
-    cfg.tile_k = (p.nz % 64 == 0) ? 64 : ((p.nz % 32 == 0) ? 32 : LES_TILE_FLAT);
+    cfg.tile_k = (p.nz % 64 == 0) ? 64 : ((p.nz % 32 == 0) ? 32 : ((p.nz % 16 == 0) ? 16 : LES_TILE_FLAT));
     cfg.xcd_map = 1;        // tuned below
     cfg.point_cap = 4096;
     cfg.probe_cap = p.num_probe_x > 0 ? 64 : 0;

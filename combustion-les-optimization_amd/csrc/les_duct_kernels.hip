@@ -760,11 +760,119 @@ struct RhsArgs {
 // copy of its code and registers) and inside it the seven-point lines of u, v,
 // w, phi and T are loaded one ahead of the reconstruction that uses them.
 // FP64: 4 waves per SIMD (125 VGPRs); FP32: 6 waves (80 VGPRs; 7 spill).
+// Sums of the RHS over the three directions: momentum convection (cv0..cv2);
+// phi and T: SGS flux minus convection (ap, aT) and Laplacian (lp, lT);
+// |grad phi|^2 (gg).
+template <typename real>
+struct RhsSums {
+    real cv0, cv1, cv2, ap, aT, lp, lT, gg;
+};
+
+// One direction's terms of the five RHS from the seven-point lines through the
+// cell.  ld(f, o) is field f (0..2 velocities u, v, w; 3 phi; 4 T; 5 nu_t) at
+// offset o - 3 along the direction; vd is the direction's velocity field
+// (0..2).  Line m + 1 is loaded while line m is reconstructed.  first: the
+// first direction (starts the sums); band: first-order faces (the x band next
+// to the inflow and outflow planes).
+template <typename real, typename LD>
+__device__ __forceinline__ void rhs_line(RhsSums<real>& A, const LD& ld, int vd, bool first, bool band, int scheme,
+                                         real rut, real rf0, real nu0, real isc, real ipr, real id, real id2,
+                                         real hid)
+{
+    const real u0 = ld(vd, 3);
+    const real up = (real)0.5 * (ld(vd, 4) + u0);
+    const real um = (real)0.5 * (u0 + ld(vd, 2));
+    const real rp = (real)0.5 * (M<real>::div(rut, ld(4, 4)) + rf0);
+    const real rm = (real)0.5 * (rf0 + M<real>::div(rut, ld(4, 2)));
+    const real mp = rp * up, mm = rm * um;
+    real ntp = (real)0, ntm = (real)0;     // nu_t at +-1, loaded with the phi line
+    real nxt[7];
+#pragma unroll
+    for (int o = 0; o < 7; ++o) nxt[o] = ld(0, o);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        real v[7];
+#pragma unroll
+        for (int o = 0; o < 7; ++o) v[o] = nxt[o];
+        if (m < 4) {
+#pragma unroll
+            for (int o = 0; o < 7; ++o) nxt[o] = ld(m + 1, o);
+        }
+        if (m == 2) { ntp = ld(5, 4); ntm = ld(5, 2); }
+        if (m < 3) {
+            real term;
+            if (band) {
+                const real fp = (up >= (real)0.0) ? v[3] : v[4];
+                const real fm = (um >= (real)0.0) ? v[2] : v[3];
+                term = (mp * fp - mm * fm) * id;
+            } else {
+                term = face_pair<real>(scheme, v, up >= (real)0.0, um >= (real)0.0, mp, mm) * id;
+            }
+            const real c = m == 0 ? A.cv0 : (m == 1 ? A.cv1 : A.cv2);
+            const real nc = first ? term : c + term;
+            A.cv0 = m == 0 ? nc : A.cv0;
+            A.cv1 = m == 1 ? nc : A.cv1;
+            A.cv2 = m == 2 ? nc : A.cv2;
+        } else {
+            // scalar phi (m = 3) or T (m = 4): upwind by the cell's velocity
+            const real vm1 = v[2], v0 = v[3], vp1 = v[4];
+            real tt;
+            if (band) {
+                tt = (u0 >= (real)0.0) ? u0 * (v0 - vm1) * id : u0 * (vp1 - v0) * id;
+            } else {
+                const bool pos = u0 >= (real)0.0;
+                tt = u0 * face_pair<real>(scheme, v, pos, pos, (real)1.0, (real)1.0) * id;
+            }
+            const real isc2 = m == 3 ? isc : ipr;
+            const real gp = (ntp + nu0) * isc2, gm = (nu0 + ntm) * isc2;
+            const real t2 = (gp * (vp1 - v0) - gm * (v0 - vm1)) * id2;
+            const real t3 = ((vp1 - v0) - (v0 - vm1)) * id2;
+            if (m == 3) {
+                A.ap = first ? t2 - tt : A.ap + (t2 - tt);
+                A.lp = first ? t3 : A.lp + t3;
+                const real gd = (vp1 - vm1) * hid;
+                A.gg = first ? gd * gd : A.gg + gd * gd;
+            } else {
+                A.aT = first ? t2 - tt : A.aT + (t2 - tt);
+                A.lT = first ? t3 : A.lT + t3;
+            }
+        }
+    }
+}
+
+// Row tiles of 64, 32 or 16 cells in k stage the y/z neighbourhood of the tile
+// (the tile plus 3 cells each side in j and k, wrapped) of the filtered u, v,
+// w, phi, T and nu_t in LDS: the y and z lines come from there, the x lines
+// from memory.
+template <int TK>
+struct RhsStage {
+    static constexpr bool on = (TK == 16 || TK == 32 || TK == 64);
+    static constexpr int TJ = on ? 256 / TK : 1, SJ = TJ + 6, SK = on ? TK + 6 : 1, SN = on ? SJ * SK : 1;
+};
+
+// The RHS of all five fields of one RK stage fused with the SSP-RK3 combine,
+// one launch:
+//   rho*u, rho*v, rho*w (:2552-2691 structure): conservative WENO5/minmod face
+//     fluxes with face-averaged rho and velocity, mu_eff Laplacian, buoyancy
+//     (rho - rho_avg) g, flame force in the progress band;
+//   phi (G equation, :2706-2818) and T (:2912-2955): upwind convection by the
+//     cell's velocity, SGS fluxes with face-averaged nu_t, S_T |grad phi|,
+//     artificial diffusion in the flame band, conduction, nudging to the
+//     G-weighted temperature;
+// and, in the last stage, the per-tile max |u| of the new state.  Per cell the
+// quantities the equations share are formed once: each direction's line
+// offsets, the cell's velocity along it, the neighbours' T, rho and nu_t, the
+// progress variable, the property tables.  Flat tiles and 128-wide rows walk
+// the directions in a rolled loop over memory; 64-, 32- and 16-wide rows take
+// their y and z lines from the LDS-staged neighbourhood (RhsStage).
+// FP64: 4 waves per SIMD (125 VGPRs); FP32: 6 waves (80 VGPRs; 7 spill).
 template <typename real, int TK>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 4 : 6)))
 k_duct_rhs_all(Pool P, RhsArgs<real> a)
 {
+    using ST = RhsStage<TK>;
     __shared__ double vred[8];
+    __shared__ real sF[ST::on ? 6 : 1][ST::SN];
     const int t = tile_of_block(P, a.xcd_map);
     TilePos tp;
     if (t < 0 || !tile_decode<TK>(P, t, tp)) return;
@@ -772,6 +880,30 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
         const SlotView D = slot_view(P, tp.s);
         const GridDims g = D.g;
         const DuctConsts& k = D.k;
+        const bool endplane = (tp.i == 0 || tp.i == g.nx - 1);      // uniform over the tile
+        if constexpr (ST::on) {
+            if (!endplane) {
+                // the block's tile origin, then its neighbourhood: SJ x SK cells of
+                // each of the six fields, rows and columns wrapped (a row or column
+                // beyond one wrap only occurs past the grid's edge, for cells outside
+                // the run, and reads cell 0)
+                const int j0 = tp.j - (int)threadIdx.x / TK, k0 = tp.kk - (int)threadIdx.x % TK;
+                const int bi = tp.i * (int)g.sx;
+                const auto b0 = run_buf((const real*)P.F[0] + D.off, g.n), b1 = run_buf((const real*)P.F[1] + D.off, g.n);
+                const auto b2 = run_buf((const real*)P.F[2] + D.off, g.n), b3 = run_buf((const real*)P.F[4] + D.off, g.n);
+                const auto b4 = run_buf((const real*)P.F[3] + D.off, g.n), b5 = run_buf((const real*)P.nut + D.off, g.n);
+                for (int r = threadIdx.x; r < ST::SN; r += 256) {
+                    const int jr = r / ST::SK, kr = r - jr * ST::SK;
+                    int gj = wrapi(j0 + jr - 3, g.ny), gk = wrapi(k0 + kr - 3, g.nz);
+                    gj = (unsigned)gj < (unsigned)g.ny ? gj : 0;
+                    gk = (unsigned)gk < (unsigned)g.nz ? gk : 0;
+                    const int c = bi + gj * g.nz + gk;
+                    sF[0][r] = b0[c]; sF[1][r] = b1[c]; sF[2][r] = b2[c];
+                    sF[3][r] = b3[c]; sF[4][r] = b4[c]; sF[5][r] = b5[c];
+                }
+                __syncthreads();
+            }
+        }
         double vmax = 0.0;
         if (tp.in) {
             const int i = tp.i, j = tp.j, kk = tp.kk;
@@ -779,7 +911,7 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
             const int q = bi + j * g.nz + kk;
             void* const* QI = a.qin_is_B ? P.B : P.A;
             void* const* QO = a.qout_is_B ? P.B : P.A;
-            if (i == 0 || i == g.nx - 1) {
+            if (endplane) {
 #pragma unroll
                 for (int f = 0; f < DNF; ++f) run_arr((real*)QO[f] + D.off)[q] = run_arr((const real*)QI[f] + D.off)[q];
             } else {
@@ -793,95 +925,80 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
                 const bool xband = (i < 3 || i > g.nx - 4);
                 const int scheme = k.scheme;
                 const RunBuf<real> vel[3] = {uf, vf, wf};
+                const RunBuf<real> fld[6] = {uf, vf, wf, pf, Tf, nut};
                 const real T0 = Tf[q];
                 const real rf0 = M<real>::div(rut, T0);
                 const real nu0 = nut[q];
                 const real isc = rcp_const<real>(k, DUCT_RC_ISC), ipr = rcp_const<real>(k, DUCT_RC_IPR);
-                // momentum convection (cv0..cv2); phi and T: SGS flux minus
-                // convection (ap, aT) and Laplacian (lp, lT); |grad phi|^2 (gg)
-                real cv0 = (real)0, cv1 = (real)0, cv2 = (real)0;
-                real ap = (real)0, aT = (real)0, lp = (real)0, lT = (real)0, gg = (real)0;
-#pragma unroll 1
-                for (int d = 0; d < 3; ++d) {
-                    const real id = rcp_dir<real>(k, DUCT_RC_ID, d);
+                RhsSums<real> S;
+                S.cv0 = S.cv1 = S.cv2 = S.ap = S.aT = S.lp = S.lT = S.gg = (real)0;
+                // the cell in the staged neighbourhood (row tiles of 64, 32 or 16)
+                const int lb = ST::on ? ((int)threadIdx.x / TK + 3) * ST::SK + (int)threadIdx.x % TK + 3 : 0;
+                if constexpr (ST::on) {
+                    // x lines from memory (the band next to the end planes reads
+                    // offsets -1..1 only: there the outer offsets repeat them)
                     int lo[7];
 #pragma unroll
-                    for (int o = 0; o < 7; ++o) {
-                        lo[o] = line_idx(g, bi, j, kk, d, o - 3) * (int)sizeof(real);
-                        asm volatile("" : "+v"(lo[o]));
-                    }
-                    const RunBuf<real> U = d == 0 ? vel[0] : (d == 1 ? vel[1] : vel[2]);
-                    const real u0 = U.at(lo[3]);
-                    const real up = (real)0.5 * (U.at(lo[4]) + u0);
-                    const real um = (real)0.5 * (u0 + U.at(lo[2]));
-                    const real rp = (real)0.5 * (M<real>::div(rut, Tf.at(lo[4])) + rf0);
-                    const real rm = (real)0.5 * (rf0 + M<real>::div(rut, Tf.at(lo[2])));
-                    const real mp = rp * up, mm = rm * um;
-                    real ntp = (real)0, ntm = (real)0;     // nu_t at +-1, loaded with the phi line
-                    const bool band = (d == 0 && xband);
-                    // the first-order band reads only offsets -1..1: there the outer
-                    // offsets repeat them, so every load stays inside the run
-                    if (band) {
+                    for (int o = 0; o < 7; ++o) lo[o] = (q + (o - 3) * (int)g.sx) * (int)sizeof(real);
+                    if (xband) {
                         lo[0] = lo[1] = lo[2];
                         lo[5] = lo[6] = lo[4];
                     }
-                    // line n + 1 (u, v, w, phi, T in turn) is loaded while line n
-                    // is reconstructed
-                    real nxt[7];
+                    auto ldx = [&](int f, int o) {
+                        return (f == 0 ? fld[0] : (f == 1 ? fld[1] : (f == 2 ? fld[2] : (f == 3 ? fld[3] : (f == 4 ? fld[4] : fld[5]))))).at(lo[o]);
+                    };
+                    rhs_line<real>(S, ldx, 0, true, xband, scheme, rut, rf0, nu0, isc, ipr,
+                                   rcp_const<real>(k, DUCT_RC_ID), rcp_const<real>(k, DUCT_RC_ID2),
+                                   rcp_const<real>(k, DUCT_RC_HID));
+                    // y and z lines from LDS
+#ifdef EXP_LDS_UNROLLED
+                    auto ldy = [&](int f, int o) { return sF[f][lb + (o - 3) * ST::SK]; };
+                    rhs_line<real>(S, ldy, 1, false, false, scheme, rut, rf0, nu0, isc, ipr,
+                                   rcp_const<real>(k, DUCT_RC_ID + 1), rcp_const<real>(k, DUCT_RC_ID2 + 1),
+                                   rcp_const<real>(k, DUCT_RC_HID + 1));
+                    __builtin_amdgcn_sched_barrier(0);
+                    auto ldz = [&](int f, int o) { return sF[f][lb + (o - 3)]; };
+                    rhs_line<real>(S, ldz, 2, false, false, scheme, rut, rf0, nu0, isc, ipr,
+                                   rcp_const<real>(k, DUCT_RC_ID + 2), rcp_const<real>(k, DUCT_RC_ID2 + 2),
+                                   rcp_const<real>(k, DUCT_RC_HID + 2));
+#else
+#pragma unroll 1
+                    for (int d = 1; d < 3; ++d) {
+                        const int st = d == 1 ? ST::SK : 1;
+                        const int l0 = lb - 3 * st;
+                        auto ldl = [&](int f, int o) { return sF[f == 6 ? d : f][l0 + o * st]; };
+                        rhs_line<real>(S, ldl, 6, false, false, scheme, rut, rf0, nu0, isc, ipr,
+                                       rcp_dir<real>(k, DUCT_RC_ID, d), rcp_dir<real>(k, DUCT_RC_ID2, d),
+                                       rcp_dir<real>(k, DUCT_RC_HID, d));
+                    }
+#endif
+                } else {
+                    // the direction loop stays rolled (one copy of its code and
+                    // registers); each direction's line offsets are selected once
+#pragma unroll 1
+                    for (int d = 0; d < 3; ++d) {
+                        int lo[7];
 #pragma unroll
-                    for (int o = 0; o < 7; ++o) nxt[o] = vel[0].at(lo[o]);
-#pragma unroll
-                    for (int m = 0; m < 5; ++m) {
-                        real v[7];
-#pragma unroll
-                        for (int o = 0; o < 7; ++o) v[o] = nxt[o];
-                        if (m < 4) {
-                            const RunBuf<real> Vn = m == 0 ? vf : (m == 1 ? wf : (m == 2 ? pf : Tf));
-#pragma unroll
-                            for (int o = 0; o < 7; ++o) nxt[o] = Vn.at(lo[o]);
+                        for (int o = 0; o < 7; ++o) {
+                            lo[o] = line_idx(g, bi, j, kk, d, o - 3) * (int)sizeof(real);
+                            asm volatile("" : "+v"(lo[o]));
                         }
-                        if (m == 2) { ntp = nut.at(lo[4]); ntm = nut.at(lo[2]); }
-                        if (m < 3) {
-                            real term;
-                            if (band) {
-                                const real fp = (up >= (real)0.0) ? v[3] : v[4];
-                                const real fm = (um >= (real)0.0) ? v[2] : v[3];
-                                term = (mp * fp - mm * fm) * id;
-                            } else {
-                                term = face_pair<real>(scheme, v, up >= (real)0.0, um >= (real)0.0, mp, mm) * id;
-                            }
-                            const real c = m == 0 ? cv0 : (m == 1 ? cv1 : cv2);
-                            const real nc = (d == 0) ? term : c + term;
-                            cv0 = m == 0 ? nc : cv0;
-                            cv1 = m == 1 ? nc : cv1;
-                            cv2 = m == 2 ? nc : cv2;
-                        } else {
-                            // scalar s = m - 3 (phi, T): upwind by the cell's velocity
-                            const real vm1 = v[2], v0 = v[3], vp1 = v[4];
-                            real tt;
-                            if (band) {
-                                tt = (u0 >= (real)0.0) ? u0 * (v0 - vm1) * id : u0 * (vp1 - v0) * id;
-                            } else {
-                                const bool pos = u0 >= (real)0.0;
-                                tt = u0 * face_pair<real>(scheme, v, pos, pos, (real)1.0, (real)1.0) * id;
-                            }
-                            const real id2 = rcp_dir<real>(k, DUCT_RC_ID2, d);
-                            const real isc2 = m == 3 ? isc : ipr;
-                            const real gp = (ntp + nu0) * isc2, gm = (nu0 + ntm) * isc2;
-                            const real t2 = (gp * (vp1 - v0) - gm * (v0 - vm1)) * id2;
-                            const real t3 = ((vp1 - v0) - (v0 - vm1)) * id2;
-                            if (m == 3) {
-                                ap = (d == 0) ? t2 - tt : ap + (t2 - tt);
-                                lp = (d == 0) ? t3 : lp + t3;
-                                const real gd = (vp1 - vm1) * rcp_dir<real>(k, DUCT_RC_HID, d);
-                                gg = (d == 0) ? gd * gd : gg + gd * gd;
-                            } else {
-                                aT = (d == 0) ? t2 - tt : aT + (t2 - tt);
-                                lT = (d == 0) ? t3 : lT + t3;
-                            }
+                        const bool band = (d == 0 && xband);
+                        if (band) {
+                            lo[0] = lo[1] = lo[2];
+                            lo[5] = lo[6] = lo[4];
                         }
+                        const RunBuf<real> U = d == 0 ? vel[0] : (d == 1 ? vel[1] : vel[2]);
+                        auto ld = [&](int f, int o) {
+                            // f == vd (the direction's velocity) is read through U
+                            return (f == 6 ? U : (f == 0 ? fld[0] : (f == 1 ? fld[1] : (f == 2 ? fld[2] : (f == 3 ? fld[3] : (f == 4 ? fld[4] : fld[5])))))).at(lo[o]);
+                        };
+                        rhs_line<real>(S, ld, 6, d == 0, band, scheme, rut, rf0, nu0, isc, ipr,
+                                       rcp_dir<real>(k, DUCT_RC_ID, d), rcp_dir<real>(k, DUCT_RC_ID2, d),
+                                       rcp_dir<real>(k, DUCT_RC_HID, d));
                     }
                 }
+                const real cv0 = S.cv0, cv1 = S.cv1, cv2 = S.cv2, ap = S.ap, aT = S.aT, lp = S.lp, lT = S.lT, gg = S.gg;
                 const real conv[3] = {cv0, cv1, cv2};
                 const real mu = table_lookup<real>(P.mu_tab, k, T0);
                 const real cpv = table_lookup<real>(P.cp_tab, k, T0);
@@ -891,14 +1008,26 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
                 const int zp = line_idx(g, bi, j, kk, 2, 1), zm = line_idx(g, bi, j, kk, 2, -1);
                 const real ix2 = rcp_const<real>(k, DUCT_RC_ID2), iy2 = rcp_const<real>(k, DUCT_RC_ID2 + 1), iz2 = rcp_const<real>(k, DUCT_RC_ID2 + 2);
                 real df0 = (real)0, df1 = (real)0, df2 = (real)0;
+                // field f (0..2 velocities, 3 phi) at the cell (n = 0) or at its
+                // +-y (n = 1, 2) and +-z (n = 3, 4) neighbours: from the staged
+                // neighbourhood where there is one
+                auto yz = [&](int f, int n) -> real {
+                    if constexpr (ST::on) {
+                        const int o = n == 0 ? 0 : (n == 1 ? ST::SK : (n == 2 ? -ST::SK : (n == 3 ? 1 : -1)));
+                        return sF[f][lb + o];
+                    } else {
+                        const int c = n == 0 ? q : (n == 1 ? yp : (n == 2 ? ym : (n == 3 ? zp : zm)));
+                        return fld[f][c];
+                    }
+                };
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
                     const RunBuf<real> U = m == 0 ? vel[0] : (m == 1 ? vel[1] : vel[2]);
-                    const real uc = U[q];
+                    const real uc = yz(m, 0);
                     // second differences as differences of neighbour differences
                     const real lx = ((U[xp] - uc) - (uc - U[xm])) * ix2;
-                    const real ly = ((U[yp] - uc) - (uc - U[ym])) * iy2;
-                    const real lz = ((U[zp] - uc) - (uc - U[zm])) * iz2;
+                    const real ly = ((yz(m, 1) - uc) - (uc - yz(m, 2))) * iy2;
+                    const real lz = ((yz(m, 3) - uc) - (uc - yz(m, 4))) * iz2;
                     const real dm = mu_eff * (lx + ly + lz);
                     df0 = m == 0 ? dm : df0;
                     df1 = m == 1 ? dm : df1;
@@ -906,16 +1035,16 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
                 }
                 const real rho_avg = (real)ldc(&P.dyn[tp.s].rho_avg);
                 const real grav = (rf0 - rho_avg) * (real)LES_G_GRAV;
-                const real g0 = progress<real>(pf[q], rcp_const<real>(k, DUCT_RC_HID));
+                const real g0 = progress<real>(yz(3, 0), rcp_const<real>(k, DUCT_RC_HID));
                 const real ctr = (real)k.accel_center, Fm = (real)k.max_force;
                 real fx = (real)0, fy = (real)0, fz = (real)0;
                 if (g0 > (real)0.01 && g0 < (real)0.99) {
                     real gx = (real)0, gy = (real)0, gz = (real)0;
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        const int pa = d == 0 ? xp : (d == 1 ? yp : zp), pb = d == 0 ? xm : (d == 1 ? ym : zm);
-                        const real gd = (progress<real>(pf[pa], rcp_const<real>(k, DUCT_RC_HID)) -
-                                         progress<real>(pf[pb], rcp_const<real>(k, DUCT_RC_HID))) * rcp_dir<real>(k, DUCT_RC_HID, d);
+                        const real pa = d == 0 ? pf[xp] : yz(3, 2 * d - 1), pb = d == 0 ? pf[xm] : yz(3, 2 * d);
+                        const real gd = (progress<real>(pa, rcp_const<real>(k, DUCT_RC_HID)) -
+                                         progress<real>(pb, rcp_const<real>(k, DUCT_RC_HID))) * rcp_dir<real>(k, DUCT_RC_HID, d);
                         gx = d == 0 ? gd : gx;
                         gy = d == 1 ? gd : gy;
                         gz = d == 2 ? gd : gz;
@@ -981,32 +1110,47 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024) k_duct_dt(Pool P)
 {
-    __shared__ double r[3][1024];
+    __shared__ double r[3][16];
     const int s = blockIdx.x;
     const SlotDesc& D = P.desc[s];
     if (!D.active || P.dyn[s].done) return;
     const DuctConsts& k = D.k;
+    // maxima are exact in any order: four tiles per thread and pass (their
+    // loads in flight together), then the waves' butterflies and 16 wave maxima
     double mv = 0.0, mn = 0.0, ma = 0.0;
-    for (int b = threadIdx.x; b < D.ntiles; b += 1024) {
-        const int t = D.tile0 + b;
-        mv = fmax(mv, P.vpart[t]);
-        for (int stg = 0; stg < 3; ++stg) {
-            const double* dp = P.dpart + 2 * ((long long)stg * P.tile_cap + t);
-            mn = fmax(mn, dp[0]);
-            ma = fmax(ma, dp[1]);
+    const int nt = D.ntiles;
+    for (int b0 = 0; b0 < nt; b0 += 4096) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int b = b0 + u * 1024 + (int)threadIdx.x;
+            if (b < nt) {
+                const int t = D.tile0 + b;
+                mv = fmax(mv, P.vpart[t]);
+#pragma unroll
+                for (int stg = 0; stg < 3; ++stg) {
+                    const double* dp = P.dpart + 2 * ((long long)stg * P.tile_cap + t);
+                    mn = fmax(mn, dp[0]);
+                    ma = fmax(ma, dp[1]);
+                }
+            }
         }
     }
     const int nbc = P.ny_max * P.nzb;
     for (int b = threadIdx.x; b < nbc; b += 1024) mv = fmax(mv, P.bcpart[(long long)s * nbc + b]);
-    r[0][threadIdx.x] = mv; r[1][threadIdx.x] = mn; r[2][threadIdx.x] = ma;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            for (int c = 0; c < 3; ++c) r[c][threadIdx.x] = fmax(r[c][threadIdx.x], r[c][threadIdx.x + w]);
-        __syncthreads();
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mv = fmax(mv, __shfl_xor(mv, o));
+        mn = fmax(mn, __shfl_xor(mn, o));
+        ma = fmax(ma, __shfl_xor(ma, o));
     }
+    if ((threadIdx.x & 63) == 0) {
+        r[0][threadIdx.x >> 6] = mv; r[1][threadIdx.x >> 6] = mn; r[2][threadIdx.x >> 6] = ma;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
-    mv = r[0][0]; mn = r[1][0]; ma = r[2][0];
+    for (int w = 1; w < 16; ++w) {
+        mv = fmax(mv, r[0][w]); mn = fmax(mn, r[1][w]); ma = fmax(ma, r[2][w]);
+    }
     const double dx = D.g.dx;
     double maxvel = mv > LES_SMALL ? mv : k.U;
     double dtc = k.courant * dx / (maxvel + LES_SMALL);

@@ -313,7 +313,7 @@ typedef struct les_batch_config {
     long long pool_cells;      /* cells of the shared arenas (sum over resident runs) */
     int       max_nx, max_ny, max_nz, max_vortices;   /* per-run limits */
     int       precision;       /* LES_PREC_*: one storage type per pool */
-    int       tile_k;          /* tile width in k: LES_TILE_FLAT, 32, 64 or 128 */
+    int       tile_k;          /* tile width in k: LES_TILE_FLAT, 16, 32, 64 or 128 */
     int       xcd_map;         /* tiles to XCDs: 0 round-robin, 1 one contiguous range per XCD,
                                   2 / 3 chunks of 16 / 64 tiles round-robin; + 4: the stencil
                                   launches alternate the order in which each XCD walks its tiles */

@@ -88,8 +88,8 @@ def test_runs_admitted_later_match_alone(lesgpu):
             assert np.array_equal(x, y)
 
 
-@pytest.mark.parametrize("tile_k,xcd", [(0, 1), (32, 1), (128, 0), (64, 0)],
-                         ids=["flat_xcd", "tk32_xcd", "tk128", "tk64"])
+@pytest.mark.parametrize("tile_k,xcd", [(0, 1), (32, 1), (128, 0), (64, 0), (16, 2)],
+                         ids=["flat_xcd", "tk32_xcd", "tk128", "tk64", "tk16_chunks"])
 def test_launch_shapes_vs_oracle(lesgpu, oracle, tile_k, xcd):
     print("case: batch tile_k=%d xcd=%d vs oracle" % (tile_k, xcd), flush=True)
     over = dict(reinit_interval=3, reinit_iters=2)
@@ -147,7 +147,7 @@ def test_batch_point_series_and_autotune(lesgpu):
     with lesgpu.Batch(cfg, probe) as b:
         t = b.tuning()
         print("tuned:", t, flush=True)
-        assert t["tile_k"] in (0, 32, 64, 128) and t["grid_blocks"] > 0 and t["tuned_ms"] > 0
+        assert t["tile_k"] in (0, 16, 32, 64, 128) and t["grid_blocks"] > 0 and t["tuned_ms"] > 0
         assert t["xcd_map"] in range(8)                          # mapping 0..3, +4 alternated sweeps
         s = b.admit(probe, 12)
         b.step(12)
