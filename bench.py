@@ -89,6 +89,8 @@ def parse():
     ap.add_argument("--requests", type=int, default=96, help="served stream: number of requests")
     ap.add_argument("--xcd-map", type=int, choices=[0, 1, 2, 3], default=None,
                     help="single runs: fix the XCD mapping (LES_XCD_MAP) instead of the creation-time trial")
+    ap.add_argument("--tile-k", type=int, choices=[0, 16, 32, 64, 128], default=None,
+                    help="single runs: fix the tile width (LES_TILE_K) instead of the creation-time trial")
     return ap.parse_args()
 
 
@@ -111,17 +113,18 @@ def valu_from_profiles(workload):
     return rec.get(workload)
 
 
-def traffic_from_profiles(workload, xcd_map=None):
+def traffic_from_profiles(workload, xcd_map=None, tile_k=None):
     """HBM bytes per stage of the RHS launch as measured with rocprofv3 --pmc
     (DRAM read and write requests in 32-byte units), where recorded, for the
-    run's XCD mapping (halo re-reads through L2 misses depend on it)."""
+    run's tile width and XCD mapping (halo re-reads through L2 misses depend
+    on both); keys "tile_k:xcd_map"."""
     path = os.path.join(ROOT, "profiles", "rhs_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         rec = json.load(f).get(workload)
     if isinstance(rec, dict):
-        return rec.get(str(xcd_map))
+        return rec.get("%s:%s" % (tile_k, xcd_map))
     return rec
 
 
@@ -246,7 +249,8 @@ def single_line(a, lesgpu, name):
                    "launch": r["launch"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": traffic_from_profiles(name, (r["launch"] or {}).get("xcd_map")),
+                     "traffic": traffic_from_profiles(name, (r["launch"] or {}).get("xcd_map"),
+                                                      (r["launch"] or {}).get("tile_k")),
                      "kernel": RHS_KERNELS if duct else "k_ref_rhs (one RK4 stage's RHS)",
                      "kernel_ms_per_stage": k_ms, "launches_timed": r["launches"],
                      "bytes_per_cell_stage": B, "cells": cells,
@@ -514,6 +518,8 @@ def main():
         sys.exit(2)
     if a.xcd_map is not None:
         os.environ["LES_XCD_MAP"] = str(a.xcd_map)
+    if a.tile_k is not None:
+        os.environ["LES_TILE_K"] = str(a.tile_k)
     lesgpu = importlib.import_module("combustion-les-optimization_amd.lesgpu")
     serving = importlib.import_module("combustion-les-optimization_amd.serving")
     if a.workload == "ensemble64-fp32":

@@ -235,10 +235,10 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
     d->xcd_map = cfg.xcd_map & 3;
     d->zigzag = (cfg.xcd_map & XCD_REVERSE) ? 1 : 0;
     // tiles of all resident runs, for any tile width: per run at most
-    // n/256 + nx*ny/2 + nx*nz/32 + nx (ceil bounds of each tile shape)
+    // n/256 + nx*ny/2 + nx*nz/16 + nx (ceil bounds of each tile shape)
     {
         long long cap = d->pool_cells / 256 +
-                        (long long)d->max_slots * d->nx_max * (d->ny_max / 2 + d->nz_max / 32 + 2) + 64;
+                        (long long)d->max_slots * d->nx_max * (d->ny_max / 2 + d->nz_max / 16 + 2) + 64;
         if (cap > (1LL << 30)) return LES_ERR_PARAM;
         d->tile_cap = (int)cap;
     }
@@ -741,11 +741,13 @@ static int slot_monitor(les_gpu* c, int s, les_step_report* rep)
 // autotuner: times each (tile width, XCD mapping) candidate on a scratch pool
 // filled with replicas of the probe run (as many as the pool's slots and cells
 // hold, so the timed launches have the pool's occupancy) and keeps the fastest,
-// then times the winner with and without alternating sweep directions.
-// xcd_only: the pool's tile width stays and only the XCD mapping is tuned (it
-// does not change results: per-tile partial sums are reduced in tile order).
+// then (pools) times the winner with and without alternating sweep
+// directions.  tune_tile / tune_xcd: whether the tile width / the XCD mapping
+// is tuned (the pool's own is kept otherwise; the XCD mapping does not change
+// results, the tile width only groups the per-tile partial sums of rho);
+// sweeps: whether alternating sweeps are tried.
 // ---------------------------------------------------------------------------
-static int pool_autotune(les_gpu* c, const les_params& probe, bool xcd_only = false)
+static int pool_autotune(les_gpu* c, const les_params& probe, bool tune_tile, bool tune_xcd, bool sweeps)
 {
     DuctState* d = c->d;
     // one candidate's fastest of three 5-step samples (the candidates differ
@@ -809,7 +811,7 @@ static int pool_autotune(les_gpu* c, const les_params& probe, bool xcd_only = fa
         return LES_OK;
     };
     int tks[5] = {0, 16, 32, 64, 128};
-    if (xcd_only) tks[0] = d->tk;
+    if (!tune_tile) tks[0] = d->tk;
     const int zz = d->zigzag ? XCD_REVERSE : 0;
     double best = 1e30;
     int btk = d->tk, bx = d->xcd_map;
@@ -817,19 +819,20 @@ static int pool_autotune(les_gpu* c, const les_params& probe, bool xcd_only = fa
     // first candidates of a single pass ran on a GPU not yet at its working
     // clocks and lost to the later ones by more than the candidates differ
     for (int pass = 0; pass < 2; ++pass)
-        for (int a = 0; a < (xcd_only ? 1 : 5); ++a)
-            for (int x = 0; x < 4; ++x) {
+        for (int a = 0; a < (tune_tile ? 5 : 1); ++a)
+            for (int x = 0; x < (tune_xcd ? 4 : 1); ++x) {
+                const int xm = tune_xcd ? x : d->xcd_map;
                 float ms = 0.f;
-                const int e = time_candidate(tks[a], x | zz, &ms);
+                const int e = time_candidate(tks[a], xm | zz, &ms);
                 if (e != LES_OK) return e;
                 if (ms > 0 && ms < best) {
                     best = ms;
                     btk = tks[a];
-                    bx = x;
+                    bx = xm;
                 }
             }
     int bz = d->zigzag;
-    if (!xcd_only && best < 1e29) {
+    if (sweeps && best < 1e29) {
         // the winner with and without alternating sweep directions, each
         // timed twice in turn
         double bs[2] = {1e30, 1e30};
@@ -992,23 +995,29 @@ int duct_create(les_gpu* c)
     cfg.table_t0 = p.table_t0; cfg.table_dt = p.table_dt; cfg.table_n = p.table_n;
     int e = pool_create(c, cfg);
     if (e != LES_OK) return e;
+    // which XCD mapping is faster depends on the grid and precision (FP64 at
+    // 128^3 prefers round-robin tiles, smaller grids and FP32 contiguous
+    // ranges per XCD); results do not depend on it, so each run times the four
+    // before it is admitted (a failed trial, e.g. no memory for the scratch
+    // pool, keeps the default), unless LES_XCD_MAP (0..3) fixes it.  The tile
+    // width is not timed: it groups the per-tile partial sums of rho, so a
+    // timed choice would make a run's rounding depend on the trial; it is the
+    // widest row that divides nz, unless LES_TILE_K (0, 16, 32, 64, 128) fixes it
+    const char* fx = getenv("LES_XCD_MAP");
+    const char* ft = getenv("LES_TILE_K");
+    const bool fixed_x = fx && fx[0] >= '0' && fx[0] <= '3' && fx[1] == 0;
+    if (ft && ft[0] && tk_valid(atoi(ft))) c->d->tk = atoi(ft);
+    if (fixed_x) {
+        c->d->xcd_map = fx[0] - '0';
+    } else if (pool_autotune(c, p, false, true, false) != LES_OK) {
+        c->d->xcd_map = 1;
+        c->err[0] = 0;
+    }
     int slot = -1;
     e = pool_admit(c, p, 0, &slot);
     if (e != LES_OK) {
         snprintf(c->err, sizeof(c->err), "duct run does not fit its own pool (status %d)", e);
         return e;
-    }
-    // which XCD mapping is faster depends on the grid and precision (FP64 at
-    // 128^3 prefers round-robin tiles, smaller grids and FP32 contiguous
-    // ranges per XCD); results do not depend on it, so each run tunes it (a
-    // failed trial, e.g. no memory for the scratch pool, keeps the default),
-    // unless LES_XCD_MAP (0..3) fixes it
-    const char* fixed = getenv("LES_XCD_MAP");
-    if (fixed && fixed[0] >= '0' && fixed[0] <= '3' && fixed[1] == 0) {
-        c->d->xcd_map = fixed[0] - '0';
-    } else if (pool_autotune(c, p, true) != LES_OK) {
-        c->d->xcd_map = 1;
-        c->err[0] = 0;
     }
     pool_set_grid(c->d);
     c->attr_h.assign((size_t)c->g.n, 3);
@@ -1259,7 +1268,7 @@ extern "C" int les_batch_pool_bytes(const les_batch_config* cfg, size_t* bytes)
         if (batch_config_check(cfg) != LES_OK || !bytes) return LES_ERR_PARAM;
         const size_t es = cfg->precision == LES_PREC_FP32 ? 4 : 8;
         const size_t n = (size_t)cfg->pool_cells, S = (size_t)cfg->max_slots;
-        const size_t tiles = n / 256 + S * cfg->max_nx * (cfg->max_ny / 2 + cfg->max_nz / 32 + 2) + 64;
+        const size_t tiles = n / 256 + S * cfg->max_nx * (cfg->max_ny / 2 + cfg->max_nz / 16 + 2) + 64;
         size_t b = 17 * n * es;                                   // A, B, F (5 each), nu_t, S_T
         if (cfg->stats_per_cell) b += 80 * n;
         if (cfg->precision == LES_PREC_FP32) b += 8 * n;           // FP64 staging of one run
@@ -1299,7 +1308,7 @@ extern "C" int les_batch_create(const les_batch_config* cfg, const les_params* p
         for (int a = 0; a < 4 && rc == LES_OK; ++a)
             if (hipEventCreate(&c->ev[a]) != hipSuccess) rc = LES_ERR_HIP;
         if (rc == LES_OK) rc = pool_create(c, *cfg);
-        if (rc == LES_OK && cfg->autotune) rc = pool_autotune(c, *probe);
+        if (rc == LES_OK && cfg->autotune) rc = pool_autotune(c, *probe, true, true, true);
         if (rc != LES_OK) {
             fprintf(stderr, "les_batch_create: %s\n", c->err);
             les_gpu_destroy(c);

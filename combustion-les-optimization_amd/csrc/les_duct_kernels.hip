@@ -848,6 +848,11 @@ template <int TK>
 struct RhsStage {
     static constexpr bool on = (TK == 16 || TK == 32 || TK == 64);
     static constexpr int TJ = on ? 256 / TK : 1, SJ = TJ + 6, SK = on ? TK + 6 : 1, SN = on ? SJ * SK : 1;
+    // occupancy hint per element size: FP64 4 waves per SIMD (64-wide rows are
+    // held there by their 33.7 KB of LDS; compiled for 5 they schedule better:
+    // 128^3 step 0.88 -> 0.84 ms, while 32- and 16-wide rows spill at 5), FP32 6
+    // (80 VGPRs unstaged; the staged form needs 58, so runs 8 waves)
+    static constexpr int waves(int esize) { return esize == 8 ? (TK == 64 ? 5 : 4) : 6; }
 };
 
 // The RHS of all five fields of one RK stage fused with the SSP-RK3 combine,
@@ -865,9 +870,9 @@ struct RhsStage {
 // progress variable, the property tables.  Flat tiles and 128-wide rows walk
 // the directions in a rolled loop over memory; 64-, 32- and 16-wide rows take
 // their y and z lines from the LDS-staged neighbourhood (RhsStage).
-// FP64: 4 waves per SIMD (125 VGPRs); FP32: 6 waves (80 VGPRs; 7 spill).
+// Occupancy: RhsStage::waves.
 template <typename real, int TK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 4 : 6)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RhsStage<TK>::waves(sizeof(real)))))
 k_duct_rhs_all(Pool P, RhsArgs<real> a)
 {
     using ST = RhsStage<TK>;

@@ -45,14 +45,18 @@ def test_profile_inputs_are_present_and_consistent():
     with open(os.path.join(ROOT, "profiles", "rhs_valu.json")) as f:
         valu = json.load(f)
     for wl in ("duct128-fp64", "duct128-fp32"):
-        # HBM bytes per stage of the RHS launch for each XCD mapping a run may pick
-        for xm in (0, 1):
-            assert bench.traffic_from_profiles(wl, xm) == traffic[wl][str(xm)] > 0
+        # HBM bytes per stage of the RHS launch for each tile width and XCD
+        # mapping recorded ("tile_k:xcd_map")
+        assert traffic[wl]
+        for key, val in traffic[wl].items():
+            tk, xm = (int(x) for x in key.split(":"))
+            assert tk in (0, 16, 32, 64, 128) and 0 <= xm <= 3
+            assert bench.traffic_from_profiles(wl, xm, tk) == val > 0
         v = bench.valu_from_profiles(wl)
         assert v == valu[wl] and v["insts_per_stage"] > 0
         # physical shares: a quad-cycle issues at most once without dual issue
         assert 0 < v["busy_frac"] <= 1 and 0 <= v["dual_issue_share"] <= 1 and 0 < v["wait_any_frac"] < 1
         assert abs(v["insts_per_simd_quad"] - v["busy_frac"] * (1 + v["dual_issue_share"])) < 0.05
-    # FP32 moves fewer bytes than FP64 for the same stage
-    for xm in ("0", "1"):
-        assert traffic["duct128-fp32"][xm] < traffic["duct128-fp64"][xm]
+    # FP32 moves fewer bytes than FP64 for the same stage and launch shape
+    for key in set(traffic["duct128-fp32"]) & set(traffic["duct128-fp64"]):
+        assert traffic["duct128-fp32"][key] < traffic["duct128-fp64"][key]
