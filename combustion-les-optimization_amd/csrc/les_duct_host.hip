@@ -275,6 +275,7 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
     P.bcpart = (double*)dalloc_bytes(c, 8 * S * d->ny_max * d->nzb);
     P.spart = (double*)dalloc_bytes(c, 8 * 10 * S * d->nx_max * STAT_CHUNKS);
     P.ppart = (double*)dalloc_bytes(c, 8 * 2 * (size_t)d->tile_cap);
+    P.sdue = (int*)dalloc_bytes(c, 4 * (size_t)S);
     d->d_desc = (SlotDesc*)dalloc_bytes(c, sizeof(SlotDesc) * S);
     d->d_dyn = (SlotDyn*)dalloc_bytes(c, sizeof(SlotDyn) * S);
     d->d_tile_slot = (int*)dalloc_bytes(c, 4 * (size_t)d->tile_cap);
@@ -530,19 +531,18 @@ static int step_launches(les_gpu* c, bool reinit)
         }
         DLAUNCH_CHECK();
     }
-    // outputs of the new state: point samples, probe rows, statistics, in
-    // line (a side branch beside the statistics measured slower: the join
-    // costs more than the short point and probe launches)
-    hipLaunchKernelGGL((k_duct_points<real>), dim3(S), dim3(64), 0, c->stream, d->P);
-    if (d->probe_cap > 0)
-        hipLaunchKernelGGL((k_duct_probe<real>), dim3(LES_NPROBE_MAX, S), dim3(256), 0, c->stream, d->P);
+    // outputs of the new state (statistics chunks, probe rows, point samples)
+    // in one launch, then the plane sums and the step end in another (the
+    // outputs as separate launches, or as a side branch, cost more than their
+    // own work: each short launch and each join added its own latency)
     {
         const long long plm = (long long)d->ny_max * d->nz_max;
         const int sch = (int)std::min<long long>(STAT_CHUNKS, std::max<long long>(1, (plm + STAT_CHUNK_CELLS - 1) / STAT_CHUNK_CELLS));
-        hipLaunchKernelGGL((k_duct_stats<real>), dim3(sch, d->nx_max, S), dim3(256), 0, c->stream, d->P);
+        const int nout = sch * d->nx_max * S + (d->probe_cap > 0 ? LES_NPROBE_MAX * S : 0) + S;
+        hipLaunchKernelGGL((k_duct_outputs<real>), dim3(nout), dim3(256), 0, c->stream, d->P, sch, S);
+        const int fx = (d->nx_max * 10 + 255) / 256;
+        hipLaunchKernelGGL(k_duct_step_close, dim3(fx * S + (S + 255) / 256), dim3(256), 0, c->stream, d->P, fx, S);
     }
-    hipLaunchKernelGGL(k_duct_stats_final, dim3((d->nx_max * 10 + 255) / 256, S), dim3(256), 0, c->stream, d->P);
-    hipLaunchKernelGGL(k_duct_step_end, dim3((S + 255) / 256), dim3(256), 0, c->stream, d->P, S);
     DLAUNCH_CHECK();
     return LES_OK;
 }
@@ -1277,7 +1277,7 @@ extern "C" int les_batch_pool_bytes(const les_batch_config* cfg, size_t* bytes)
         b += S * 8 * (7 * nv + 3 * (size_t)cfg->max_ny * cfg->max_nz + 10 * (size_t)cfg->max_nx +
                       4 * (size_t)DUCT_NPTS * cfg->point_cap + (size_t)cfg->max_ny * ((cfg->max_nz + 255) / 256) +
                       10 * (size_t)cfg->max_nx * STAT_CHUNKS);
-        b += S * (sizeof(SlotDesc) + sizeof(SlotDyn)) + 16 * (size_t)cfg->table_n + 8 * 6 * 1024;
+        b += S * (sizeof(SlotDesc) + sizeof(SlotDyn) + 4) + 16 * (size_t)cfg->table_n + 8 * 6 * 1024;
         b += S * 8 * 3 * (size_t)cfg->max_ny * cfg->max_nz * ((nv + SEM_CHUNK - 1) / SEM_CHUNK);   // SEM partials
         if (cfg->probe_cap > 0)                                   // flame-region probe rings
             b += S * 8 * (size_t)cfg->probe_cap * LES_NPROBE_MAX * (2 + 6 * (size_t)cfg->max_ny);

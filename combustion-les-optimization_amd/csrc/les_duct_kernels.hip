@@ -178,6 +178,7 @@ struct Pool {
     int sem_split;          // eddy chunks of SEM_CHUNK per run
     double* probe;          // probe ring: [slot][probe_cap][station][2 + 6*ny_max]
     int probe_cap;          // steps per probe ring (0: no probes)
+    int* sdue;              // per slot: 1 if the run sampled statistics this step
 };
 
 #define PROBE_STRIDE(P) (2 + 6 * (P).ny_max)
@@ -1577,22 +1578,25 @@ __device__ __forceinline__ int stat_chunks(long long PL)
     return c < 1 ? 1 : (c > STAT_CHUNKS ? STAT_CHUNKS : (int)c);
 }
 
+// Statistics samples of chunk ch of plane i of run s: FP64 sums of u, v, w, c
+// and their products into the chunk's partials (and the per-cell FP64
+// accumulators when the run keeps them).  The block of chunk 0 of plane 0
+// records whether the run sampled this step, for the plane sums that follow.
 template <typename real>
-__global__ void __launch_bounds__(256) k_duct_stats(Pool P)
+__device__ __forceinline__ void stats_block(const Pool& P, int s, int i, int ch, double (*red)[10])
 {
-    __shared__ double red[4][10];
-    const int s = blockIdx.z;
     const SlotDesc& D = P.desc[s];
-    if (!D.active || P.dyn[s].done || !stats_due(D, P.dyn[s])) return;
+    const bool due = D.active && !P.dyn[s].done && stats_due(D, P.dyn[s]);
+    if (i == 0 && ch == 0 && threadIdx.x == 0) P.sdue[s] = due ? 1 : 0;
+    if (!due) return;
     const GridDims& g = D.g;
-    const int i = blockIdx.y;
     if (i >= g.nx) return;
     const DuctConsts& k = D.k;
     const long long PL = (long long)g.ny * g.nz;
     const int nch = stat_chunks(PL);
-    if ((int)blockIdx.x >= nch) return;
+    if (ch >= nch) return;
     const long long chunk = (PL + nch - 1) / nch;
-    const long long p0 = (long long)blockIdx.x * chunk;
+    const long long p0 = (long long)ch * chunk;
     const long long p1 = p0 + chunk < PL ? p0 + chunk : PL;
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const auto A0 = run_arr((const real*)P.A[0] + D.off);
@@ -1627,16 +1631,19 @@ __global__ void __launch_bounds__(256) k_duct_stats(Pool P)
         for (int a = 0; a < 10; ++a) red[wv][a] = acc[a];
     __syncthreads();
     if (threadIdx.x < 10)
-        P.spart[(((long long)s * P.nx_max + i) * STAT_CHUNKS + blockIdx.x) * 10 + threadIdx.x] =
+        P.spart[(((long long)s * P.nx_max + i) * STAT_CHUNKS + ch) * 10 + threadIdx.x] =
             ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
 }
 
-__global__ void k_duct_stats_final(Pool P)
+// The chunk partials of plane values t = 10 i + a of run s added in order into
+// the run's plane profiles, where the run sampled this step (the flag its
+// statistics launch recorded: the step counter itself advances in the same
+// launch).
+__device__ __forceinline__ void stats_final_block(const Pool& P, int s, int bx)
 {
-    const int s = blockIdx.y;
     const SlotDesc& D = P.desc[s];
-    if (!D.active || P.dyn[s].done || !stats_due(D, P.dyn[s])) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!D.active || !P.sdue[s]) return;
+    int t = bx * blockDim.x + threadIdx.x;
     if (t >= D.g.nx * 10) return;
     int i = t / 10, a = t % 10;
     double sm = 0.0;
@@ -1645,11 +1652,10 @@ __global__ void k_duct_stats_final(Pool P)
     P.prof[D.xoff + t] += sm;
 }
 
-// fixed-point series (u, v, w, G) of each slot into its ring, then the step end
+// fixed-point series (u, v, w, G) of run s into its ring
 template <typename real>
-__global__ void k_duct_points(Pool P)
+__device__ __forceinline__ void points_block(const Pool& P, int s)
 {
-    const int s = blockIdx.x;
     const SlotDesc& D = P.desc[s];
     if (!D.active || P.dyn[s].done) return;
     const int t = threadIdx.x;
@@ -1671,10 +1677,8 @@ __global__ void k_duct_points(Pool P)
 // v, w, G, nu_t and |S| (|S| recomputed from the last stage's filtered
 // velocities, nu_t of the last stage) go into the run's ring, one thread per j.
 template <typename real>
-__global__ void __launch_bounds__(256) k_duct_probe(Pool P)
+__device__ __forceinline__ void probe_block(const Pool& P, int s, int n, int& jmin)
 {
-    __shared__ int jmin;
-    const int s = blockIdx.y, n = blockIdx.x;
     const SlotDesc& D = P.desc[s];
     if (P.probe_cap <= 0 || !D.active || P.dyn[s].done || n >= D.nprobe) return;
     const GridDims g = D.g;
@@ -1734,10 +1738,42 @@ __global__ void __launch_bounds__(256) k_duct_probe(Pool P)
     }
 }
 
-__global__ void k_duct_step_end(Pool P, int nslots)
-
+// The outputs of the new state in one launch (they only read it): blocks
+// [0, nst) the statistics chunks (sch per plane, nx_max planes per run), then
+// LES_NPROBE_MAX probe stations per run (when probes are on), then one
+// point-series block per run.
+template <typename real>
+__global__ void __launch_bounds__(256) k_duct_outputs(Pool P, int sch, int nslots)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double red[4][10];
+    __shared__ int jmin;
+    int b = blockIdx.x;
+    const int nst = sch * P.nx_max * nslots;
+    if (b < nst) {
+        stats_block<real>(P, b / (sch * P.nx_max), (b / sch) % P.nx_max, b % sch, red);
+        return;
+    }
+    b -= nst;
+    const int npr = P.probe_cap > 0 ? LES_NPROBE_MAX * nslots : 0;
+    if (b < npr) {
+        probe_block<real>(P, b / LES_NPROBE_MAX, b % LES_NPROBE_MAX, jmin);
+        return;
+    }
+    b -= npr;
+    if (b < nslots) points_block<real>(P, b);
+}
+
+// The plane sums of the statistics (blocks [0, nfin), nfin = fx per run), then
+// the step end of every run (the remaining blocks, one thread per run): time,
+// step counter, done flag.
+__global__ void __launch_bounds__(256) k_duct_step_close(Pool P, int fx, int nslots)
+{
+    const int nfin = fx * nslots;
+    if ((int)blockIdx.x < nfin) {
+        stats_final_block(P, (int)blockIdx.x / fx, (int)blockIdx.x % fx);
+        return;
+    }
+    const int s = ((int)blockIdx.x - nfin) * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
     const SlotDesc& D = P.desc[s];
     SlotDyn& y = P.dyn[s];
