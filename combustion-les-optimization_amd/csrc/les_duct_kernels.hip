@@ -268,17 +268,23 @@ __device__ __forceinline__ real table_lookup(const double* tab, const DuctConsts
 // WENO5 face value (Jiang-Shu, linear weights 0.1/0.6/0.3, eps 1e-6), one
 // division per face: the nonlinear weights are scaled by the product of the
 // three (eps + IS_k)^2.  Constant factors are folded for the instruction
-// count: sqrt(13/12) into the second differences and 1/2 into the first, so
-// each IS_k is two FMAs; the linear weights and the common 1/6 into the
+// count: the three eps + IS_k are taken times 12/13 (a common factor of the
+// weights' numerators and denominator), so the second differences enter
+// unscaled and sqrt(12/13) x 1/2 goes into the first differences' coefficients,
+// each IS_k being two FMAs; the linear weights and the common 1/6 go into the
 // candidate values and the denominator.  Algebraically the textbook formula;
-// only the rounding differs from it.
+// only the rounding differs from it.  weno5_nd returns the quotient's
+// numerator and denominator.
 template <typename real>
 __device__ __forceinline__ void weno5_nd(real v0, real v1, real v2, real v3, real v4, real& num_out, real& den_out)
 {
-    const real k = (real)1.0408329997330663, eps = (real)1.0e-6;    // k = sqrt(13/12)
-    const real a0 = k * (v0 - (real)2.0 * v1 + v2), h0 = (real)0.5 * v0 - (real)2.0 * v1 + (real)1.5 * v2;
-    const real a1 = k * (v1 - (real)2.0 * v2 + v3), h1 = (real)0.5 * (v1 - v3);
-    const real a2 = k * (v2 - (real)2.0 * v3 + v4), h2 = (real)1.5 * v2 - (real)2.0 * v3 + (real)0.5 * v4;
+    // the three IS_k + eps scaled by 12/13 (the weights only see their ratios):
+    // the second differences unscaled, c = sqrt(12/13) folded into the first
+    constexpr double c = 0.96076892283052284;
+    const real eps = (real)(1.0e-6 * 12.0 / 13.0);
+    const real a0 = v0 - (real)2.0 * v1 + v2, h0 = (real)(0.5 * c) * v0 - (real)(2.0 * c) * v1 + (real)(1.5 * c) * v2;
+    const real a1 = v1 - (real)2.0 * v2 + v3, h1 = (real)(0.5 * c) * (v1 - v3);
+    const real a2 = v2 - (real)2.0 * v3 + v4, h2 = (real)(1.5 * c) * v2 - (real)(2.0 * c) * v3 + (real)(0.5 * c) * v4;
     const real e0 = a0 * a0 + (h0 * h0 + eps);
     const real e1 = a1 * a1 + (h1 * h1 + eps);
     const real e2 = a2 * a2 + (h2 * h2 + eps);
@@ -352,6 +358,31 @@ __device__ __forceinline__ real face_pair(int scheme, const real* v, bool pos_p,
         }
     }
     return p * face_plus<real>(scheme, v, pos_p) - m * face_minus<real>(scheme, v, pos_m);
+}
+
+// f(i+1/2) - f(i-1/2) of the cell's two upwind faces when both take the same
+// side (the scalars: upwind by the cell's own velocity): the line is reversed
+// once when the side is negative, so the two faces read fixed positions of it
+// (A = f(r1..r5), B = f(r0..r4): the faces are (A, B) on the positive side
+// and (B, A) on the negative one) instead of selecting ten stencil values
+template <typename real>
+__device__ __forceinline__ real face_diff_same(int scheme, const real* v, bool pos)
+{
+    real r[7];
+#pragma unroll
+    for (int o = 0; o < 7; ++o) r[o] = (o == 3 || pos) ? v[o] : v[6 - o];
+    real d;
+    if constexpr (sizeof(real) == 8) {
+        if (scheme == LES_ADV_WENO5) {
+            real na, da, nb, db;
+            weno5_nd<real>(r[1], r[2], r[3], r[4], r[5], na, da);
+            weno5_nd<real>(r[0], r[1], r[2], r[3], r[4], nb, db);
+            d = M<real>::div(na * db - nb * da, da * db);
+            return pos ? d : -d;
+        }
+    }
+    d = face_left<real>(scheme, r[1], r[2], r[3], r[4], r[5]) - face_left<real>(scheme, r[0], r[1], r[2], r[3], r[4]);
+    return pos ? d : -d;
 }
 
 
@@ -822,7 +853,7 @@ __device__ __forceinline__ void rhs_line(RhsSums<real>& A, const LD& ld, int vd,
                 tt = (u0 >= (real)0.0) ? u0 * (v0 - vm1) * id : u0 * (vp1 - v0) * id;
             } else {
                 const bool pos = u0 >= (real)0.0;
-                tt = u0 * face_pair<real>(scheme, v, pos, pos, (real)1.0, (real)1.0) * id;
+                tt = u0 * face_diff_same<real>(scheme, v, pos) * id;
             }
             const real isc2 = m == 3 ? isc : ipr;
             const real gp = (ntp + nu0) * isc2, gm = (nu0 + ntm) * isc2;
