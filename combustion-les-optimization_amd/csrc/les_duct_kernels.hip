@@ -697,8 +697,11 @@ __device__ __forceinline__ void sgs_cell(const Pool& P, const DuctConsts& k, con
     }
     real eta = M<real>::div(DX, d_eta + (real)LES_SMALL);
     const real dF = rcp_const<real>(k, DUCT_RC_DF);
-    real asf = (real)8.0 * M<real>::exp(M<real>::div((real)10.0 * dF, (real)8.0 * eta + (real)LES_SMALL));
-    real wr = M<real>::pow(asf * eta * rcp_const<real>(k, DUCT_RC_ID), (real)2.0 - (real)k.d3);
+    // wrinkling (asf eta / dx)^(2 - D3) with asf = 8 exp(10 dF / (8 eta + small)),
+    // taken as one exponential: exp((2 - D3) (ln 8 + 10 dF / (8 eta + small) + ln(eta / dx)))
+    const real lg = (real)2.0794415416798357 + M<real>::div((real)10.0 * dF, (real)8.0 * eta + (real)LES_SMALL) +
+                    M<real>::log(eta * rcp_const<real>(k, DUCT_RC_ID));
+    real wr = M<real>::exp(((real)2.0 - (real)k.d3) * lg);
     real te = DX * fmax((real)0.0, div) * rcp_const<real>(k, DUCT_RC_ITSL);
     st_out = fmax(SL, SL * fmax((real)0.0, wr + te));
 }
