@@ -87,7 +87,7 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--time-scale", type=float, default=1.0, help="served stream: arrival-time scale")
     ap.add_argument("--requests", type=int, default=96, help="served stream: number of requests")
-    ap.add_argument("--xcd-map", type=int, choices=[0, 1, 2, 3], default=None,
+    ap.add_argument("--xcd-map", type=int, choices=[0, 1, 2, 3, 4], default=None,
                     help="single runs: fix the XCD mapping (LES_XCD_MAP) instead of the creation-time trial")
     ap.add_argument("--tile-k", type=int, choices=[0, 16, 32, 64, 128], default=None,
                     help="single runs: fix the tile width (LES_TILE_K) instead of the creation-time trial")

@@ -232,7 +232,7 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
         d->cus = prop.multiProcessorCount;
-    d->xcd_map = cfg.xcd_map & 3;
+    d->xcd_map = cfg.xcd_map & 7;
     d->zigzag = (cfg.xcd_map & XCD_REVERSE) ? 1 : 0;
     // tiles of all resident runs, for any tile width: per run at most
     // n/256 + nx*ny/2 + nx*nz/16 + nx (ceil bounds of each tile shape)
@@ -319,10 +319,15 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
 }
 
 // The stencil grid: the tile high-water mark rounded up to whole rounds of
-// the XCD mapping (8 blocks, or 8 chunks of tiles).
+// the XCD mapping (8 blocks, 8 chunks of tiles, or whole planes of slot 0's
+// layout for the slabs, so that the slab permutation covers every tile).
 static void pool_set_grid(DuctState* d)
 {
-    const int r = 8 * xcd_chunk(d->xcd_map);
+    int r = 8 * xcd_chunk(d->xcd_map);
+    if ((d->xcd_map & 7) == 4) {
+        const int tpp = d->desc.empty() ? 0 : d->desc[0].tiles_j * d->desc[0].tiles_k;
+        if (tpp > 0 && tpp % 8 == 0) r = tpp;
+    }
     const int grid = d->ntile_hwm > 0 ? (d->ntile_hwm + r - 1) / r * r : r;
     if (grid != d->grid) { d->grid = grid; d->graph_ok[0] = d->graph_ok[1] = false; }
 }
@@ -820,7 +825,7 @@ static int pool_autotune(les_gpu* c, const les_params& probe, bool tune_tile, bo
     // clocks and lost to the later ones by more than the candidates differ
     for (int pass = 0; pass < 2; ++pass)
         for (int a = 0; a < (tune_tile ? 5 : 1); ++a)
-            for (int x = 0; x < (tune_xcd ? 4 : 1); ++x) {
+            for (int x = 0; x < (tune_xcd ? 5 : 1); ++x) {
                 const int xm = tune_xcd ? x : d->xcd_map;
                 float ms = 0.f;
                 const int e = time_candidate(tks[a], xm | zz, &ms);
@@ -999,13 +1004,13 @@ int duct_create(les_gpu* c)
     // 128^3 prefers round-robin tiles, smaller grids and FP32 contiguous
     // ranges per XCD); results do not depend on it, so each run times the four
     // before it is admitted (a failed trial, e.g. no memory for the scratch
-    // pool, keeps the default), unless LES_XCD_MAP (0..3) fixes it.  The tile
+    // pool, keeps the default), unless LES_XCD_MAP (0..4) fixes it.  The tile
     // width is not timed: it groups the per-tile partial sums of rho, so a
     // timed choice would make a run's rounding depend on the trial; it is the
     // widest row that divides nz, unless LES_TILE_K (0, 16, 32, 64, 128) fixes it
     const char* fx = getenv("LES_XCD_MAP");
     const char* ft = getenv("LES_TILE_K");
-    const bool fixed_x = fx && fx[0] >= '0' && fx[0] <= '3' && fx[1] == 0;
+    const bool fixed_x = fx && fx[0] >= '0' && fx[0] <= '4' && fx[1] == 0;
     if (ft && ft[0] && tk_valid(atoi(ft))) c->d->tk = atoi(ft);
     if (fixed_x) {
         c->d->xcd_map = fx[0] - '0';
@@ -1257,7 +1262,7 @@ static int batch_config_check(const les_batch_config* cfg)
     if (!cfg || cfg->max_slots < 1 || cfg->max_slots > 8192 || cfg->pool_cells < 1 || cfg->max_nx < 6 ||
         cfg->max_ny < 1 || cfg->max_nz < 1 || cfg->max_vortices < 0 || !tk_valid(cfg->tile_k) ||
         (cfg->precision != LES_PREC_FP64 && cfg->precision != LES_PREC_FP32) || cfg->point_cap < 1 ||
-        cfg->table_n < 2 || cfg->table_dt <= 0.0)
+        cfg->table_n < 2 || cfg->table_dt <= 0.0 || cfg->xcd_map < 0 || (cfg->xcd_map & ~15) || (cfg->xcd_map & 7) > 4)
         return LES_ERR_PARAM;
     return LES_OK;
 }

@@ -391,10 +391,10 @@ __device__ __forceinline__ real face_diff_same(int scheme, const real* v, bool p
 // ---------------------------------------------------------------------------
 #define DUCT_CAS __attribute__((address_space(4)))
 
-#define XCD_REVERSE 4      // flag on a launch's xcd_map: its blocks walk their XCD's tiles backwards
+#define XCD_REVERSE 8      // flag on a launch's xcd_map: its blocks walk their XCD's tiles backwards
 
 // tiles per chunk of the chunked XCD mappings (xcd_map 2 and 3); 1 otherwise
-__host__ __device__ inline int xcd_chunk(int xcd_map) { return (xcd_map & 3) == 2 ? 16 : ((xcd_map & 3) == 3 ? 64 : 1); }
+__host__ __device__ inline int xcd_chunk(int xcd_map) { return (xcd_map & 7) == 2 ? 16 : ((xcd_map & 7) == 3 ? 64 : 1); }
 
 // One tile per block.  Blocks are dispatched round-robin over the 8 XCDs
 // (block b runs on XCD b % 8); with xcd_map the blocks of one XCD take one
@@ -409,7 +409,20 @@ __device__ __forceinline__ int tile_of_block(const Pool& P, int xcd_map)
         // follows one in forward order starts on the tiles the other wrote last
         b = (((int)(gridDim.x >> 3) - 1 - (b >> 3)) << 3) | (b & 7);
     }
-    xcd_map &= 3;
+    xcd_map &= 7;
+    if (xcd_map == 4) {
+        // slabs: each XCD takes the same eighth of every x-plane's tiles (the
+        // plane layout of the run in slot 0), plane after plane, so a tile's
+        // x neighbours are read through the XCD's own L2; planes whose tile
+        // count is not a multiple of 8 fall back to contiguous ranges
+        const int tpp = *(const DUCT_CAS int*)&P.desc[0].tiles_j * *(const DUCT_CAS int*)&P.desc[0].tiles_k;
+        if (tpp > 0 && (tpp & 7) == 0) {
+            const int w = tpp >> 3, l = b >> 3;
+            const int tt = (l / w) * tpp + (b & 7) * w + l % w;
+            return tt < *(const DUCT_CAS int*)P.ntile ? tt : -1;
+        }
+        xcd_map = 1;
+    }
     if (xcd_map == 1 && (gridDim.x & 7) == 0) {
         b = (b & 7) * (int)(gridDim.x >> 3) + (b >> 3);
     } else if (xcd_map >= 2) {
@@ -973,7 +986,8 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
                 RhsSums<real> S;
                 S.cv0 = S.cv1 = S.cv2 = S.ap = S.aT = S.lp = S.lT = S.gg = (real)0;
                 // the cell in the staged neighbourhood (row tiles of 64, 32 or 16)
-                const int lb = ST::on ? ((int)threadIdx.x / TK + 3) * ST::SK + (int)threadIdx.x % TK + 3 : 0;
+                constexpr int TKS = TK > 0 ? TK : 1;
+                const int lb = ST::on ? ((int)threadIdx.x / TKS + 3) * ST::SK + (int)threadIdx.x % TKS + 3 : 0;
                 if constexpr (ST::on) {
                     // x lines from memory (the band next to the end planes reads
                     // offsets -1..1 only: there the outer offsets repeat them)

@@ -278,8 +278,9 @@ int les_gpu_set_timing(les_gpu* ctx, int on);
  * reference has no device launches): tile_k (LES_TILE_FLAT or the tile row
  * width) and xcd_map (how tiles are dealt to the GPU's 8 XCDs: 0 round-robin,
  * 1 one contiguous range per XCD, 2 and 3 chunks of 16 and 64 consecutive
- * tiles round-robin).  A run times the mappings at creation and keeps the
- * fastest; the environment variable LES_XCD_MAP=0..3, set before
+ * tiles round-robin, 4 slabs: each XCD the same eighth of every x-plane).  A
+ * run times the mappings at creation and keeps the fastest; the environment
+ * variable LES_XCD_MAP=0..4 (and LES_TILE_K for the tile width), set before
  * les_gpu_create, fixes the mapping instead (results do not depend on it).
  * LES_ERR_PARAM in mode R, whose kernels launch one thread per cell. */
 int les_gpu_launch_config(const les_gpu* ctx, int* tile_k, int* xcd_map);
@@ -315,7 +316,8 @@ typedef struct les_batch_config {
     int       precision;       /* LES_PREC_*: one storage type per pool */
     int       tile_k;          /* tile width in k: LES_TILE_FLAT, 16, 32, 64 or 128 */
     int       xcd_map;         /* tiles to XCDs: 0 round-robin, 1 one contiguous range per XCD,
-                                  2 / 3 chunks of 16 / 64 tiles round-robin; + 4: the stencil
+                                  2 / 3 chunks of 16 / 64 tiles round-robin, 4 slabs (each XCD the
+                                  same eighth of every x-plane of slot 0's layout); + 8: the stencil
                                   launches alternate the order in which each XCD walks its tiles */
     int       autotune;        /* 1: time tile-width and XCD-mapping candidates at create
                                   on the probe grid and keep the fastest */

@@ -88,8 +88,9 @@ def test_runs_admitted_later_match_alone(lesgpu):
             assert np.array_equal(x, y)
 
 
-@pytest.mark.parametrize("tile_k,xcd", [(0, 1), (32, 1), (128, 0), (64, 0), (16, 2)],
-                         ids=["flat_xcd", "tk32_xcd", "tk128", "tk64", "tk16_chunks"])
+@pytest.mark.parametrize("tile_k,xcd", [(0, 1), (32, 1), (128, 0), (64, 0), (16, 2), (64, 4), (32, 12)],
+                         ids=["flat_xcd", "tk32_xcd", "tk128", "tk64", "tk16_chunks", "tk64_slabs",
+                              "tk32_slabs_alternating"])
 def test_launch_shapes_vs_oracle(lesgpu, oracle, tile_k, xcd):
     print("case: batch tile_k=%d xcd=%d vs oracle" % (tile_k, xcd), flush=True)
     over = dict(reinit_interval=3, reinit_iters=2)
@@ -148,7 +149,7 @@ def test_batch_point_series_and_autotune(lesgpu):
         t = b.tuning()
         print("tuned:", t, flush=True)
         assert t["tile_k"] in (0, 16, 32, 64, 128) and t["grid_blocks"] > 0 and t["tuned_ms"] > 0
-        assert t["xcd_map"] in range(8)                          # mapping 0..3, +4 alternated sweeps
+        assert (t["xcd_map"] & 7) in range(5) and t["xcd_map"] < 16    # mapping 0..4, +8 alternated sweeps
         s = b.admit(probe, 12)
         b.step(12)
         ser = b.point_series(s, 64)
@@ -195,14 +196,14 @@ def test_xface_copy_fills_ghost_planes(lesgpu, precision):
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
 def test_xcd_mapping_does_not_change_results(lesgpu, precision):
     """The block-to-tile mapping only changes which XCD runs a tile: the state
-    and profiles are bitwise the same under all four mappings (round-robin,
-    contiguous ranges, chunks of 16 and of 64 tiles) and with the stencil
-    launches' sweep direction alternated (+4) (single runs tune the mapping
+    and profiles are bitwise the same under all five mappings (round-robin,
+    contiguous ranges, chunks of 16 and of 64 tiles, slabs) and with the
+    stencil launches' sweep direction alternated (+8) (single runs tune the mapping
     per run and pools the sweep, so results must not depend on either)."""
     print("case: XCD mapping neutrality, precision=%d" % precision, flush=True)
     p = run_params(lesgpu, precision, nx=64, ny=32, nz=64, seed=9)
     out = []
-    for xm in (0, 1, 2, 3, 4, 6):
+    for xm in (0, 1, 2, 3, 4, 8, 10, 12):
         with pool(lesgpu, precision, slots=1, cells=p.nx * p.ny * p.nz, xcd_map=xm) as b:
             s = b.admit(p, n_steps=6)
             b.step(6)

@@ -131,11 +131,11 @@ def test_graph_replay_bitwise(lesgpu):
 def test_xcd_map_override_and_launch_config(lesgpu, monkeypatch):
     """LES_XCD_MAP fixes a single run's XCD mapping (les_gpu_launch_config
     reports it) and the mapping does not change results: the runs under all
-    four mappings equal bit for bit after 4 steps.  Mode R reports no launch
+    five mappings equal bit for bit after 4 steps.  Mode R reports no launch
     shape."""
-    print("case: LES_XCD_MAP=0..3, 64x32x32 duct, 4 steps", flush=True)
+    print("case: LES_XCD_MAP=0..4, 64x32x32 duct, 4 steps", flush=True)
     states = []
-    for xm in (0, 1, 2, 3):
+    for xm in (0, 1, 2, 3, 4):
         monkeypatch.setenv("LES_XCD_MAP", str(xm))
         g = lesgpu.LesGpu(lesgpu.params(1))
         try:
