@@ -50,7 +50,7 @@ def test_profile_inputs_are_present_and_consistent():
         assert traffic[wl]
         for key, val in traffic[wl].items():
             tk, xm = (int(x) for x in key.split(":"))
-            assert tk in (0, 16, 32, 64, 128) and 0 <= xm <= 3
+            assert tk in (0, 16, 32, 64, 128) and 0 <= xm <= 4
             assert bench.traffic_from_profiles(wl, xm, tk) == val > 0
         v = bench.valu_from_profiles(wl)
         assert v == valu[wl] and v["insts_per_stage"] > 0
