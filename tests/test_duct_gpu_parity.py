@@ -51,10 +51,18 @@ def test_duct_init_and_rhs(lesgpu, oracle, advection):
         o.close()
 
 
-@pytest.mark.parametrize("advection", [1, 2], ids=["weno5", "muscl_minmod"])
-def test_duct_steps_fp64(lesgpu, oracle, advection):
-    print("case: duct64 FP64 10 steps (reinit every 5), advection=%d" % advection, flush=True)
+@pytest.mark.parametrize("advection,tile_k", [(1, None), (2, None), (1, 16), (1, 64), (1, 0)],
+                         ids=["weno5", "muscl_minmod", "weno5_tk16", "weno5_tk64", "weno5_flat"])
+def test_duct_steps_fp64(lesgpu, oracle, advection, tile_k, monkeypatch):
+    """10 FP64 steps against the oracle on 64x32x32: 32-wide row tiles by
+    default (RHS and filter staged in LDS), 16- and 64-wide rows (a 64-wide
+    row holds the 32 cells of a line and 32 idle lanes), flat tiles."""
+    print("case: duct64 FP64 10 steps (reinit every 5), advection=%d tile_k=%s" % (advection, tile_k), flush=True)
+    if tile_k is not None:
+        monkeypatch.setenv("LES_TILE_K", str(tile_k))
     g, o = make_pair(lesgpu, oracle, advection, reinit_interval=5, reinit_iters=3)
+    if tile_k is not None:
+        assert g.launch_config()["tile_k"] == tile_k
     try:
         g.init_state()
         o.init_state()
@@ -91,10 +99,18 @@ def o_lib():
     return pyoracle.lib()
 
 
-@pytest.mark.parametrize("advection", [1, 2], ids=["weno5", "muscl_minmod"])
-def test_duct_fp32_error_vs_fp64(lesgpu, oracle, advection):
-    print("case: duct64 FP32 form vs FP64 oracle, 5 steps, advection=%d" % advection, flush=True)
+@pytest.mark.parametrize("advection,tile_k", [(1, None), (2, None), (1, 16), (1, 0)],
+                         ids=["weno5", "muscl_minmod", "weno5_tk16", "weno5_flat"])
+def test_duct_fp32_error_vs_fp64(lesgpu, oracle, advection, tile_k, monkeypatch):
+    """The FP32 form on 64x32x32 (32-wide row tiles by default: the RHS and the
+    filter take their y/z neighbours from LDS; 16-wide rows; flat tiles, no
+    staging) against the FP64 oracle after 5 steps."""
+    print("case: duct64 FP32 form vs FP64 oracle, 5 steps, advection=%d tile_k=%s" % (advection, tile_k), flush=True)
+    if tile_k is not None:
+        monkeypatch.setenv("LES_TILE_K", str(tile_k))
     g, o = make_pair(lesgpu, oracle, advection, precision=1)
+    if tile_k is not None:
+        assert g.launch_config()["tile_k"] == tile_k
     try:
         g.init_state()
         o.init_state()
