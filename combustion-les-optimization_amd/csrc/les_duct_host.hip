@@ -276,13 +276,15 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
     P.spart = (double*)dalloc_bytes(c, 8 * 10 * S * d->nx_max * STAT_CHUNKS);
     P.ppart = (double*)dalloc_bytes(c, 8 * 2 * (size_t)d->tile_cap);
     P.sdue = (int*)dalloc_bytes(c, 4 * (size_t)S);
+    P.dtpart = (double*)dalloc_bytes(c, 8 * 3 * DT_BLOCKS_MAX * S);
+    P.dtcnt = (unsigned*)dalloc_bytes(c, 4 * S);
     d->d_desc = (SlotDesc*)dalloc_bytes(c, sizeof(SlotDesc) * S);
     d->d_dyn = (SlotDyn*)dalloc_bytes(c, sizeof(SlotDyn) * S);
     d->d_tile_slot = (int*)dalloc_bytes(c, 4 * (size_t)d->tile_cap);
     d->d_ntile = (int*)dalloc_bytes(c, 64);
     d->mon_part = (double*)dalloc_bytes(c, 8 * 6 * 1024);
     if (!P.eddy || !P.inlet || !P.prof || !P.points || !P.rho_part || !P.vpart || !P.dpart || !P.bcpart ||
-        !P.spart || !P.ppart || !d->d_desc || !d->d_dyn || !d->d_tile_slot || !d->d_ntile || !d->mon_part)
+        !P.spart || !P.ppart || !P.sdue || !P.dtpart || !P.dtcnt || !d->d_desc || !d->d_dyn || !d->d_tile_slot || !d->d_ntile || !d->mon_part)
         return LES_ERR_HIP;
     if (d->precision == LES_PREC_FP32 && !(d->stage64 = (double*)dalloc_bytes(c, 8 * (size_t)d->pool_cells)))
         return LES_ERR_HIP;
@@ -500,7 +502,12 @@ static int step_launches(les_gpu* c, bool reinit)
     DuctState* d = c->d;
     const int S = d->max_slots;
     hipStream_t side = c->timing ? c->stream : d->aux;
-    hipLaunchKernelGGL(k_duct_dt, dim3(S), dim3(1024), 0, c->stream, d->P);
+    {
+        // blocks per slot for the DT maxima: one per 1024 tiles of the largest run
+        const long long tl = (long long)d->nx_max * d->ny_max * d->nz_max / 256;
+        const int nb = (int)std::max<long long>(1, std::min<long long>(DT_BLOCKS_MAX, (tl + 1023) / 1024));
+        hipLaunchKernelGGL(k_duct_dt, dim3(nb, S), dim3(256), 0, c->stream, d->P, nb);
+    }
     // the inflow turbulence (eddies -> SEM plane) is read only by the boundary
     // update after the stages, so it runs beside them
     int e = side_fork(c, 0);
@@ -1282,7 +1289,7 @@ extern "C" int les_batch_pool_bytes(const les_batch_config* cfg, size_t* bytes)
         b += S * 8 * (7 * nv + 3 * (size_t)cfg->max_ny * cfg->max_nz + 10 * (size_t)cfg->max_nx +
                       4 * (size_t)DUCT_NPTS * cfg->point_cap + (size_t)cfg->max_ny * ((cfg->max_nz + 255) / 256) +
                       10 * (size_t)cfg->max_nx * STAT_CHUNKS);
-        b += S * (sizeof(SlotDesc) + sizeof(SlotDyn) + 4) + 16 * (size_t)cfg->table_n + 8 * 6 * 1024;
+        b += S * (sizeof(SlotDesc) + sizeof(SlotDyn) + 8 + 24 * DT_BLOCKS_MAX) + 16 * (size_t)cfg->table_n + 8 * 6 * 1024;
         b += S * 8 * 3 * (size_t)cfg->max_ny * cfg->max_nz * ((nv + SEM_CHUNK - 1) / SEM_CHUNK);   // SEM partials
         if (cfg->probe_cap > 0)                                   // flame-region probe rings
             b += S * 8 * (size_t)cfg->probe_cap * LES_NPROBE_MAX * (2 + 6 * (size_t)cfg->max_ny);

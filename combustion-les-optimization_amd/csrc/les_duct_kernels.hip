@@ -36,6 +36,7 @@
 #define DNF 5
 #define DUCT_NPTS 64
 #define STAT_CHUNKS 16            // most chunks of one x-plane in the statistics pass
+#define DT_BLOCKS_MAX 16          // most blocks per slot of the DT reduction
 #define STAT_CHUNK_CELLS 2048     // target cells per chunk (8 per thread)
 
 // ---------------------------------------------------------------------------
@@ -179,6 +180,8 @@ struct Pool {
     double* probe;          // probe ring: [slot][probe_cap][station][2 + 6*ny_max]
     int probe_cap;          // steps per probe ring (0: no probes)
     int* sdue;              // per slot: 1 if the run sampled statistics this step
+    double* dtpart;         // [slot][DT_BLOCKS_MAX][3]: block maxima of the DT reduction
+    unsigned* dtcnt;        // per slot: blocks of the DT reduction counted in (0 between launches)
 };
 
 #define PROBE_STRIDE(P) (2 + 6 * (P).ny_max)
@@ -738,12 +741,12 @@ __device__ __forceinline__ void sgs_cell(const Pool& P, const DuctConsts& k, con
 {
     const real SL = (real)k.s_l;
     const real DX = (real)g.dx;
-    real rf = M<real>::div((real)k.rho_u_Tu, T);
+    // nu = mu / rho with rho = rho_u T_u / T, and alpha = lambda / (rho cp) with
+    // lambda = mu cp / Pr, so alpha = nu / Pr: the cp table and the divisions
+    // cancel (the same quantities up to rounding)
     real mu = table_lookup<real>(P.mu_tab, k, T);
-    real cp = table_lookup<real>(P.cp_tab, k, T);
-    real nu = M<real>::div(mu, rf);
-    real lam = mu * cp * rcp_const<real>(k, DUCT_RC_IPRAN);
-    real al = M<real>::div(lam, rf * cp);
+    real nu = mu * T * rcp_const<real>(k, DUCT_RC_IRUT);
+    real al = nu * rcp_const<real>(k, DUCT_RC_IPRAN);
     mn = (double)nu; ma = (double)al;
     real S12 = (real)0.5 * (du_dy + dv_dx), S13 = (real)0.5 * (du_dz + dw_dx), S23 = (real)0.5 * (dv_dz + dw_dy);
     real ss = (real)2.0 * (S12 * S12 + S13 * S13 + S23 * S23) + du_dx * du_dx + dv_dy * dv_dy + dw_dz * dw_dz;
@@ -763,12 +766,13 @@ __device__ __forceinline__ void sgs_cell(const Pool& P, const DuctConsts& k, con
         real alg = (real)0.2187 * px;
         d_eta = (real)0.2340 * px * (alg > (real)1.0 ? M<real>::log(alg) : (real)0.0);
     }
-    real eta = M<real>::div(DX, d_eta + (real)LES_SMALL);
+    const real dd = d_eta + (real)LES_SMALL;   // the Kolmogorov scale eta = dx / dd
     const real dF = rcp_const<real>(k, DUCT_RC_DF);
     // wrinkling (asf eta / dx)^(2 - D3) with asf = 8 exp(10 dF / (8 eta + small)),
     // taken as one exponential: exp((2 - D3) (ln 8 + 10 dF / (8 eta + small) + ln(eta / dx)))
-    const real lg = (real)2.0794415416798357 + M<real>::div((real)10.0 * dF, (real)8.0 * eta + (real)LES_SMALL) +
-                    M<real>::log(eta * rcp_const<real>(k, DUCT_RC_ID));
+    // with 10 dF / (8 eta + small) = 10 dF dd / (8 dx + small dd) and ln(eta / dx) = -ln dd
+    const real lg = (real)2.0794415416798357 +
+                    M<real>::div((real)10.0 * dF * dd, (real)8.0 * DX + (real)LES_SMALL * dd) - M<real>::log(dd);
     real wr = M<real>::exp(((real)2.0 - (real)k.d3) * lg);
     real te = DX * fmax((real)0.0, div) * rcp_const<real>(k, DUCT_RC_ITSL);
     st_out = fmax(SL, SL * fmax((real)0.0, wr + te));
@@ -1213,25 +1217,27 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
 }
 
 // ---------------------------------------------------------------------------
-// DT per slot (:4399-4411 structure): one block per slot reduces the per-tile
-// maxima of the previous step (max |u| of the new state, max nu / alpha over
-// its 3 stages) and the inflow-plane maxima.
+// DT per slot (:4399-4411 structure): the per-tile maxima of the previous step
+// (max |u| of the new state, max nu / alpha over its 3 stages) and the
+// inflow-plane maxima, reduced by nb blocks per slot (grid nb x slots), each a
+// strided share of the tiles, four per thread and pass with their loads in
+// flight together.  The maxima are exact in any order.  The slot's last block
+// to count itself in takes the nb block maxima and sets the DT; it also resets
+// the slot's arrival counter, so the counter is zero at every launch.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_duct_dt(Pool P)
+__global__ void __launch_bounds__(256) k_duct_dt(Pool P, int nb)
 {
-    __shared__ double r[3][16];
-    const int s = blockIdx.x;
+    __shared__ double r[3][4];
+    __shared__ int last;
+    const int s = blockIdx.y, bk = blockIdx.x;
     const SlotDesc& D = P.desc[s];
     if (!D.active || P.dyn[s].done) return;
-    const DuctConsts& k = D.k;
-    // maxima are exact in any order: four tiles per thread and pass (their
-    // loads in flight together), then the waves' butterflies and 16 wave maxima
     double mv = 0.0, mn = 0.0, ma = 0.0;
     const int nt = D.ntiles;
-    for (int b0 = 0; b0 < nt; b0 += 4096) {
+    for (int b0 = bk * 1024; b0 < nt; b0 += nb * 1024) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int b = b0 + u * 1024 + (int)threadIdx.x;
+            const int b = b0 + u * 256 + (int)threadIdx.x;
             if (b < nt) {
                 const int t = D.tile0 + b;
                 mv = fmax(mv, P.vpart[t]);
@@ -1245,7 +1251,8 @@ __global__ void __launch_bounds__(1024) k_duct_dt(Pool P)
         }
     }
     const int nbc = P.ny_max * P.nzb;
-    for (int b = threadIdx.x; b < nbc; b += 1024) mv = fmax(mv, P.bcpart[(long long)s * nbc + b]);
+    for (int b = bk * 256 + (int)threadIdx.x; b < nbc; b += nb * 256)
+        mv = fmax(mv, P.bcpart[(long long)s * nbc + b]);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         mv = fmax(mv, __shfl_xor(mv, o));
@@ -1256,10 +1263,25 @@ __global__ void __launch_bounds__(1024) k_duct_dt(Pool P)
         r[0][threadIdx.x >> 6] = mv; r[1][threadIdx.x >> 6] = mn; r[2][threadIdx.x >> 6] = ma;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int w = 1; w < 16; ++w) {
-        mv = fmax(mv, r[0][w]); mn = fmax(mn, r[1][w]); ma = fmax(ma, r[2][w]);
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            mv = fmax(mv, r[0][w]); mn = fmax(mn, r[1][w]); ma = fmax(ma, r[2][w]);
+        }
+        double* bp = P.dtpart + 3 * ((long long)s * DT_BLOCKS_MAX + bk);
+        bp[0] = mv; bp[1] = mn; bp[2] = ma;
+        // the block maxima are released before the block counts itself in
+        __threadfence();
+        last = atomicAdd(&P.dtcnt[s], 1u) == (unsigned)(nb - 1);
     }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
+    P.dtcnt[s] = 0u;
+    for (int b = 0; b < nb; ++b) {
+        const double* bp = P.dtpart + 3 * ((long long)s * DT_BLOCKS_MAX + b);
+        mv = fmax(mv, bp[0]); mn = fmax(mn, bp[1]); ma = fmax(ma, bp[2]);
+    }
+    const DuctConsts& k = D.k;
     const double dx = D.g.dx;
     double maxvel = mv > LES_SMALL ? mv : k.U;
     double dtc = k.courant * dx / (maxvel + LES_SMALL);
