@@ -125,8 +125,9 @@ struct DuctState {
     double* mon_part = nullptr;
     int max_reinit_iters = 0;
     int max_proj_iters = 0;
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};   // [0] plain step, [1] with reinitialisation
-    bool graph_ok[2] = {false, false};
+    // [0] plain step, [1] step with reinitialisation, [2] DUCT_GRAPH_STEPS plain steps
+    hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+    bool graph_ok[3] = {false, false, false};
     int use_graph = 1;
     // side stream for the inflow turbulence, which runs beside the RK stages,
     // forked from and joined back into the main stream
@@ -331,7 +332,7 @@ static void pool_set_grid(DuctState* d)
         if (tpp > 0 && tpp % 8 == 0) r = tpp;
     }
     const int grid = d->ntile_hwm > 0 ? (d->ntile_hwm + r - 1) / r * r : r;
-    if (grid != d->grid) { d->grid = grid; d->graph_ok[0] = d->graph_ok[1] = false; }
+    if (grid != d->grid) { d->grid = grid; d->graph_ok[0] = d->graph_ok[1] = d->graph_ok[2] = false; }
 }
 
 // Uploads the descriptor table, tile list and tile high-water mark.
@@ -352,7 +353,8 @@ static int pool_sync_tables(les_gpu* c)
         if (x.active && x.k.reinit_interval > 0 && x.k.reinit_iters > maxit) maxit = x.k.reinit_iters;
         if (x.active && x.k.proj_iters > maxpj) maxpj = x.k.proj_iters;
     }
-    if (maxit != d->max_reinit_iters || maxpj != d->max_proj_iters) d->graph_ok[0] = d->graph_ok[1] = false;
+    if (maxit != d->max_reinit_iters || maxpj != d->max_proj_iters)
+        d->graph_ok[0] = d->graph_ok[1] = d->graph_ok[2] = false;
     d->max_reinit_iters = maxit;
     d->max_proj_iters = maxpj;
     return LES_OK;
@@ -438,6 +440,7 @@ static int pool_release(les_gpu* c, int s)
 }
 
 // ---- launches ----
+#define DUCT_GRAPH_STEPS 4   // steps of the multi-step captured graph
 template <typename real, int TK>
 static int stage_launch(les_gpu* c, int stage, real* const* Lout)
 {
@@ -509,17 +512,23 @@ static int step_launches(les_gpu* c, bool reinit)
         hipLaunchKernelGGL(k_duct_dt, dim3(nb, S), dim3(256), 0, c->stream, d->P, nb);
     }
     // the inflow turbulence (eddies -> SEM plane) is read only by the boundary
-    // update after the stages, so it runs beside them
+    // update after the stages, so it runs beside them, forked after the DT.
+    // The first stage's launches are captured before the side chain's: a
+    // captured step keeps the branch captured first on the queue of the node
+    // it forks from, and with the side chain captured first the stages moved
+    // to a second queue (about 10 us more before the first filter and 4 us
+    // more at the join, per step)
     int e = side_fork(c, 0);
     if (e != LES_OK) return e;
-    hipLaunchKernelGGL(k_duct_eddy_update, dim3((d->nvort_max + 255) / 256, S), dim3(256), 0, side, d->P);
-    hipLaunchKernelGGL(k_duct_sem_plane, dim3((d->nz_max + 63) / 64 * d->P.sem_split, d->ny_max, S), dim3(64), 0,
-                       side, d->P);
-    hipLaunchKernelGGL(k_duct_sem_norm, dim3(3, S), dim3(1024), 0, side, d->P);
-    DLAUNCH_CHECK();
     for (int s = 0; s < 3; ++s) {
         e = stage_launch<real, TK>(c, s, nullptr);
         if (e != LES_OK) return e;
+        if (s != 0) continue;
+        hipLaunchKernelGGL(k_duct_eddy_update, dim3((d->nvort_max + 255) / 256, S), dim3(256), 0, side, d->P);
+        hipLaunchKernelGGL(k_duct_sem_plane, dim3((d->nz_max + 63) / 64 * d->P.sem_split, d->ny_max, S), dim3(64),
+                           0, side, d->P);
+        hipLaunchKernelGGL(k_duct_sem_norm, dim3(3, S), dim3(1024), 0, side, d->P);
+        DLAUNCH_CHECK();
     }
     if (d->max_proj_iters > 0) {
         hipLaunchKernelGGL((k_duct_proj_div<real, TK>), dim3(d->grid), dim3(256), 0, c->stream, d->P, d->xcd_map);
@@ -572,20 +581,33 @@ static int pool_step_launches(les_gpu* c, bool reinit)
 static int pool_steps(les_gpu* c, int n)
 {
     DuctState* d = c->d;
-    for (int a = 0; a < n; ++a) {
-        bool reinit = false;
+    // the reinitialisation is due in the step m steps from now for some
+    // unfinished run
+    auto reinit_due = [&](int m) {
         for (int s = 0; s < d->max_slots; ++s) {
             const SlotDesc& D = d->desc[s];
-            if (!D.active || (D.n_steps > 0 && d->host_step[s] >= D.n_steps)) continue;
-            if (D.k.reinit_interval > 0 && (d->host_step[s] + 1) % D.k.reinit_interval == 0) reinit = true;
+            if (!D.active || (D.n_steps > 0 && d->host_step[s] + m >= D.n_steps)) continue;
+            if (D.k.reinit_interval > 0 && (d->host_step[s] + m + 1) % D.k.reinit_interval == 0) return true;
         }
-        const int gi = reinit ? 1 : 0;
+        return false;
+    };
+    for (int a = 0; a < n;) {
+        const bool reinit = reinit_due(0);
+        // runs of DUCT_GRAPH_STEPS plain steps replay one graph of that many
+        // steps (one graph launch, and its start-up gap, instead of several)
+        int gi = reinit ? 1 : 0, steps = 1;
+        if (!reinit && n - a >= DUCT_GRAPH_STEPS) {
+            bool plain = true;
+            for (int m = 1; m < DUCT_GRAPH_STEPS && plain; ++m) plain = !reinit_due(m);
+            if (plain) { gi = 2; steps = DUCT_GRAPH_STEPS; }
+        }
         bool done = false;
         if (d->use_graph && !c->timing) {
             if (!d->graph_ok[gi]) {
                 if (d->gexec[gi]) { (void)hipGraphExecDestroy(d->gexec[gi]); d->gexec[gi] = nullptr; }
                 if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-                    int e = pool_step_launches(c, reinit);
+                    int e = LES_OK;
+                    for (int m = 0; m < steps && e == LES_OK; ++m) e = pool_step_launches(c, reinit);
                     hipGraph_t gr = nullptr;
                     hipError_t ce = hipStreamEndCapture(c->stream, &gr);
                     if (e == LES_OK && ce == hipSuccess && gr &&
@@ -602,14 +624,18 @@ static int pool_steps(les_gpu* c, int n)
             }
         }
         if (!done) {
-            int e = pool_step_launches(c, reinit);
-            if (e != LES_OK) return e;
+            for (int m = 0; m < steps; ++m) {
+                int e = pool_step_launches(c, reinit);
+                if (e != LES_OK) return e;
+            }
         }
-        for (int s = 0; s < d->max_slots; ++s) {
-            const SlotDesc& D = d->desc[s];
-            if (!D.active || (D.n_steps > 0 && d->host_step[s] >= D.n_steps)) continue;
-            d->host_step[s]++;
-        }
+        for (int m = 0; m < steps; ++m)
+            for (int s = 0; s < d->max_slots; ++s) {
+                const SlotDesc& D = d->desc[s];
+                if (!D.active || (D.n_steps > 0 && d->host_step[s] >= D.n_steps)) continue;
+                d->host_step[s]++;
+            }
+        a += steps;
     }
     return LES_OK;
 }
@@ -865,7 +891,7 @@ static int pool_autotune(les_gpu* c, const les_params& probe, bool tune_tile, bo
     d->zigzag = bz;
     pool_set_grid(d);
     d->tuned_ms = best / 5.0;
-    d->graph_ok[0] = d->graph_ok[1] = false;
+    d->graph_ok[0] = d->graph_ok[1] = d->graph_ok[2] = false;
     return LES_OK;
 }
 
