@@ -525,9 +525,9 @@ static int step_launches(les_gpu* c, bool reinit)
         if (e != LES_OK) return e;
         if (s != 0) continue;
         hipLaunchKernelGGL(k_duct_eddy_update, dim3((d->nvort_max + 255) / 256, S), dim3(256), 0, side, d->P);
-        hipLaunchKernelGGL(k_duct_sem_plane, dim3((d->nz_max + 63) / 64 * d->P.sem_split, d->ny_max, S), dim3(64),
-                           0, side, d->P);
-        hipLaunchKernelGGL(k_duct_sem_norm, dim3(3, S), dim3(1024), 0, side, d->P);
+        hipLaunchKernelGGL(k_duct_sem_plane, dim3((d->ny_max * d->nz_max + 63) / 64 * d->P.sem_split, S),
+                           dim3(64), 0, side, d->P);
+        hipLaunchKernelGGL(k_duct_sem_norm, dim3(3, S), dim3(256), 0, side, d->P);
         DLAUNCH_CHECK();
     }
     if (d->max_proj_iters > 0) {

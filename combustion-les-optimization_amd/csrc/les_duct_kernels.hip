@@ -1358,23 +1358,24 @@ __global__ void k_duct_eddy_update(Pool P)
     }
 }
 
-// One thread per inflow cell and eddy chunk: block (k-chunk of 64 x eddy
-// chunk, j, run) sums the Gaussian contributions of one chunk of SEM_CHUNK eddies
-// (staged through LDS) into an FP64 partial plane; k_duct_sem_norm adds the
-// chunks in order.
+// One thread per inflow cell and eddy chunk: block (64 consecutive cells of
+// the plane x eddy chunk, run) sums the Gaussian contributions of one chunk of
+// SEM_CHUNK eddies (staged through LDS) into an FP64 partial plane;
+// k_duct_sem_norm adds the chunks in order.  The cells are taken in plane
+// order p = j nz + k, so no lane idles where nz is not a multiple of 64.
 #define SEM_CHUNK 64
 __global__ void __launch_bounds__(64) k_duct_sem_plane(Pool P)
 {
     __shared__ double se[SEM_CHUNK * 7];
-    const int nkc = (P.nz_max + 63) / 64;
-    const int s = blockIdx.z, sp = (int)blockIdx.x / nkc;
+    const int npc = (P.ny_max * P.nz_max + 63) / 64;
+    const int s = blockIdx.y, sp = (int)blockIdx.x / npc, pc = (int)blockIdx.x - sp * npc;
     const SlotDesc& D = P.desc[s];
     if (!D.active || P.dyn[s].done) return;
     const GridDims& g = D.g;
     const DuctConsts& k = D.k;
-    const int j = blockIdx.y;
-    if (j >= g.ny) return;
-    const int kk = ((int)blockIdx.x - sp * nkc) * 64 + threadIdx.x;
+    const int PL = g.ny * g.nz;
+    if (pc * 64 >= PL) return;
+    const int p = pc * 64 + (int)threadIdx.x;
     const long long PLM = (long long)P.ny_max * P.nz_max;
     double* out = P.sem_part + ((long long)s * P.sem_split + sp) * 3 * PLM;
     const int e0 = sp * SEM_CHUNK;
@@ -1384,7 +1385,8 @@ __global__ void __launch_bounds__(64) k_duct_sem_plane(Pool P)
         const double* eddy = P.eddy + 7 * (D.eoff + e0);
         for (int u = threadIdx.x; u < cnt * 7; u += blockDim.x) se[u] = eddy[u];
         __syncthreads();
-        if (kk < g.nz) {
+        if (p < PL) {
+            const int j = p / g.nz, kk = p - j * g.nz;
             const double y = j * g.dy, z = kk * g.dz, inv_l2 = 1.0 / (k.L * k.L);
             for (int e = 0; e < cnt; ++e) {
                 const double* E = se + 7 * e;
@@ -1393,22 +1395,22 @@ __global__ void __launch_bounds__(64) k_duct_sem_plane(Pool P)
                 ddy -= k.ly * rint(ddy * k.rc[DUCT_RC_ILY]);     // nearest periodic image
                 double ddz = z - E[2];
                 ddz -= k.lz * rint(ddz * k.rc[DUCT_RC_ILZ]);
-                double f = exp(-M_PI * (ddx * ddx + ddy * ddy + ddz * ddz) * inv_l2);
+                const double f = exp(-M_PI * (ddx * ddx + ddy * ddy + ddz * ddz) * inv_l2);
                 su += E[3] * f; sv += E[4] * f; sw += E[5] * f;
             }
         }
     }
-    if (kk < g.nz) {
-        long long p = (long long)j * g.nz + kk;
+    if (p < PL) {
         out[p] = su; out[PLM + p] = sv; out[2 * PLM + p] = sw;
     }
 }
 
 // chunk sums -> plane (fixed order), mean removal and rms rescale; one
-// 1024-thread block per (component, run)
-__global__ void __launch_bounds__(1024) k_duct_sem_norm(Pool P)
+// 256-thread block per (component, run); the plane sums in a fixed order
+// (strided per thread, then the block's fixed-order sum)
+__global__ void __launch_bounds__(256) k_duct_sem_norm(Pool P)
 {
-    __shared__ double red[1024];
+    __shared__ double red[4];
     const int s = blockIdx.y;
     const SlotDesc& D = P.desc[s];
     if (!D.active || P.dyn[s].done) return;
@@ -1418,35 +1420,22 @@ __global__ void __launch_bounds__(1024) k_duct_sem_norm(Pool P)
     double* f = P.inlet + D.poff + blockIdx.x * PL;
     const double* part = P.sem_part + (long long)s * P.sem_split * 3 * PLM + blockIdx.x * PLM;
     double sm = 0.0;
-    for (long long p = threadIdx.x; p < PL; p += 1024) {
+    for (long long p = threadIdx.x; p < PL; p += 256) {
         double v = 0.0;
         for (int sp = 0; sp < P.sem_split; ++sp) v += part[(long long)sp * 3 * PLM + p];
         f[p] = v;
         sm += v;
     }
-    red[threadIdx.x] = sm;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    double mean = red[0] / (double)PL;
-    __syncthreads();
+    const double mean = block_sum256(sm, red) / (double)PL;
     double qq = 0.0;
-    for (long long p = threadIdx.x; p < PL; p += 1024) {
+    for (long long p = threadIdx.x; p < PL; p += 256) {
         double v = f[p] - mean;
         f[p] = v;
         qq += v * v;
     }
-    red[threadIdx.x] = qq;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    double rms = sqrt(red[0] / (double)PL);
-    double sc = (rms > LES_SMALL) ? target / rms : 0.0;
-    for (long long p = threadIdx.x; p < PL; p += 1024) f[p] *= sc;
+    const double rms = sqrt(block_sum256(qq, red) / (double)PL);
+    const double sc = (rms > LES_SMALL) ? target / rms : 0.0;
+    for (long long p = threadIdx.x; p < PL; p += 256) f[p] *= sc;
 }
 
 // inflow plane from the SEM planes, zero-gradient outflow plane
