@@ -89,6 +89,11 @@ struct RangeAlloc {
     }
 };
 
+// groups of runs in a pool's stage launches: one per DUCT_GROUP_BYTES of a
+// field's values (12288 FP32 tiles), at most DUCT_GROUPS_MAX
+#define DUCT_GROUP_BYTES (12288LL * 256 * 4)
+#define DUCT_GROUPS_MAX 4
+
 struct DuctState {
     int precision = LES_PREC_FP64;
     size_t esize = 8;
@@ -103,6 +108,11 @@ struct DuctState {
     // launch configuration (autotuned)
     int tk = 64;
     int grid = 8;                  // tile high-water mark rounded up to a multiple of 8
+    // the stage launches in groups of runs (tile ranges split at run
+    // boundaries), each group's three stages before the next group's
+    int group_max = DUCT_GROUPS_MAX;   // LES_GROUPS=n caps the group count (1: one group)
+    int ngroups = 1;
+    int gtb[DUCT_GROUPS_MAX] = {}, gtn[DUCT_GROUPS_MAX] = {}, ggrid[DUCT_GROUPS_MAX] = {};
     int xcd_map = 1;
     int zigzag = 0;                // stencil launches alternate their sweep direction (XCD_REVERSE)
     double tuned_ms = 0.0;
@@ -220,6 +230,7 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
             return LES_ERR_HIP;
     d->precision = cfg.precision;
     d->esize = (cfg.precision == LES_PREC_FP32) ? 4 : 8;
+    if (const char* fg = getenv("LES_GROUPS")) d->group_max = std::max(1, std::min(DUCT_GROUPS_MAX, atoi(fg)));
     d->max_slots = cfg.max_slots;
     d->pool_cells = cfg.pool_cells;
     d->nx_max = cfg.max_nx; d->ny_max = cfg.max_ny; d->nz_max = cfg.max_nz;
@@ -333,6 +344,44 @@ static void pool_set_grid(DuctState* d)
     }
     const int grid = d->ntile_hwm > 0 ? (d->ntile_hwm + r - 1) / r * r : r;
     if (grid != d->grid) { d->grid = grid; d->graph_ok[0] = d->graph_ok[1] = d->graph_ok[2] = false; }
+    // groups for a pool of several runs large enough that each group's fields
+    // are a fraction of the whole pool's: one group per DUCT_GROUP_BYTES of a
+    // field (at most group_max), split at the run boundaries nearest the even
+    // split points (runs' tile ranges are disjoint, so no run straddles a
+    // boundary of another)
+    const int hwm = d->ntile_hwm;
+    std::vector<int> bnd;
+    int nact = 0;
+    for (const SlotDesc& D : d->desc) {
+        if (!D.active) continue;
+        ++nact;
+        for (int c : {D.tile0, D.tile0 + D.ntiles})
+            if (c > 0 && c < hwm) bnd.push_back(c);
+    }
+    int want = (int)std::min<long long>(d->group_max, (long long)hwm * 256 * d->esize / DUCT_GROUP_BYTES);
+    if (nact < 2) want = 1;
+    int cut[DUCT_GROUPS_MAX + 1] = {0};
+    int ng = 1;
+    for (int g = 1; g < want; ++g) {
+        const long long target = (long long)hwm * g / want;
+        int best = -1;
+        for (int c : bnd)
+            if (c > cut[ng - 1] && (best < 0 || std::llabs(c - target) < std::llabs(best - target))) best = c;
+        if (best > 0) cut[ng++] = best;
+    }
+    cut[ng] = hwm;
+    int tb[DUCT_GROUPS_MAX] = {}, tn[DUCT_GROUPS_MAX] = {}, gg[DUCT_GROUPS_MAX] = {};
+    if (ng > 1)
+        for (int g = 0; g < ng; ++g) {
+            tb[g] = cut[g]; tn[g] = cut[g + 1] - cut[g];
+            gg[g] = (tn[g] + r - 1) / r * r;
+        }
+    bool same = ng == d->ngroups;
+    for (int g = 0; g < DUCT_GROUPS_MAX; ++g)
+        same = same && tb[g] == d->gtb[g] && tn[g] == d->gtn[g] && gg[g] == d->ggrid[g];
+    if (!same) d->graph_ok[0] = d->graph_ok[1] = d->graph_ok[2] = false;
+    d->ngroups = ng;
+    for (int g = 0; g < DUCT_GROUPS_MAX; ++g) { d->gtb[g] = tb[g]; d->gtn[g] = tn[g]; d->ggrid[g] = gg[g]; }
 }
 
 // Uploads the descriptor table, tile list and tile high-water mark.
@@ -442,10 +491,13 @@ static int pool_release(les_gpu* c, int s)
 // ---- launches ----
 #define DUCT_GRAPH_STEPS 4   // steps of the multi-step captured graph
 template <typename real, int TK>
-static int stage_launch(les_gpu* c, int stage, real* const* Lout)
+static int stage_launch(les_gpu* c, int stage, real* const* Lout, int group = -1)
 {
     DuctState* d = c->d;
-    const int G = d->grid;
+    // a group's launches carry its tile range and cover only its grid
+    Pool P = d->P;
+    int G = d->grid;
+    if (group >= 0) { P.tb = d->gtb[group]; P.tn = d->gtn[group]; G = d->ggrid[group]; }
     const int qin_B = stage > 0, qout_B = stage < 2;
     // zigzag: successive launches alternate the direction in which each XCD
     // walks its tiles (three per stage, so the pattern flips from stage to
@@ -454,8 +506,8 @@ static int stage_launch(les_gpu* c, int stage, real* const* Lout)
     // small runs, not for one large run)
     const int zz = d->zigzag ? XCD_REVERSE : 0;
     const int zf = (stage & 1) ? zz : 0;
-    hipLaunchKernelGGL((k_duct_filter<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, qin_B, d->xcd_map | zf);
-    hipLaunchKernelGGL((k_duct_sgs<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, stage, d->xcd_map | (zz ^ zf));
+    hipLaunchKernelGGL((k_duct_filter<real, TK>), dim3(G), dim3(256), 0, c->stream, P, qin_B, d->xcd_map | zf);
+    hipLaunchKernelGGL((k_duct_sgs<real, TK>), dim3(G), dim3(256), 0, c->stream, P, stage, d->xcd_map | (zz ^ zf));
     DLAUNCH_CHECK();
     RhsArgs<real> a;
     a.qin_is_B = qin_B; a.qout_is_B = qout_B; a.stage = stage;
@@ -465,7 +517,7 @@ static int stage_launch(les_gpu* c, int stage, real* const* Lout)
     a.xcd_map = d->xcd_map | zf;
     for (int f = 0; f < DNF; ++f) a.Lout[f] = Lout ? Lout[f] : nullptr;
     if (c->timing) DHIPC(hipEventRecord(c->ev[0], c->stream));
-    hipLaunchKernelGGL((k_duct_rhs_all<real, TK>), dim3(G), dim3(256), 0, c->stream, d->P, a);
+    hipLaunchKernelGGL((k_duct_rhs_all<real, TK>), dim3(G), dim3(256), 0, c->stream, P, a);
     DLAUNCH_CHECK();
     if (c->timing) {
         DHIPC(hipEventRecord(c->ev[1], c->stream));
@@ -520,10 +572,10 @@ static int step_launches(les_gpu* c, bool reinit)
     // more at the join, per step)
     int e = side_fork(c, 0);
     if (e != LES_OK) return e;
-    for (int s = 0; s < 3; ++s) {
-        e = stage_launch<real, TK>(c, s, nullptr);
+    for (int gs = 0; gs < 3 * d->ngroups; ++gs) {
+        e = stage_launch<real, TK>(c, gs % 3, nullptr, d->ngroups > 1 ? gs / 3 : -1);
         if (e != LES_OK) return e;
-        if (s != 0) continue;
+        if (gs != 0) continue;
         hipLaunchKernelGGL(k_duct_eddy_update, dim3((d->nvort_max + 255) / 256, S), dim3(256), 0, side, d->P);
         hipLaunchKernelGGL(k_duct_sem_plane, dim3((d->ny_max * d->nz_max + 63) / 64 * d->P.sem_split, S),
                            dim3(64), 0, side, d->P);
@@ -1376,6 +1428,20 @@ extern "C" int les_batch_tuning(const les_batch* b, int* tile_k, int* grid_block
         if (grid_blocks) *grid_blocks = d->grid;
         if (xcd_map) *xcd_map = d->xcd_map | (d->zigzag ? XCD_REVERSE : 0);
         if (tuned_ms) *tuned_ms = d->tuned_ms;
+        return LES_OK;
+    });
+}
+
+extern "C" int les_batch_groups(const les_batch* b, int cap, int* ngroups, int* tile_base, int* tile_count)
+{
+    return abi_guard(b ? b->c : nullptr, [&]() -> int {
+        if (!b || !ngroups || cap < 0 || (cap > 0 && (!tile_base || !tile_count))) return LES_ERR_PARAM;
+        const DuctState* d = b->c->d;
+        *ngroups = d->ngroups;
+        for (int g = 0; g < std::min(cap, d->ngroups); ++g) {
+            tile_base[g] = d->ngroups > 1 ? d->gtb[g] : 0;
+            tile_count[g] = d->ngroups > 1 ? d->gtn[g] : d->ntile_hwm;
+        }
         return LES_OK;
     });
 }

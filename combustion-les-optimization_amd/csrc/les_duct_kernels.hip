@@ -160,6 +160,7 @@ struct Pool {
     SlotDyn* dyn;
     const int* tile_slot;   // tile -> slot (-1: free)
     const int* ntile;       // tile high-water mark
+    int tb, tn;             // tn > 0: the launch covers tiles [tb, tb + tn) only (a group of runs)
     double* cp_tab;
     double* mu_tab;
     double* eddy;           // 7 doubles per eddy
@@ -404,6 +405,14 @@ __host__ __device__ inline int xcd_chunk(int xcd_map) { return (xcd_map & 7) == 
 // contiguous range of the tile list, so tiles that share stencil halos
 // (neighbours in j, k and i) are read through the same L2.  The grid is the
 // tile high-water mark rounded up to whole rounds (8 blocks, or 8 chunks).
+// Tile of a launch-local tile index: the launch's group of runs (tiles
+// [tb, tb + tn)) or the whole pool (tn = 0: up to the tile high-water mark)
+__device__ __forceinline__ int tile_in_range(const Pool& P, int b)
+{
+    if (P.tn > 0) return b < P.tn ? P.tb + b : -1;
+    return b < *(const DUCT_CAS int*)P.ntile ? b : -1;
+}
+
 __device__ __forceinline__ int tile_of_block(const Pool& P, int xcd_map)
 {
     int b = blockIdx.x;
@@ -421,8 +430,7 @@ __device__ __forceinline__ int tile_of_block(const Pool& P, int xcd_map)
         const int tpp = *(const DUCT_CAS int*)&P.desc[0].tiles_j * *(const DUCT_CAS int*)&P.desc[0].tiles_k;
         if (tpp > 0 && (tpp & 7) == 0) {
             const int w = tpp >> 3, l = b >> 3;
-            const int tt = (l / w) * tpp + (b & 7) * w + l % w;
-            return tt < *(const DUCT_CAS int*)P.ntile ? tt : -1;
+            return tile_in_range(P, (l / w) * tpp + (b & 7) * w + l % w);
         }
         xcd_map = 1;
     }
@@ -435,7 +443,7 @@ __device__ __forceinline__ int tile_of_block(const Pool& P, int xcd_map)
         const int C = xcd_chunk(xcd_map), m = b >> 3;
         b = (((m / C) << 3) + (b & 7)) * C + m % C;
     }
-    return b < *(const DUCT_CAS int*)P.ntile ? b : -1;
+    return tile_in_range(P, b);
 }
 
 // Loads through the constant address space: run descriptors, the tile list

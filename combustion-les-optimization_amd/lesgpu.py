@@ -66,6 +66,7 @@ def lib():
         L.les_batch_last_error.argtypes = [vp]
         L.les_batch_last_error.restype = C.c_char_p
         L.les_batch_tuning.argtypes = [vp, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_double)]
+        L.les_batch_groups.argtypes = [vp, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int)]
         L.les_batch_admit.argtypes = [vp, P(abi.Params), C.c_int, C.c_int, P(C.c_int)]
         L.les_batch_release.argtypes = [vp, C.c_int]
         L.les_batch_step.argtypes = [vp, C.c_int, P(C.c_double)]
@@ -286,6 +287,14 @@ class Batch:
         self._check(lib().les_batch_tuning(self.h, C.byref(tk), C.byref(gb), C.byref(xm), C.byref(ms)),
                     "les_batch_tuning")
         return {"tile_k": tk.value, "grid_blocks": gb.value, "xcd_map": xm.value, "tuned_ms": ms.value}
+
+    def groups(self):
+        """(first tile, tile count) of each group of runs the stage launches
+        advance in turn (one group: the whole tile range)."""
+        cap = 8
+        n, base, cnt = C.c_int(), (C.c_int * cap)(), (C.c_int * cap)()
+        self._check(lib().les_batch_groups(self.h, cap, C.byref(n), base, cnt), "les_batch_groups")
+        return [(base[g], cnt[g]) for g in range(min(n.value, cap))]
 
     def admit(self, p, n_steps=0, init=True):
         """Slot of the admitted run, or None when it does not fit (admission refused)."""

@@ -355,6 +355,15 @@ const char* les_batch_last_error(const les_batch* b);
  * launch (one per tile of the resident runs), XCD mapping, and the tuned step
  * time on the probe grid (ms, 0 if not tuned). */
 int les_batch_tuning(const les_batch* b, int* tile_k, int* grid_blocks, int* xcd_map, double* tuned_ms);
+/* Groups of runs the stage launches advance in turn (no counterpart in the
+ * reference, which steps one run): a pool whose fields span more than
+ * 12288 x 256 FP32 values (half that in FP64) is split at run boundaries into
+ * up to four contiguous tile ranges, and each step runs the three RK stages
+ * of one range before the next's, so each group's fields are a smaller share
+ * of the caches; results are bitwise those of one group (LES_GROUPS=1 keeps
+ * one).  Writes the group count and, for the first cap groups, each one's
+ * first tile and tile count. */
+int les_batch_groups(const les_batch* b, int cap, int* ngroups, int* tile_base, int* tile_count);
 
 /* Admit a duct-mode run of n_steps steps (0: unbounded) into a free slot.
  * LES_ERR_ADMISSION when no slot is free, its cells or tiles do not fit in

@@ -214,6 +214,42 @@ def test_xcd_mapping_does_not_change_results(lesgpu, precision):
         assert np.array_equal(out[0][1], o[1])
 
 
+@pytest.mark.parametrize("precision,ngroups", [(0, 4), (1, 2)], ids=["fp64", "fp32"])
+def test_run_groups_do_not_change_results(lesgpu, precision, ngroups, monkeypatch):
+    """A pool large enough is stepped in groups of runs (contiguous tile
+    ranges split at run boundaries, each group's three RK stages in turn):
+    20 runs of 96x64x64 (7.9M cells; 4 groups in FP64, 2 in FP32) give
+    bitwise the states and profiles of the same pool stepped as one group
+    (LES_GROUPS=1), over plain steps and a reinitialisation step."""
+    print("case: run groups vs one group, precision=%d" % precision, flush=True)
+    runs = [run_params(lesgpu, precision, nx=96, ny=64, nz=64, seed=20 + i, advection=1 + (i % 3 == 2),
+                       inlet_velocity=5.0 + 0.25 * (i % 4)) for i in range(20)]
+    cells = 20 * 96 * 64 * 64
+
+    def stepped(one_group):
+        if one_group:
+            monkeypatch.setenv("LES_GROUPS", "1")
+        else:
+            monkeypatch.delenv("LES_GROUPS", raising=False)
+        with pool(lesgpu, precision, slots=20, cells=cells) as b:
+            slots = [b.admit(p, n_steps=12) for p in runs]
+            assert None not in slots
+            b.step(5)
+            grp = b.groups()
+            return grp, [(b.download(s), b.field(s, "profiles", 10 * 96)) for s in slots]
+
+    grp, split = stepped(False)
+    one, whole = stepped(True)
+    assert len(one) == 1 and len(grp) == ngroups, (one, grp)
+    assert grp[0][0] == 0 and sum(c for _, c in grp) == one[0][1]
+    for (b0, c0), (b1, _) in zip(grp, grp[1:]):
+        assert b0 + c0 == b1 and c0 > 0
+    for (fa, pa), (fb, pb) in zip(split, whole):
+        for f in range(5):
+            assert np.array_equal(fa[f], fb[f]), f
+        assert np.array_equal(pa, pb)
+
+
 def wrinkled_front_state(p, amp=3e-3):
     """Duct state with the planar front wrinkled along y (the probe lines at
     k = nz/2 cross G = 0.5 at every station)."""
