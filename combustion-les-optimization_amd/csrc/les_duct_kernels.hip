@@ -1413,12 +1413,30 @@ __global__ void __launch_bounds__(64) k_duct_sem_plane(Pool P)
     }
 }
 
-// chunk sums -> plane (fixed order), mean removal and rms rescale; one
-// 256-thread block per (component, run); the plane sums in a fixed order
-// (strided per thread, then the block's fixed-order sum)
-__global__ void __launch_bounds__(256) k_duct_sem_norm(Pool P)
+// fixed-order sum over a block of NT threads: butterfly within each wave,
+// then the wave sums in order
+template <int NT>
+__device__ __forceinline__ double block_sum_nt(double v, double* red)
 {
-    __shared__ double red[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = red[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) r += red[w];
+    __syncthreads();
+    return r;
+}
+
+// chunk sums -> plane (fixed order), mean removal and rms rescale; one block
+// of NT threads per (component, run) (1024 for large planes, 256 where a pool
+// of many small runs launches many of them beside the stages); the plane sums
+// in a fixed order (strided per thread, then the block's fixed-order sum)
+template <int NT>
+__global__ void __launch_bounds__(NT) k_duct_sem_norm(Pool P)
+{
+    __shared__ double red[NT / 64];
     const int s = blockIdx.y;
     const SlotDesc& D = P.desc[s];
     if (!D.active || P.dyn[s].done) return;
@@ -1428,22 +1446,22 @@ __global__ void __launch_bounds__(256) k_duct_sem_norm(Pool P)
     double* f = P.inlet + D.poff + blockIdx.x * PL;
     const double* part = P.sem_part + (long long)s * P.sem_split * 3 * PLM + blockIdx.x * PLM;
     double sm = 0.0;
-    for (long long p = threadIdx.x; p < PL; p += 256) {
+    for (long long p = threadIdx.x; p < PL; p += NT) {
         double v = 0.0;
         for (int sp = 0; sp < P.sem_split; ++sp) v += part[(long long)sp * 3 * PLM + p];
         f[p] = v;
         sm += v;
     }
-    const double mean = block_sum256(sm, red) / (double)PL;
+    const double mean = block_sum_nt<NT>(sm, red) / (double)PL;
     double qq = 0.0;
-    for (long long p = threadIdx.x; p < PL; p += 256) {
+    for (long long p = threadIdx.x; p < PL; p += NT) {
         double v = f[p] - mean;
         f[p] = v;
         qq += v * v;
     }
-    const double rms = sqrt(block_sum256(qq, red) / (double)PL);
+    const double rms = sqrt(block_sum_nt<NT>(qq, red) / (double)PL);
     const double sc = (rms > LES_SMALL) ? target / rms : 0.0;
-    for (long long p = threadIdx.x; p < PL; p += 256) f[p] *= sc;
+    for (long long p = threadIdx.x; p < PL; p += NT) f[p] *= sc;
 }
 
 // inflow plane from the SEM planes, zero-gradient outflow plane
