@@ -134,6 +134,7 @@ struct DuctState {
     double* stage64 = nullptr;
     double* mon_part = nullptr;
     int max_reinit_iters = 0;
+    int reinit_odd = 0;            // some active run reinitialises with an odd sweep count
     int max_proj_iters = 0;
     // [0] plain step, [1] step with reinitialisation, [2] DUCT_GRAPH_STEPS plain steps
     hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
@@ -397,13 +398,15 @@ static int pool_sync_tables(les_gpu* c)
     DHIPC(hipMemcpyAsync(d->d_tile_slot, d->tile_slot.data(), 4 * (size_t)d->tile_cap, hipMemcpyHostToDevice, c->stream));
     DHIPC(hipMemcpyAsync(d->d_ntile, &d->ntile_hwm, 4, hipMemcpyHostToDevice, c->stream));
     DHIPC(hipStreamSynchronize(c->stream));
-    int maxit = 0, maxpj = 0;
+    int maxit = 0, maxpj = 0, odd = 0;
     for (auto& x : d->desc) {
         if (x.active && x.k.reinit_interval > 0 && x.k.reinit_iters > maxit) maxit = x.k.reinit_iters;
+        if (x.active && x.k.reinit_interval > 0 && (x.k.reinit_iters & 1)) odd = 1;
         if (x.active && x.k.proj_iters > maxpj) maxpj = x.k.proj_iters;
     }
-    if (maxit != d->max_reinit_iters || maxpj != d->max_proj_iters)
+    if (maxit != d->max_reinit_iters || maxpj != d->max_proj_iters || odd != d->reinit_odd)
         d->graph_ok[0] = d->graph_ok[1] = d->graph_ok[2] = false;
+    d->reinit_odd = odd;
     d->max_reinit_iters = maxit;
     d->max_proj_iters = maxpj;
     return LES_OK;
@@ -598,13 +601,14 @@ static int step_launches(les_gpu* c, bool reinit)
     hipLaunchKernelGGL((k_duct_bc<real>), dim3(d->nzb, d->ny_max, S), dim3(256), 0, c->stream, d->P);
     DLAUNCH_CHECK();
     if (reinit) {
-        hipLaunchKernelGGL((k_duct_reinit<real, TK>), dim3(d->grid), dim3(256), 0, c->stream, d->P, 0, 0, d->xcd_map);
-        for (int it = 0; it < d->max_reinit_iters; ++it) {
+        // one launch per sweep (ping-pong), and the move back into phi where
+        // some run has an odd sweep count
+        for (int it = 0; it < d->max_reinit_iters; ++it)
             hipLaunchKernelGGL((k_duct_reinit<real, TK>), dim3(d->grid), dim3(256), 0, c->stream, d->P, 1, it,
                                d->xcd_map);
-            hipLaunchKernelGGL((k_duct_reinit<real, TK>), dim3(d->grid), dim3(256), 0, c->stream, d->P, 2, it,
+        if (d->reinit_odd)
+            hipLaunchKernelGGL((k_duct_reinit<real, TK>), dim3(d->grid), dim3(256), 0, c->stream, d->P, 2, 0,
                                d->xcd_map);
-        }
         DLAUNCH_CHECK();
     }
     // outputs of the new state (statistics chunks, probe rows, point samples)

@@ -1512,6 +1512,12 @@ __device__ __forceinline__ bool reinit_due(const SlotView& D, int step, int iter
     return D.k.reinit_interval > 0 && (step + 1) % D.k.reinit_interval == 0 && iter < D.k.reinit_iters;
 }
 
+// Level-set reinitialisation sweeps, ping-ponged between phi (A[3]) and a
+// stage buffer (B[4]): sweep `iter` reads the even sweeps' input from phi and
+// the odd ones' from B[4] and writes the other (the first sweep also keeps
+// phi0 = phi in B[3]), so each sweep is one launch; phase 2 moves the result of
+// a run with an odd sweep count back into phi.  The same arithmetic as a sweep
+// into a temporary followed by a copy, one pass over the field fewer.
 template <typename real, int TK>
 __global__ void __launch_bounds__(256) k_duct_reinit(Pool P, int phase, int iter, int xcd_map)
 {
@@ -1520,31 +1526,32 @@ __global__ void __launch_bounds__(256) k_duct_reinit(Pool P, int phase, int iter
     if (t < 0 || !tile_decode<TK>(P, t, tp)) return;
     {
         const SlotView D = slot_view(P, tp.s);
-        if (!reinit_due(D, ldc(&P.dyn[tp.s].step), iter) || !tp.in) return;
+        const int step = ldc(&P.dyn[tp.s].step);
+        if (!tp.in) return;
         const GridDims g = D.g;
         const int i = tp.i, j = tp.j, kk = tp.kk;
         const int q = (int)lidx(g, i, j, kk);
         const auto phi = run_arr((real*)P.A[3] + D.off);
         const auto phi0 = run_arr((real*)P.B[3] + D.off);
         const auto tmp = run_arr((real*)P.B[4] + D.off);
-        if (phase == 0) {             // save phi0 (first iteration only)
-            if (iter == 0) phi0[q] = phi[q];
+        if (phase == 2) {             // odd sweep count: the result is in B[4]
+            if (reinit_due(D, step, 0) && (D.k.reinit_iters & 1)) phi[q] = tmp[q];
             return;
         }
-        if (phase == 2) {             // copy back
-            phi[q] = tmp[q];
-            return;
-        }
-        if (i == 0 || i == g.nx - 1) { tmp[q] = phi[q]; return; }
+        if (!reinit_due(D, step, iter)) return;
+        const auto src = (iter & 1) ? tmp : phi;
+        const auto dst = (iter & 1) ? phi : tmp;
+        const real c = src[q];
+        if (iter == 0) phi0[q] = c;
+        if (i == 0 || i == g.nx - 1) { dst[q] = c; return; }
         const real dx = (real)g.dx, dtau = (real)0.3 * dx;
         const real idx = rcp_const<real>(D.k, DUCT_RC_ID), idy = rcp_const<real>(D.k, DUCT_RC_ID + 1),
                    idz = rcp_const<real>(D.k, DUCT_RC_ID + 2);
-        real p0 = phi0[q];
+        const real p0 = iter == 0 ? c : phi0[q];
         real sgn = M<real>::div(p0, sqrt(p0 * p0 + dx * dx));
-        real c = phi[q];
-        real ax = (c - phi[q - (int)g.sx]) * idx, bx = (phi[q + (int)g.sx] - c) * idx;
-        real ay = (c - phi[(int)lidx(g, i, wrapi(j - 1, g.ny), kk)]) * idy, by = (phi[(int)lidx(g, i, wrapi(j + 1, g.ny), kk)] - c) * idy;
-        real az = (c - phi[(int)lidx(g, i, j, wrapi(kk - 1, g.nz))]) * idz, bz = (phi[(int)lidx(g, i, j, wrapi(kk + 1, g.nz))] - c) * idz;
+        real ax = (c - src[q - (int)g.sx]) * idx, bx = (src[q + (int)g.sx] - c) * idx;
+        real ay = (c - src[(int)lidx(g, i, wrapi(j - 1, g.ny), kk)]) * idy, by = (src[(int)lidx(g, i, wrapi(j + 1, g.ny), kk)] - c) * idy;
+        real az = (c - src[(int)lidx(g, i, j, wrapi(kk - 1, g.nz))]) * idz, bz = (src[(int)lidx(g, i, j, wrapi(kk + 1, g.nz))] - c) * idz;
         real gx, gy, gz;
         if (p0 > (real)0.0) {
             real a1 = fmax(ax, (real)0), b1 = fmin(bx, (real)0), a2 = fmax(ay, (real)0), b2 = fmin(by, (real)0);
@@ -1556,7 +1563,7 @@ __global__ void __launch_bounds__(256) k_duct_reinit(Pool P, int phase, int iter
             gx = fmax(a1 * a1, b1 * b1); gy = fmax(a2 * a2, b2 * b2); gz = fmax(a3 * a3, b3 * b3);
         }
         real gm = sqrt(gx + gy + gz);
-        tmp[q] = c - dtau * sgn * (gm - (real)1.0);
+        dst[q] = c - dtau * sgn * (gm - (real)1.0);
     }
 }
 

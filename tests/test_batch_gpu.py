@@ -51,7 +51,8 @@ def test_member_in_mix_equals_alone(lesgpu, precision):
     with pool(lesgpu, precision) as b:
         others = [
             run_params(lesgpu, precision, nx=48, ny=40, nz=24, seed=3, inlet_velocity=4.0),
-            run_params(lesgpu, precision, nx=96, ny=48, nz=48, seed=11, advection=2, reinit_interval=3),
+            run_params(lesgpu, precision, nx=96, ny=48, nz=48, seed=11, advection=2, reinit_interval=3,
+                       reinit_iters=3),   # odd sweep count beside the even ones
         ]
         s1 = b.admit(others[0], n_steps=4)          # finishes first, then sits done
         sa = b.admit(pa, n_steps=9)
