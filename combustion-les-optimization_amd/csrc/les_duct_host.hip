@@ -580,8 +580,10 @@ static int step_launches(les_gpu* c, bool reinit)
         if (e != LES_OK) return e;
         if (gs != 0) continue;
         hipLaunchKernelGGL(k_duct_eddy_update, dim3((d->nvort_max + 255) / 256, S), dim3(256), 0, side, d->P);
-        hipLaunchKernelGGL(k_duct_sem_plane, dim3((d->ny_max * d->nz_max + 63) / 64 * d->P.sem_split, S),
-                           dim3(64), 0, side, d->P);
+        hipLaunchKernelGGL(k_duct_sem_plane,
+                           dim3((d->ny_max + SEM_PATCH - 1) / SEM_PATCH * ((d->nz_max + SEM_PATCH - 1) / SEM_PATCH) *
+                                    d->P.sem_split, S),
+                           dim3(256), 0, side, d->P);
         if ((long long)d->ny_max * d->nz_max >= 8192)
             hipLaunchKernelGGL(k_duct_sem_norm<1024>, dim3(3, S), dim3(1024), 0, side, d->P);
         else

@@ -1366,49 +1366,64 @@ __global__ void k_duct_eddy_update(Pool P)
     }
 }
 
-// One thread per inflow cell and eddy chunk: block (64 consecutive cells of
-// the plane x eddy chunk, run) sums the Gaussian contributions of one chunk of
-// SEM_CHUNK eddies (staged through LDS) into an FP64 partial plane;
-// k_duct_sem_norm adds the chunks in order.  The cells are taken in plane
-// order p = j nz + k, so no lane idles where nz is not a multiple of 64.
+// One block per 16 x 16 patch of the inflow plane (rows j, columns k), eddy
+// chunk of SEM_CHUNK eddies and run, into an FP64 partial plane per chunk
+// (k_duct_sem_norm adds the chunks in order).  An eddy's Gaussian
+// exp(-pi |d|^2 / L^2) is the product of its x, y and z factors, so the block
+// takes the chunk's y factors for its 16 rows and z factors for its 16 columns
+// (32 exponentials per eddy instead of 256, staged in LDS with each eddy's
+// amplitudes times its x factor), and each cell adds
+// (amplitude x x factor) x (y factor x z factor) over the chunk's eddies.
 #define SEM_CHUNK 64
-__global__ void __launch_bounds__(64) k_duct_sem_plane(Pool P)
+#define SEM_PATCH 16
+__global__ void __launch_bounds__(256) k_duct_sem_plane(Pool P)
 {
-    __shared__ double se[SEM_CHUNK * 7];
-    const int npc = (P.ny_max * P.nz_max + 63) / 64;
+    __shared__ double fy[SEM_CHUNK][SEM_PATCH], fz[SEM_CHUNK][SEM_PATCH], amp[SEM_CHUNK][3];
+    const int pk = (P.nz_max + SEM_PATCH - 1) / SEM_PATCH;
+    const int npc = (P.ny_max + SEM_PATCH - 1) / SEM_PATCH * pk;
     const int s = blockIdx.y, sp = (int)blockIdx.x / npc, pc = (int)blockIdx.x - sp * npc;
     const SlotDesc& D = P.desc[s];
     if (!D.active || P.dyn[s].done) return;
     const GridDims& g = D.g;
     const DuctConsts& k = D.k;
-    const int PL = g.ny * g.nz;
-    if (pc * 64 >= PL) return;
-    const int p = pc * 64 + (int)threadIdx.x;
+    const int J0 = (pc / pk) * SEM_PATCH, K0 = (pc % pk) * SEM_PATCH;
+    if (J0 >= g.ny || K0 >= g.nz) return;
     const long long PLM = (long long)P.ny_max * P.nz_max;
     double* out = P.sem_part + ((long long)s * P.sem_split + sp) * 3 * PLM;
     const int e0 = sp * SEM_CHUNK;
     const int cnt = min(SEM_CHUNK, k.nvort - e0);
+    const int jj = (int)threadIdx.x / SEM_PATCH, kk = (int)threadIdx.x % SEM_PATCH;
     double su = 0.0, sv = 0.0, sw = 0.0;
     if (cnt > 0) {
         const double* eddy = P.eddy + 7 * (D.eoff + e0);
-        for (int u = threadIdx.x; u < cnt * 7; u += blockDim.x) se[u] = eddy[u];
-        __syncthreads();
-        if (p < PL) {
-            const int j = p / g.nz, kk = p - j * g.nz;
-            const double y = j * g.dy, z = kk * g.dz, inv_l2 = 1.0 / (k.L * k.L);
-            for (int e = 0; e < cnt; ++e) {
-                const double* E = se + 7 * e;
-                double ddx = 0.0 - E[0];
-                double ddy = y - E[1];
+        const double inv_l2 = 1.0 / (k.L * k.L);
+        for (int u = threadIdx.x; u < cnt * 2 * SEM_PATCH; u += blockDim.x) {
+            const int e = u / (2 * SEM_PATCH), r = u % (2 * SEM_PATCH);
+            const double* E = eddy + 7 * e;
+            if (r < SEM_PATCH) {
+                double ddy = (J0 + r) * g.dy - E[1];
                 ddy -= k.ly * rint(ddy * k.rc[DUCT_RC_ILY]);     // nearest periodic image
-                double ddz = z - E[2];
+                fy[e][r] = exp(-M_PI * (ddy * ddy) * inv_l2);
+            } else {
+                double ddz = (K0 + r - SEM_PATCH) * g.dz - E[2];
                 ddz -= k.lz * rint(ddz * k.rc[DUCT_RC_ILZ]);
-                const double f = exp(-M_PI * (ddx * ddx + ddy * ddy + ddz * ddz) * inv_l2);
-                su += E[3] * f; sv += E[4] * f; sw += E[5] * f;
+                fz[e][r - SEM_PATCH] = exp(-M_PI * (ddz * ddz) * inv_l2);
             }
         }
+        for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
+            const double* E = eddy + 7 * e;
+            const double fx = exp(-M_PI * (E[0] * E[0]) * inv_l2);
+            amp[e][0] = E[3] * fx; amp[e][1] = E[4] * fx; amp[e][2] = E[5] * fx;
+        }
+        __syncthreads();
+        for (int e = 0; e < cnt; ++e) {
+            const double w = fy[e][jj] * fz[e][kk];
+            su += amp[e][0] * w; sv += amp[e][1] * w; sw += amp[e][2] * w;
+        }
     }
-    if (p < PL) {
+    const int j = J0 + jj, kc = K0 + kk;
+    if (j < g.ny && kc < g.nz) {
+        const long long p = (long long)j * g.nz + kc;
         out[p] = su; out[PLM + p] = sv; out[2 * PLM + p] = sw;
     }
 }
