@@ -251,6 +251,34 @@ def test_run_groups_do_not_change_results(lesgpu, precision, ngroups, monkeypatc
         assert np.array_equal(pa, pb)
 
 
+def test_groups_with_continuous_batching(lesgpu):
+    """Runs released and admitted while the pool steps in groups of runs (the
+    resident tile ranges change, the captured steps are rebuilt): a run
+    admitted mid-way into freed ranges, and a run resident throughout, equal
+    the same runs alone, bitwise."""
+    print("case: groups with releases and admissions, FP64", flush=True)
+    runs = [run_params(lesgpu, 0, nx=96, ny=64, nz=64, seed=40 + i) for i in range(20)]
+    late = run_params(lesgpu, 0, nx=64, ny=48, nz=48, seed=77, advection=2)
+    kept_alone, _ = single_run(lesgpu, runs[7], 7)
+    late_alone, _ = single_run(lesgpu, late, 4)
+    with pool(lesgpu, 0, slots=20, cells=20 * 96 * 64 * 64) as b:
+        slots = [b.admit(p, n_steps=7) for p in runs]
+        assert None not in slots
+        b.step(3)
+        before = b.groups()
+        for s in (slots[2], slots[3], slots[12]):
+            b.release(s)
+        sl = b.admit(late, n_steps=4)
+        assert sl is not None
+        b.step(4)
+        after = b.groups()
+        assert len(before) == 4 and len(after) >= 2, (before, after)
+        for x, y in zip(b.download(slots[7]), kept_alone):
+            assert np.array_equal(x, y)
+        for x, y in zip(b.download(sl), late_alone):
+            assert np.array_equal(x, y)
+
+
 def wrinkled_front_state(p, amp=3e-3):
     """Duct state with the planar front wrinkled along y (the probe lines at
     k = nz/2 cross G = 0.5 at every station)."""
