@@ -265,17 +265,16 @@ static int pool_create(les_gpu* c, const les_batch_config& cfg)
     if (d->stats_cells)
         for (int a = 0; a < 10; ++a)
             if (!(P.stat[a] = (double*)dalloc_bytes(c, 8 * (size_t)d->pool_cells))) return LES_ERR_HIP;
+    // the viscosity table (the kernels need no cp: alpha = lambda / (rho cp)
+    // with lambda = mu cp / Pr is mu / (rho Pr))
     const int tn = cfg.table_n;
-    std::vector<double> cp(tn), mu(tn);
+    std::vector<double> mu(tn);
     for (int t = 0; t < tn; ++t) {
         double T = cfg.table_t0 + t * cfg.table_dt;
-        cp[t] = 1000.0 + 0.2 * (T - 300.0);
         mu[t] = 1.8e-5 * pow(T / 300.0, 0.7);
     }
-    P.cp_tab = (double*)dalloc_bytes(c, 8 * (size_t)tn);
     P.mu_tab = (double*)dalloc_bytes(c, 8 * (size_t)tn);
-    if (!P.cp_tab || !P.mu_tab) return LES_ERR_HIP;
-    DHIPC(hipMemcpy(P.cp_tab, cp.data(), 8 * (size_t)tn, hipMemcpyHostToDevice));
+    if (!P.mu_tab) return LES_ERR_HIP;
     DHIPC(hipMemcpy(P.mu_tab, mu.data(), 8 * (size_t)tn, hipMemcpyHostToDevice));
     const size_t S = (size_t)d->max_slots;
     P.eddy = (double*)dalloc_bytes(c, 8 * 7 * S * d->nvort_max);
@@ -1376,7 +1375,7 @@ extern "C" int les_batch_pool_bytes(const les_batch_config* cfg, size_t* bytes)
         b += S * 8 * (7 * nv + 3 * (size_t)cfg->max_ny * cfg->max_nz + 10 * (size_t)cfg->max_nx +
                       4 * (size_t)DUCT_NPTS * cfg->point_cap + (size_t)cfg->max_ny * ((cfg->max_nz + 255) / 256) +
                       10 * (size_t)cfg->max_nx * STAT_CHUNKS);
-        b += S * (sizeof(SlotDesc) + sizeof(SlotDyn) + 8 + 24 * DT_BLOCKS_MAX) + 16 * (size_t)cfg->table_n + 8 * 6 * 1024;
+        b += S * (sizeof(SlotDesc) + sizeof(SlotDyn) + 8 + 24 * DT_BLOCKS_MAX) + 8 * (size_t)cfg->table_n + 8 * 6 * 1024;
         b += S * 8 * 3 * (size_t)cfg->max_ny * cfg->max_nz * ((nv + SEM_CHUNK - 1) / SEM_CHUNK);   // SEM partials
         if (cfg->probe_cap > 0)                                   // flame-region probe rings
             b += S * 8 * (size_t)cfg->probe_cap * LES_NPROBE_MAX * (2 + 6 * (size_t)cfg->max_ny);

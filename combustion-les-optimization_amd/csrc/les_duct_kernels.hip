@@ -161,8 +161,7 @@ struct Pool {
     const int* tile_slot;   // tile -> slot (-1: free)
     const int* ntile;       // tile high-water mark
     int tb, tn;             // tn > 0: the launch covers tiles [tb, tb + tn) only (a group of runs)
-    double* cp_tab;
-    double* mu_tab;
+    double* mu_tab;         // viscosity against T (tables: t0, step, n in DuctConsts)
     double* eddy;           // 7 doubles per eddy
     double* inlet;          // 3 planes per slot
     double* prof;           // 10 FP64 per x-plane per slot
@@ -1122,7 +1121,6 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
                 const real cv0 = S.cv0, cv1 = S.cv1, cv2 = S.cv2, ap = S.ap, aT = S.aT, lp = S.lp, lT = S.lT, gg = S.gg;
                 const real conv[3] = {cv0, cv1, cv2};
                 const real mu = table_lookup<real>(P.mu_tab, k, T0);
-                const real cpv = table_lookup<real>(P.cp_tab, k, T0);
                 const real mu_eff = mu + rf0 * nu0;
                 const int xp = q + (int)g.sx, xm = q - (int)g.sx;
                 const int yp = line_idx(g, bi, j, kk, 1, 1), ym = line_idx(g, bi, j, kk, 1, -1);
@@ -1158,6 +1156,13 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
                 const real grav = (rf0 - rho_avg) * (real)LES_G_GRAV;
                 const real g0 = progress<real>(yz(3, 0), rcp_const<real>(k, DUCT_RC_HID));
                 const real ctr = (real)k.accel_center, Fm = (real)k.max_force;
+                // the flame window exp(-(c - c0)^2 / (2 w^2)), shared by the flame
+                // force (0.01 < c < 0.99) and the artificial diffusion (0.1 < c < 0.99)
+                real win = (real)0;
+                if (g0 > (real)0.01 && g0 < (real)0.99) {
+                    const real df = g0 - ctr;
+                    win = M<real>::exp(-(df * df) * rcp_const<real>(k, DUCT_RC_IW2));
+                }
                 real fx = (real)0, fy = (real)0, fz = (real)0;
                 if (g0 > (real)0.01 && g0 < (real)0.99) {
                     real gx = (real)0, gy = (real)0, gz = (real)0;
@@ -1170,8 +1175,6 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
                         gy = d == 1 ? gd : gy;
                         gz = d == 2 ? gd : gz;
                     }
-                    const real df = g0 - ctr;
-                    const real win = M<real>::exp(-(df * df) * rcp_const<real>(k, DUCT_RC_IW2));
                     const real common = (real)k.accel_strength * rf0 * win;
                     fx = common * gx * (real)k.weight_x;
                     fy = common * gy;
@@ -1186,16 +1189,12 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
                 L[2] = -conv[2] + df2 + fz;
                 // phi: S_T |grad phi|, SGS flux, artificial diffusion in the flame band
                 real art = (real)0;
-                if (g0 > (real)0.1 && g0 < (real)0.99) {
-                    const real df = g0 - ctr;
-                    const real win = M<real>::exp(-(df * df) * rcp_const<real>(k, DUCT_RC_IW2));
-                    art = (real)5.0 * (nu0 + (real)1.0e-5) * win * lp;
-                }
+                if (g0 > (real)0.1 && g0 < (real)0.99) art = (real)5.0 * (nu0 + (real)1.0e-5) * win * lp;
                 const auto stv = run_buf((const real*)P.st + D.off, g.n);
                 L[3] = ap + stv[q] * sqrt(gg) + art;
-                // T: conduction with alpha = mu cp / Pr / (rho cp), SGS flux, nudging
-                const real lam = mu * cpv * rcp_const<real>(k, DUCT_RC_IPRAN);
-                const real al = M<real>::div(lam, rf0 * cpv);
+                // T: conduction with alpha = lambda / (rho cp), lambda = mu cp / Pr, so
+                // alpha = mu T / (rho_u T_u Pr) (no cp table, no division), SGS flux, nudging
+                const real al = mu * T0 * rcp_const<real>(k, DUCT_RC_IRUT) * rcp_const<real>(k, DUCT_RC_IPRAN);
                 const real Tu = (real)k.Tu, Tb = (real)k.Tb;
                 const real Ttg = Tu + g0 * (Tb - Tu);
                 L[4] = aT + al * lT + (real)k.relaxation * (Ttg - T0);
