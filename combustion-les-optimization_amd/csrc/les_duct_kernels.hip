@@ -896,8 +896,12 @@ __device__ __forceinline__ void rhs_line(RhsSums<real>& A, const LD& ld, int vd,
     const real u0 = ld(vd, 3);
     const real up = (real)0.5 * (ld(vd, 4) + u0);
     const real um = (real)0.5 * (u0 + ld(vd, 2));
-    const real rp = (real)0.5 * (M<real>::div(rut, ld(4, 4)) + rf0);
-    const real rm = (real)0.5 * (rf0 + M<real>::div(rut, ld(4, 2)));
+    // face densities from the neighbours' rho = rho_u T_u / T, the two
+    // neighbours over one division: rho_u T_u / (T+ T-) times T- and T+
+    const real Tp = ld(4, 4), Tm = ld(4, 2);
+    const real rq = M<real>::div(rut, Tp * Tm);
+    const real rp = (real)0.5 * (rq * Tm + rf0);
+    const real rm = (real)0.5 * (rf0 + rq * Tp);
     const real mp = rp * up, mm = rm * um;
     real ntp = (real)0, ntm = (real)0;     // nu_t at +-1, loaded with the phi line
     real nxt[7];
