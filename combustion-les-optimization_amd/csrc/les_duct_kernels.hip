@@ -388,6 +388,33 @@ __device__ __forceinline__ real face_diff_same(int scheme, const real* v, bool p
     return pos ? d : -d;
 }
 
+// FP64 WENO5 only: the quotients of face_pair and face_diff_same as numerator
+// and denominator, so that two of them can share one division
+__device__ __forceinline__ void face_pair_nd(const double* v, bool pos_p, bool pos_m, double p, double m, double& n,
+                                             double& d)
+{
+    double np, dp, nm, dm;
+    weno5_nd<double>(pos_p ? v[1] : v[6], pos_p ? v[2] : v[5], pos_p ? v[3] : v[4], pos_p ? v[4] : v[3],
+                     pos_p ? v[5] : v[2], np, dp);
+    weno5_nd<double>(pos_m ? v[0] : v[5], pos_m ? v[1] : v[4], pos_m ? v[2] : v[3], pos_m ? v[3] : v[2],
+                     pos_m ? v[4] : v[1], nm, dm);
+    n = p * np * dm - m * nm * dp;
+    d = dp * dm;
+}
+
+__device__ __forceinline__ void face_diff_same_nd(const double* v, bool pos, double& n, double& d)
+{
+    double r[7];
+#pragma unroll
+    for (int o = 0; o < 7; ++o) r[o] = (o == 3 || pos) ? v[o] : v[6 - o];
+    double na, da, nb, db;
+    weno5_nd<double>(r[1], r[2], r[3], r[4], r[5], na, da);
+    weno5_nd<double>(r[0], r[1], r[2], r[3], r[4], nb, db);
+    const double nn = na * db - nb * da;
+    n = pos ? nn : -nn;
+    d = da * db;
+}
+
 
 // ---------------------------------------------------------------------------
 // tile-to-block mapping (XCD-aware) and tile decoding
@@ -888,7 +915,7 @@ struct RhsSums {
 // (0..2).  Line m + 1 is loaded while line m is reconstructed.  first: the
 // first direction (starts the sums); band: first-order faces (the x band next
 // to the inflow and outflow planes).
-template <typename real, typename LD>
+template <typename real, bool PAIR, typename LD>
 __device__ __forceinline__ void rhs_line(RhsSums<real>& A, const LD& ld, int vd, bool first, bool band, int scheme,
                                          real rut, real rf0, real nu0, real isc, real ipr, real id, real id2,
                                          real hid)
@@ -904,6 +931,13 @@ __device__ __forceinline__ void rhs_line(RhsSums<real>& A, const LD& ld, int vd,
     const real rm = (real)0.5 * (rf0 + rq * Tp);
     const real mp = rp * up, mm = rm * um;
     real ntp = (real)0, ntm = (real)0;     // nu_t at +-1, loaded with the phi line
+    // FP64 WENO5 away from the end-plane band (PAIR: the LDS-staged kernels,
+    // whose registers leave room for the held pair): the quotients of fields 0 and 1,
+    // and of fields 2 and 3, share one division each (1/(a b) times b and a;
+    // the denominators are products of the smoothness weights, far inside
+    // the FP64 range even in pairs); (pn, pd) holds the first of a pair
+    const bool pairdiv = PAIR && sizeof(real) == 8 && scheme == LES_ADV_WENO5 && !band;
+    real pn = (real)0, pd = (real)1;
     real nxt[7];
 #pragma unroll
     for (int o = 0; o < 7; ++o) nxt[o] = ld(0, o);
@@ -919,12 +953,30 @@ __device__ __forceinline__ void rhs_line(RhsSums<real>& A, const LD& ld, int vd,
         if (m == 2) { ntp = ld(5, 4); ntm = ld(5, 2); }
         if (m < 3) {
             real term;
-            if (band) {
-                const real fp = (up >= (real)0.0) ? v[3] : v[4];
-                const real fm = (um >= (real)0.0) ? v[2] : v[3];
-                term = (mp * fp - mm * fm) * id;
-            } else {
-                term = face_pair<real>(scheme, v, up >= (real)0.0, um >= (real)0.0, mp, mm) * id;
+            if constexpr (sizeof(real) == 8) {
+                if (pairdiv) {
+                    real n, d;
+                    face_pair_nd(v, up >= (real)0.0, um >= (real)0.0, mp, mm, n, d);
+                    if (m == 1) {
+                        // fields 0 and 1 over one division
+                        const real R = M<real>::div((real)1.0, pd * d);
+                        const real t0 = pn * (d * R) * id;
+                        A.cv0 = first ? t0 : A.cv0 + t0;
+                        term = n * (pd * R) * id;
+                    } else {
+                        pn = n; pd = d;            // fields 0 and 2: completed by the next field
+                        continue;
+                    }
+                }
+            }
+            if (!pairdiv) {
+                if (band) {
+                    const real fp = (up >= (real)0.0) ? v[3] : v[4];
+                    const real fm = (um >= (real)0.0) ? v[2] : v[3];
+                    term = (mp * fp - mm * fm) * id;
+                } else {
+                    term = face_pair<real>(scheme, v, up >= (real)0.0, um >= (real)0.0, mp, mm) * id;
+                }
             }
             const real c = m == 0 ? A.cv0 : (m == 1 ? A.cv1 : A.cv2);
             const real nc = first ? term : c + term;
@@ -939,7 +991,20 @@ __device__ __forceinline__ void rhs_line(RhsSums<real>& A, const LD& ld, int vd,
                 tt = (u0 >= (real)0.0) ? u0 * (v0 - vm1) * id : u0 * (vp1 - v0) * id;
             } else {
                 const bool pos = u0 >= (real)0.0;
-                tt = u0 * face_diff_same<real>(scheme, v, pos) * id;
+                bool done = false;
+                if constexpr (sizeof(real) == 8) {
+                    if (pairdiv && m == 3) {
+                        // field 2 (held) and phi over one division
+                        real n, d;
+                        face_diff_same_nd(v, pos, n, d);
+                        const real R = M<real>::div((real)1.0, pd * d);
+                        const real t2 = pn * (d * R) * id;
+                        A.cv2 = first ? t2 : A.cv2 + t2;
+                        tt = u0 * (n * (pd * R)) * id;
+                        done = true;
+                    }
+                }
+                if (!done) tt = u0 * face_diff_same<real>(scheme, v, pos) * id;
             }
             const real isc2 = m == 3 ? isc : ipr;
             const real gp = (ntp + nu0) * isc2, gm = (nu0 + ntm) * isc2;
@@ -1071,18 +1136,18 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
                     auto ldx = [&](int f, int o) {
                         return (f == 0 ? fld[0] : (f == 1 ? fld[1] : (f == 2 ? fld[2] : (f == 3 ? fld[3] : (f == 4 ? fld[4] : fld[5]))))).at(lo[o]);
                     };
-                    rhs_line<real>(S, ldx, 0, true, xband, scheme, rut, rf0, nu0, isc, ipr,
+                    rhs_line<real, ST::on>(S, ldx, 0, true, xband, scheme, rut, rf0, nu0, isc, ipr,
                                    rcp_const<real>(k, DUCT_RC_ID), rcp_const<real>(k, DUCT_RC_ID2),
                                    rcp_const<real>(k, DUCT_RC_HID));
                     // y and z lines from LDS
 #ifdef EXP_LDS_UNROLLED
                     auto ldy = [&](int f, int o) { return sF[f][lb + (o - 3) * ST::SK]; };
-                    rhs_line<real>(S, ldy, 1, false, false, scheme, rut, rf0, nu0, isc, ipr,
+                    rhs_line<real, ST::on>(S, ldy, 1, false, false, scheme, rut, rf0, nu0, isc, ipr,
                                    rcp_const<real>(k, DUCT_RC_ID + 1), rcp_const<real>(k, DUCT_RC_ID2 + 1),
                                    rcp_const<real>(k, DUCT_RC_HID + 1));
                     __builtin_amdgcn_sched_barrier(0);
                     auto ldz = [&](int f, int o) { return sF[f][lb + (o - 3)]; };
-                    rhs_line<real>(S, ldz, 2, false, false, scheme, rut, rf0, nu0, isc, ipr,
+                    rhs_line<real, ST::on>(S, ldz, 2, false, false, scheme, rut, rf0, nu0, isc, ipr,
                                    rcp_const<real>(k, DUCT_RC_ID + 2), rcp_const<real>(k, DUCT_RC_ID2 + 2),
                                    rcp_const<real>(k, DUCT_RC_HID + 2));
 #else
@@ -1091,7 +1156,7 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
                         const int st = d == 1 ? ST::SK : 1;
                         const int l0 = lb - 3 * st;
                         auto ldl = [&](int f, int o) { return sF[f == 6 ? d : f][l0 + o * st]; };
-                        rhs_line<real>(S, ldl, 6, false, false, scheme, rut, rf0, nu0, isc, ipr,
+                        rhs_line<real, ST::on>(S, ldl, 6, false, false, scheme, rut, rf0, nu0, isc, ipr,
                                        rcp_dir<real>(k, DUCT_RC_ID, d), rcp_dir<real>(k, DUCT_RC_ID2, d),
                                        rcp_dir<real>(k, DUCT_RC_HID, d));
                     }
@@ -1117,7 +1182,7 @@ k_duct_rhs_all(Pool P, RhsArgs<real> a)
                             // f == vd (the direction's velocity) is read through U
                             return (f == 6 ? U : (f == 0 ? fld[0] : (f == 1 ? fld[1] : (f == 2 ? fld[2] : (f == 3 ? fld[3] : (f == 4 ? fld[4] : fld[5])))))).at(lo[o]);
                         };
-                        rhs_line<real>(S, ld, 6, d == 0, band, scheme, rut, rf0, nu0, isc, ipr,
+                        rhs_line<real, false>(S, ld, 6, d == 0, band, scheme, rut, rf0, nu0, isc, ipr,
                                        rcp_dir<real>(k, DUCT_RC_ID, d), rcp_dir<real>(k, DUCT_RC_ID2, d),
                                        rcp_dir<real>(k, DUCT_RC_HID, d));
                     }
